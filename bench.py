@@ -1,0 +1,152 @@
+"""Benchmark: grid-cell-updates/s of the batched FP32 forward (256 ragged tubes,
+13-19 cm, 30 ms) against the plain FP64 single-tube GPU path, on one MI355X.
+
+One bench "step" is one 256-time-step block advanced over the whole batch
+(the unit the step kernel / graph replays).  Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vocalsim-2p5d-fdtd_amd"))
+
+BYTES_FP32 = 37   # P, Vx, Vy read+write, dP, dX, dY read (4 B each), 1 B class
+BYTES_FP64 = 73
+HBM_PEAK_GBS = 8000.0
+BLOCK = 256
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed 256-step blocks")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed 256-step blocks")
+    ap.add_argument("--tubes", type=int, default=256)
+    ap.add_argument("--threads", type=int, default=0, help="0: autotuned / default")
+    ap.add_argument("--kernel", type=int, default=-1, help="-1: best available")
+    ap.add_argument("--fp64-blocks", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(tubes would shard by batch; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import vocalsim
+    from vocalsim import _abi, tubes
+
+    torch.cuda.set_device(0)
+    vocalsim.load()
+
+    # ---------------- batched FP32 (256 ragged tubes) ----------------
+    total_blocks = args.warmup + args.steps
+    nsteps = max(19845, total_blocks * BLOCK)
+    specs = tubes.batch_tubes(args.tubes, seed=0, num_steps=nsteps)
+    kernel = args.kernel if args.kernel >= 0 else _abi.K_EXACT
+    threads = args.threads or 256
+    log(f"[bench] building {args.tubes} tubes (FP32), kernel={kernel} threads={threads}")
+    s = vocalsim.Solver(specs, precision=32, threads=threads, steps_per_launch=BLOCK, kernel=kernel)
+    cells = s.cells_per_step
+    log(f"[bench] interior cells per step: {cells}")
+    s.run(args.warmup * BLOCK)
+    torch.cuda.synchronize()
+    s.set_option(_abi.OPT_TIMING, 0)
+    t0 = time.perf_counter()
+    s.run(args.steps * BLOCK, sync=False)
+    s.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    ev_ms, launches, _ = s.timing()
+    value = cells * args.steps * BLOCK / wall
+    log(f"[bench] batched FP32: {wall*1e3:.1f} ms wall ({ev_ms:.1f} ms events) for {args.steps} blocks "
+        f"-> {value:.3e} cell-updates/s")
+    # per-launch duration on the library's stream, timed with HIP events
+    s.set_option(_abi.OPT_TIMING, 1)
+    s.run(min(4, max(1, nsteps // BLOCK - total_blocks)) * BLOCK)
+    _, nl, mean_launch_ms = s.timing()
+    s.close()
+    alg_bytes = cells * BLOCK * BYTES_FP32
+    achieved = alg_bytes / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
+
+    # ---------------- plain FP64 single tube ----------------
+    spec64 = tubes.vowel_tube(0, num_steps=33075)
+    s64 = vocalsim.Solver(spec64, precision=64, threads=256, steps_per_launch=BLOCK)
+    c64 = s64.cells_per_step
+    s64.run(BLOCK)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s64.run(args.fp64_blocks * BLOCK)
+    torch.cuda.synchronize()
+    w64 = time.perf_counter() - t0
+    s64.close()
+    fp64_rate = c64 * args.fp64_blocks * BLOCK / w64
+    log(f"[bench] plain FP64 single tube ({c64} cells): {fp64_rate:.3e} cell-updates/s")
+
+    # ---------------- CPU baseline (oracle, one core) ----------------
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        t = orc.OracleTube(spec64, 64)
+        n_try = 200
+        _, dt = (None, None)
+        t0 = time.perf_counter()
+        t.run(n_try)
+        dt = time.perf_counter() - t0
+        n_cpu = max(n_try, int(n_try * args.cpu_seconds / max(dt, 1e-6)))
+        t.reset()
+        t0 = time.perf_counter()
+        t.run(n_cpu)
+        dt = time.perf_counter() - t0
+        cpu = {"value": c64 * n_cpu / dt, "unit": "grid-cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 oracle (flat arrays, -O2, 1 thread), 44-section vowel tube "
+                         f"{t.H}x{t.W}, first {n_cpu} of 33075 steps"}
+        log(f"[bench] CPU oracle: {cpu['value']:.3e} cell-updates/s over {n_cpu} steps")
+
+    out = {
+        "metric": "grid-cell-updates/s (batched FP32 vs plain FP64) at <=1 Hz F1-F4 error",
+        "value": value,
+        "unit": "grid-cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batch of {args.tubes} ragged elliptical tubes 13-19 cm, 30-50 sections, "
+                               f"half Gaussian / half two-mass, FP32 state, 256-step blocks",
+                   "tubes": args.tubes, "cells_per_step": cells, "block_steps": BLOCK,
+                   "kernel": kernel, "threads": threads},
+        "fp64_plain": {"value": fp64_rate, "cells_per_step": c64,
+                       "workload": "44-section vowel tube, two-mass 800 Pa, FP64, one tube"},
+        "ratio_batched_fp32_over_plain_fp64": value / fp64_rate,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step (batched FP32)", "mean_launch_ms": mean_launch_ms,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+/*
+ * vocalsim_abi.h -- C ABI of the MI355X-native 2.5D FDTD vocal-tract solver.
+ *
+ * Plain C: POD structs, raw pointers and sizes, int status codes.  No HIP or
+ * torch types appear in any signature; device pointers are `void*`.
+ *
+ * One handle owns a batch of n >= 1 tubes (n = 1 is the plain single-tube
+ * solver).  The handle owns all device memory; pointers handed out stay valid
+ * until vs_destroy.  Every call returns VS_OK (0) or a negative VS_E* code and
+ * sets a thread-local message readable with vs_last_error().  Nothing calls
+ * exit() or traps.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   vs_create            FDTDSolver::FDTDSolver + setupWaveSolver()
+ *                        (src/FDTDSolver.cu:71-83, 289-476) together with the
+ *                        TubeGeometry / TubeDepth setup they drive
+ *                        (src/TubeGeometry.cu:53-237, src/TubeDepth.cu:95-265)
+ *   vs_get_dims          public fields frameH/frameW/domainH/domainW,
+ *                        listenerX/Y, excitationX/Y, num_steps
+ *                        (include/FDTDSolver.h:39-81)
+ *   vs_get_field         coefficient getters getN_out() ... getZ_inv() and
+ *                        getDepthP/X/Y(), getPV_N() (include/FDTDSolver.h:143-170)
+ *   vs_run               FDTDEngineCPU::startSolverEngine time loop
+ *                        (src/FDTDEngineCPU.cpp:222-304), FDTDEngineEigen
+ *                        (src/FDTDEngineEigen.cpp:121-183)
+ *   vs_get_probe         record_pressure / savePressureSamples
+ *                        (src/FDTDEngineCPU.cpp:342, 435-450)
+ *   vs_destroy           FDTDEngineCPU::freeCUDAMemory (src/FDTDEngineCPU.cpp:199-219)
+ *                        and the solver's device vectors
+ */
+#ifndef VOCALSIM_ABI_H_
+#define VOCALSIM_ABI_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VS_ABI_VERSION 1
+
+enum vs_status {
+    VS_OK = 0,
+    VS_E_INVALID = -1,     /* bad argument or spec */
+    VS_E_GEOMETRY = -2,    /* geometry cannot be built (e.g. depth sqrt domain) */
+    VS_E_DEVICE = -3,      /* HIP runtime error */
+    VS_E_RANGE = -4,       /* index / size out of range */
+    VS_E_STATE = -5,       /* call not valid in the current state */
+    VS_E_NUMERIC = -6      /* non-finite value detected on the device */
+};
+
+enum vs_precision { VS_FP32 = 32, VS_FP64 = 64 };
+
+/* Tube description: the reference's SimulationParams flags
+ * (include/SimulationParams.h:74-98) plus the inputs the workloads add.
+ * Section arrays are float32, as the reference reads them
+ * (src/TubeGeometry.cu:83-88). */
+typedef struct vs_tube_spec {
+    int fdtd_solver_type;       /* 0: 2D, 1: 2.5D                          */
+    int cross_sectional_shape;  /* 1: circular, 2: elliptical               */
+    int mouth_radiation_cond;   /* 1: Dirichlet mouth, 0: open              */
+    int srate_mul;              /* fs = 44100 * srate_mul                   */
+    int pml_flag;               /* 0 / 1                                    */
+    int source_type;            /* 0: sine, 1: Gaussian, 2: two-mass        */
+    int num_steps;              /* > 0 explicit; <= 0 float accumulation of audio_dur */
+    float audio_dur;            /* seconds                                  */
+    int num_sections;
+    const float* length_cm;     /* [num_sections]                           */
+    const float* area_cm2;      /* [num_sections]                           */
+    const float* aspect;        /* ellipse major:minor per section, NULL -> 3 */
+    const float* depth_scale;   /* per-section depth multiplier, NULL -> 1  */
+    const float* mu3d;          /* per-section wall admittance, NULL -> default */
+    float mu3d_default;         /* <= 0 -> 0.025 (2.5D) / 0.005 (2D)        */
+    double gauss_amp;           /* <= 0 -> reference pulse (0.025, 64.6 us, 0.29*t0) */
+    double gauss_t0;
+    double gauss_width;
+    double tm_ps;               /* two-mass subglottal pressure, <= 0 -> 686.46 Pa */
+    double tm_q;                /* two-mass pitch factor, <= 0 -> 1         */
+    int engine_variant;         /* 0: FDTDEngineCPU rounding, 1: FDTDEngineEigen */
+} vs_tube_spec;
+
+typedef struct vs_dims {
+    int frameH, frameW, domainH, domainW;
+    int listenerX, listenerY, excitationX, excitationY;
+    int num_steps, tube_start_x, tube_end_x, tube_len_cells;
+    int midY, num_sections, mic_cells;
+} vs_dims;
+
+/* Field ids for vs_get_field / vs_field_ptr. */
+enum vs_field {
+    VS_F_CELL_TYPE = 0,   /* int32 [H*W]  CellTypes enum value                     */
+    VS_F_BOUNDARY_SEG,    /* int32 [H*W]  boundary_segment_type                    */
+    VS_F_DEPTH_P,         /* Real  [H*W]                                           */
+    VS_F_DEPTH_X,         /* Real  [H*W]                                           */
+    VS_F_DEPTH_Y,         /* Real  [H*W]                                           */
+    VS_F_Z_INV,           /* Real  [H*W]                                           */
+    VS_F_SOURCE,          /* Real  [num_steps] precomputed excitation table        */
+    VS_F_UPPER_WALL,      /* int32 [L]                                             */
+    VS_F_LOWER_WALL,      /* int32 [L]                                             */
+    VS_F_SEGMENT,         /* int32 [L]   section index of each tube column         */
+    VS_F_PRESSURE,        /* Real  [H*W] current P                                 */
+    VS_F_VX,              /* Real  [H*W] current Vx                                */
+    VS_F_VY,              /* Real  [H*W] current Vy                                */
+    VS_F_CELL_CLASS,      /* uint8 [H*W] packed class byte (type | segment-wall bit) */
+    VS_F_COEFF0 = 100     /* + k, k in [0,17): Real [(H-2)*(W-2)] coefficient grid k
+                             in getter order N_out, N_in, minVxBeta, minVyBeta,
+                             maxVxSigmaPrimedt, maxVySigmaPrimedt,
+                             betaVxSqr_dt_inv_rho_inv_ds, betaVySqr_dt_inv_rho_inv_ds,
+                             pressureSigmaPrimedt, areWeNotExcitationVx/Vy,
+                             excitationWeightVx/Vy, xor_val1..4                     */
+};
+
+/* Tunables for vs_set_option. */
+enum vs_option {
+    VS_OPT_STEPS_PER_LAUNCH = 0, /* steps advanced by one launch of the step kernel (default 256) */
+    VS_OPT_THREADS = 1,          /* work-group size of the step kernel (64..1024) */
+    VS_OPT_USE_GRAPH = 2,        /* 1: replay step blocks from a captured hipGraph */
+    VS_OPT_KERNEL = 3,           /* step kernel family, see vs_kernel */
+    VS_OPT_TIMING = 4            /* 1: time every launch with HIP events */
+};
+
+enum vs_kernel {
+    VS_K_EXACT = 0,   /* reference operation order (bit-exact vs the oracle's instantiation) */
+    VS_K_FAST = 1     /* FP32 reassociated coefficient form (batched throughput path) */
+};
+
+typedef struct vs_handle vs_handle;
+
+const char* vs_last_error(void);
+int vs_abi_version(void);
+
+/* Builds n tubes on the current HIP device: host float32 integerisation
+ * followed by the setup kernels (walls, classification, depth maps, z_inv,
+ * source tables).  precision: VS_FP32 or VS_FP64. */
+int vs_create(const vs_tube_spec* specs, int n_tubes, int precision, vs_handle** out);
+int vs_destroy(vs_handle* h);
+
+int vs_num_tubes(const vs_handle* h, int* n);
+int vs_get_dims(const vs_handle* h, int tube, vs_dims* out);
+
+/* Copies a field of one tube to host memory. bytes must be at least the
+ * field size; returns VS_E_RANGE otherwise. */
+int vs_get_field(vs_handle* h, int tube, int field, void* dst, size_t bytes);
+
+/* Advances every tube by n_steps (tubes stop at their own num_steps).
+ * Asynchronous on the handle's stream; vs_sync waits. */
+int vs_run(vs_handle* h, int n_steps);
+int vs_sync(vs_handle* h);
+/* Steps already advanced. */
+int vs_steps_done(const vs_handle* h, int* steps);
+/* Zeros P, V and the source state; probe history restarts at step 0. */
+int vs_reset(vs_handle* h);
+
+/* Probe waveform (P at the listener after each step's pressure phase) and
+ * the injected source velocity of each step, as float64. */
+int vs_get_probe(vs_handle* h, int tube, double* dst, size_t n);
+int vs_get_source_trace(vs_handle* h, int tube, double* dst, size_t n);
+
+int vs_set_option(vs_handle* h, int option, int value);
+int vs_get_option(const vs_handle* h, int option, int* value);
+
+/* Device pointer of a field (same ids as vs_get_field, tube's slice), and the
+ * handle's stream as an opaque pointer (hipStream_t). */
+int vs_field_ptr(vs_handle* h, int tube, int field, void** dptr, size_t* bytes);
+int vs_stream(vs_handle* h, void** stream);
+
+/* Timing of the last vs_run with HIP events on the handle's stream: total
+ * milliseconds, number of step-kernel launches and (with VS_OPT_TIMING) the
+ * mean duration of one launch. */
+int vs_last_run_timing(const vs_handle* h, double* total_ms, int* launches, double* mean_launch_ms);
+
+/* Interior cell updates of one step summed over the batch:
+ * sum (frameH-2)(frameW-2)  (FDTDEngineCPU.cpp:316-317). */
+int vs_cells_per_step(const vs_handle* h, long long* cells);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

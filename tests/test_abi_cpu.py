@@ -1,0 +1,39 @@
+"""CPU-side checks of the C ABI library: it loads, exports every symbol the
+header declares, and rejects bad input on the host (no GPU compute calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import vocalsim
+from vocalsim import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "vocalsim_abi.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(vs_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_loads_and_exports_all_declared_symbols():
+    L = vocalsim.load()
+    syms = declared_symbols()
+    assert len(syms) >= 19
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(_abi.EXPORTS)
+
+
+def test_abi_version():
+    assert vocalsim.load().vs_abi_version() == 1
+
+
+def test_null_arguments_rejected_without_device():
+    L = vocalsim.load()
+    h = ctypes.c_void_p()
+    assert L.vs_create(None, 1, 64, ctypes.byref(h)) == -1
+    assert b"specs" in L.vs_last_error()
+    assert L.vs_run(None, 10) == -1
+    assert L.vs_destroy(None) == 0

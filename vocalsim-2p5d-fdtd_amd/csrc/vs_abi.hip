@@ -1,0 +1,701 @@
+// vs_abi.hip -- host side of the library: the float32 geometry prelude,
+// device buffer management and the extern "C" entry points declared in
+// include/vocalsim_abi.h.
+
+#include "vs_common.h"
+#include "../../include/vocalsim_abi.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace vs {
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define VS_HIP(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(VS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Host prelude: O(sections) float32 integerisation of one tube
+// (TubeGeometry.cu:53-179, 241-286; FDTDSolver.cu:591-617).  Unit constants
+// are float32 as in include/SimulationUnits.h:11-25.
+// ---------------------------------------------------------------------------
+constexpr float kCm = 1e-2f;
+constexpr float kMm = 1e-3f;
+
+struct Prelude {
+    int H, W, domainH, domainW, L, ts, te, midY, mic, listenerX, listenerY, excX, excY;
+    int num_steps;
+    std::vector<float> cum;        // float32 inclusive scan of section lengths [m]
+    std::vector<int> diam;         // diameter in cells (odd)
+    std::vector<double> minor;     // ellipse minor axis [m] (Real-evaluated, stored in double)
+    std::vector<double> zsec;      // z_inv per section
+    std::vector<double> dscale;
+    PhysTables ph;
+    float ds32;
+};
+
+// SimulationParams.cu:7-33 in float32, SolverParams.cu:5-57, DeviceSymbols.cu:19-20.
+void phys_tables(int srate_mul, bool fp64, PhysTables& t, float& ds32) {
+    const float c32 = 350.0f, rho32 = 1.14f;
+    const int srate = 44100 * srate_mul;
+    const float dt32 = 1.0f / (float)srate;
+    ds32 = dt32 * c32 * sqrtf(2.0f);
+    if (fp64) {
+        t.c = 350.0; t.rho = 1.14;
+        t.dt = 1.0 / (double)srate;
+        t.ds = t.dt * t.c * std::sqrt(2.0);
+        const double kappa = t.rho * t.c * t.c;
+        t.K1 = (kappa * t.dt) / t.ds;
+        t.K2 = kappa * t.dt;
+        t.rds = t.rho * t.ds;
+    } else {
+        const float kappa = rho32 * c32 * c32;
+        t.c = c32; t.rho = rho32; t.dt = dt32; t.ds = ds32;
+        t.K1 = (float)((kappa * dt32) / ds32);
+        t.K2 = (float)(kappa * dt32);
+        t.rds = (float)(rho32 * ds32);
+    }
+    for (int k = 0; k < 16; ++k) { t.beta[k] = 0; t.sigma[k] = 0; }
+    t.beta[kAir] = 1; t.beta[kNoPressure] = 1;
+    for (int k = 0; k < 6; ++k) t.beta[kPml0 + k] = 1;
+    t.sigma[kWall] = t.dt; t.sigma[kExcitation] = t.dt; t.sigma[kHead] = t.dt;
+    t.sigma[kDead] = 1000000;
+    for (int k = 0; k < 6; ++k) {
+        if (fp64) t.sigma[kPml0 + k] = ((double)k / (double)5) * 0.5;
+        else t.sigma[kPml0 + k] = ((float)k / (float)5) * 0.5f;
+    }
+    t.air_normal[0] = 0; t.air_normal[1] = 1; t.air_normal[2] = 1;
+    t.air_normal[3] = fp64 ? 0.707106 : (double)0.707106f;
+    t.wall_normal[0] = 1; t.wall_normal[1] = 1; t.wall_normal[2] = 1; t.wall_normal[3] = 0;
+    t.src_dir[0] = 0; t.src_dir[1] = 0; t.src_dir[2] = 1; t.src_dir[3] = 0;
+}
+
+// Wall admittance -> z_inv (FDTDSolver.cu:613-617).
+double z_inv_of(float mu, bool fp64, const PhysTables& t) {
+    if (fp64) {
+        const double m2 = (double)mu * ((2.0 * 0.5 * M_PI) / 1.84);
+        const double a = 1.0 / (0.5 + 0.25 * (m2 + (1 / m2)));
+        return 1 / (t.rho * t.c * ((1 + std::sqrt(1 - a)) / (1 - std::sqrt(1 - a))));
+    }
+    const float m2 = mu * ((2.0f * 0.5F * M_PI) / 1.84f);
+    const float a = 1.0f / (0.5f + 0.25f * (m2 + (1 / m2)));
+    const float rc = (float)t.rho * (float)t.c;
+    return 1 / (rc * ((1 + sqrtf(1 - a)) / (1 - sqrtf(1 - a))));
+}
+
+// Time stamps accumulated in float32 (SolverParams.cu:59-66).
+int float_step_count(float dur, float dt) {
+    int n = 0;
+    for (float t = 0; t <= dur - dt; t = t + dt) ++n;
+    return n;
+}
+
+int prelude(const vs_tube_spec& s, bool fp64, Prelude& p) {
+    if (s.num_sections <= 0 || !s.length_cm || !s.area_cm2) return fail(VS_E_INVALID, "empty area function");
+    if (s.srate_mul <= 0) return fail(VS_E_INVALID, "srate_mul must be positive");
+    if (s.fdtd_solver_type != 0 && s.fdtd_solver_type != 1) return fail(VS_E_INVALID, "fdtd_solver_type must be 0 or 1");
+    if (s.cross_sectional_shape != 1 && s.cross_sectional_shape != 2)
+        return fail(VS_E_INVALID, "cross_sectional_shape must be 1 (circular) or 2 (elliptical)");
+    if (s.source_type < 0 || s.source_type > 2) return fail(VS_E_INVALID, "Invalid source type");
+    const int n = s.num_sections;
+    for (int i = 0; i < n; ++i)
+        if (!(s.area_cm2[i] > 0.0f) || !(s.length_cm[i] > 0.0f))
+            return fail(VS_E_INVALID, "section area and length must be positive");
+    phys_tables(s.srate_mul, fp64, p.ph, p.ds32);
+    const float ds = p.ds32;
+    p.num_steps = s.num_steps > 0 ? s.num_steps : float_step_count(s.audio_dur, 1.0f / (float)(44100 * s.srate_mul));
+    if (p.num_steps <= 0) return fail(VS_E_INVALID, "number of steps must be positive");
+    // sagittal widths (circle / q:1 ellipse) and minor axes
+    std::vector<float> width(n);
+    p.minor.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const float a = s.area_cm2[i] * kCm * kCm;
+        if (s.cross_sectional_shape == 1) {
+            width[i] = 2.0 * sqrt(a / M_PI);
+            p.minor[i] = fp64 ? 2.0 * std::sqrt((double)s.area_cm2[i] * 1e-2 * 1e-2 / M_PI) : (double)width[i];
+        } else {
+            const float q = s.aspect ? s.aspect[i] : 3.0f;
+            if (!(q > 0.0f)) return fail(VS_E_INVALID, "ellipse aspect must be positive");
+            const float lx = sqrtf(a / (q * 1.0f * M_PI));
+            width[i] = lx * (q * 2);
+            if (fp64) p.minor[i] = std::sqrt((double)s.area_cm2[i] * 1e-2 * 1e-2 / ((double)q * 1.0 * M_PI)) * 2.0;
+            else p.minor[i] = lx * (1.0f * 2);
+        }
+    }
+    if (s.fdtd_solver_type == 0) {
+        // 2D area scaling (TubeGeometry.cu:241-286); pow(x, 2) as x * x
+        const float mx = *std::max_element(width.begin(), width.end());
+        size_t mi = 0;
+        for (size_t i = 0; i < width.size(); ++i) if (width[i] == mx) mi = i;
+        std::vector<float> sc(n, 0.0f), m(n, 1.0f);
+        sc[mi] = (mx * 0.5 * M_PI) / 1.84f;
+        for (int i = 0; i < n; ++i) {
+            if ((size_t)i < mi) { const float q = width[i] / width[i + 1]; m[i] = width[i + 1] != 0.0f ? q * q : 1.0f; }
+            if ((size_t)i > mi) { const float q = width[i] / width[i - 1]; m[i] = width[i - 1] != 0.0f ? q * q : 1.0f; }
+        }
+        for (int i = (int)mi - 1; i >= 0; --i) sc[i] = sc[i + 1] * m[i];
+        for (int i = (int)mi + 1; i < n; ++i) sc[i] = sc[i - 1] * m[i];
+        width = sc;
+    }
+    p.mic = (int)roundf((3 * kMm) / ds);
+    float total = 0.0f;
+    p.cum.resize(n);
+    for (int i = 0; i < n; ++i) { total = total + s.length_cm[i] * kCm; p.cum[i] = total; }
+    p.L = (int)roundf(total / ds);
+    if (p.L < 2) return fail(VS_E_INVALID, "tube shorter than two cells");
+    p.diam.resize(n);
+    for (int i = 0; i < n; ++i) {
+        int d = (int)roundf(width[i] / ds);
+        if (d == 0) d = 1;
+        if (d % 2 == 0) {
+            const float over = (float)d - width[i] / ds;
+            d = over > 0 ? d - 1 : d + 1;
+        }
+        p.diam[i] = d;
+    }
+    const int pml = s.pml_flag ? 1 : 0;
+    const int maxd = *std::max_element(p.diam.begin(), p.diam.end());
+    p.domainW = s.mouth_radiation_cond == 1 ? p.L + 2 : p.L + 1 + p.mic;
+    p.domainH = maxd + 2;
+    p.W = p.domainW + 2 + 12 * pml;
+    p.H = p.domainH + 2 + 12 * pml;
+    p.ts = 2 + 6 * pml;
+    p.te = p.ts + p.L - 1;
+    p.midY = (int)ceilf(p.H / 2.0f) - 1;
+    p.listenerY = p.midY;
+    p.listenerX = s.mouth_radiation_cond == 1 ? p.te - p.mic : p.te + 3 + p.mic;
+    if (p.listenerX > p.W - 2) return fail(VS_E_INVALID, "listener falls outside the frame (open mouth without PML)");
+    p.excX = p.ts - 1;
+    p.excY = p.midY;
+    const float mu_def = s.mu3d_default > 0 ? s.mu3d_default : (s.fdtd_solver_type == 1 ? 0.025f : 0.005f);
+    p.zsec.resize(n);
+    p.dscale.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const float mu = s.mu3d ? s.mu3d[i] : mu_def;
+        if (!(mu > 0.0f)) return fail(VS_E_INVALID, "wall admittance must be positive");
+        p.zsec[i] = z_inv_of(mu, fp64, p.ph);
+        const float sc = s.depth_scale ? s.depth_scale[i] : 1.0f;
+        if (!(sc > 0.0f)) return fail(VS_E_INVALID, "depth scale must be positive");
+        p.dscale[i] = sc;
+    }
+    return VS_OK;
+}
+
+// Excitation table (GlottalExcitations.cu:9-52) in Real.
+template <typename Real>
+void source_table(const vs_tube_spec& s, const PhysTables& ph, int num_steps, std::vector<Real>& out) {
+    out.assign(num_steps, Real(0));
+    const double dt = ph.dt;
+    if (s.source_type == 0) {
+        const Real f = 440.0f, amp = 25;
+        for (int i = 0; i < num_steps; ++i) {
+            const Real t = i * dt;
+            out[i] = amp * std::sin(2 * M_PI * f * t);
+        }
+    } else if (s.source_type == 1) {
+        Real f0 = 10.0f * 1e3f;
+        Real peak = 0.646f / f0;
+        Real width = 0.29f * peak;
+        Real amp = 0.025f;
+        if (s.gauss_amp > 0) amp = (Real)s.gauss_amp;
+        if (s.gauss_t0 > 0) peak = (Real)s.gauss_t0;
+        if (s.gauss_width > 0) width = (Real)s.gauss_width;
+        for (int i = 0; i < num_steps; ++i) {
+            const Real t = i * dt;
+            const Real e = (t - peak) / width;
+            out[i] = amp * std::exp(-1 * (e * e));
+        }
+    }
+}
+
+}  // namespace
+}  // namespace vs
+
+using namespace vs;
+
+struct vs_handle {
+    int n = 0;
+    int precision = 64;
+    std::vector<TubeDesc> desc;
+    std::vector<vs_dims> dims;
+    int max_steps = 0;
+    int steps_done = 0;
+    long long cells_per_step = 0;
+    // options
+    int steps_per_launch = 256;
+    int threads = 256;
+    int use_graph = 0;
+    int kernel = VS_K_EXACT;
+    int timing = 0;
+    // device memory
+    std::vector<void*> allocs;
+    TubeDesc* d_desc = nullptr;
+    int* d_order = nullptr;
+    void* bufs = nullptr;   // Bufs<float> or Bufs<double>
+    Bufs<float> bf{};
+    Bufs<double> bd{};
+    long long total_cells = 0, total_cols = 0, total_walls = 0, total_secs = 0, total_src = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0, last_mean_launch_ms = 0;
+    int last_launches = 0;
+    std::vector<TwoMassState> tm0;   // initial two-mass states (for vs_reset)
+
+    ~vs_handle() {
+        for (void* p : allocs) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    template <typename T>
+    int alloc(T** p, size_t count) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = static_cast<T*>(q);
+        e = hipMemsetAsync(q, 0, std::max<size_t>(count, 1) * sizeof(T), stream);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+        return VS_OK;
+    }
+};
+
+namespace {
+
+template <typename Real>
+Bufs<Real>& bufs_of(vs_handle* h);
+template <>
+Bufs<float>& bufs_of<float>(vs_handle* h) { return h->bf; }
+template <>
+Bufs<double>& bufs_of<double>(vs_handle* h) { return h->bd; }
+
+template <typename Real>
+int build(vs_handle* h, const vs_tube_spec* specs) {
+    const bool fp64 = sizeof(Real) == 8;
+    std::vector<Prelude> pre(h->n);
+    for (int k = 0; k < h->n; ++k) {
+        int rc = prelude(specs[k], fp64, pre[k]);
+        if (rc != VS_OK) return fail(rc, "tube " + std::to_string(k) + ": " + g_last_error);
+    }
+    h->desc.resize(h->n);
+    h->dims.resize(h->n);
+    long long cell = 0, col = 0, wall = 0, sec = 0, src = 0;
+    for (int k = 0; k < h->n; ++k) {
+        const Prelude& p = pre[k];
+        const vs_tube_spec& s = specs[k];
+        TubeDesc& d = h->desc[k];
+        std::memset(&d, 0, sizeof(d));
+        d.H = p.H; d.W = p.W; d.L = p.L; d.nsec = s.num_sections;
+        d.ts = p.ts; d.te = p.te; d.midY = p.midY;
+        d.listener = p.listenerY * p.W + p.listenerX;
+        d.excitation = p.excY * p.W + p.excX;
+        d.num_steps = p.num_steps;
+        d.source_type = s.source_type; d.dirichlet = s.mouth_radiation_cond == 1;
+        d.variant = s.engine_variant; d.srate_mul = s.srate_mul;
+        d.shape = s.cross_sectional_shape; d.solver_type = s.fdtd_solver_type;
+        d.ds_f32 = p.ds32; d.pml = s.pml_flag ? 1 : 0;
+        d.cell_off = cell; d.col_off = col; d.wall_off = wall; d.sec_off = sec; d.src_off = src;
+        d.ph = p.ph;
+        cell += ((long long)p.H * p.W + 63) / 64 * 64;   // 256-byte aligned tube blocks
+        col += p.W; wall += p.L; sec += s.num_sections; src += p.num_steps;
+        h->max_steps = std::max(h->max_steps, p.num_steps);
+        h->cells_per_step += (long long)(p.H - 2) * (p.W - 2);
+        vs_dims& dm = h->dims[k];
+        dm.frameH = p.H; dm.frameW = p.W; dm.domainH = p.domainH; dm.domainW = p.domainW;
+        dm.listenerX = p.listenerX; dm.listenerY = p.listenerY;
+        dm.excitationX = p.excX; dm.excitationY = p.excY; dm.num_steps = p.num_steps;
+        dm.tube_start_x = p.ts; dm.tube_end_x = p.te; dm.tube_len_cells = p.L;
+        dm.midY = p.midY; dm.num_sections = s.num_sections; dm.mic_cells = p.mic;
+    }
+    h->total_cells = cell; h->total_cols = col; h->total_walls = wall; h->total_secs = sec; h->total_src = src;
+    Bufs<Real>& b = bufs_of<Real>(h);
+    int rc;
+#define ALLOC(ptr, n) if ((rc = h->alloc(&(ptr), (n))) != VS_OK) return rc
+    ALLOC(b.P, cell); ALLOC(b.Vx, cell); ALLOC(b.Vy, cell);
+    ALLOC(b.dP, cell); ALLOC(b.dX, cell); ALLOC(b.dY, cell);
+    ALLOC(b.cls, cell); ALLOC(b.bst, cell);
+    ALLOC(b.zcol, col);
+    ALLOC(b.upper, wall); ALLOC(b.lower, wall); ALLOC(b.seg, wall);
+    ALLOC(b.cum, sec); ALLOC(b.diam, sec); ALLOC(b.minor, sec); ALLOC(b.dscale, sec); ALLOC(b.zsec, sec);
+    ALLOC(b.source, src); ALLOC(b.probe, src); ALLOC(b.strace, src);
+    ALLOC(b.tm, h->n); ALLOC(b.pexc, h->n); ALLOC(b.status, 1);
+    ALLOC(h->d_desc, h->n); ALLOC(h->d_order, h->n);
+    // host staging of the per-section and per-step inputs
+    std::vector<float> cum; std::vector<int> diam; std::vector<Real> minor, dsc, zs, srct;
+    std::vector<TwoMassState> tms(h->n);
+    for (int k = 0; k < h->n; ++k) {
+        const Prelude& p = pre[k];
+        cum.insert(cum.end(), p.cum.begin(), p.cum.end());
+        diam.insert(diam.end(), p.diam.begin(), p.diam.end());
+        for (int i = 0; i < specs[k].num_sections; ++i) {
+            minor.push_back((Real)p.minor[i]);
+            dsc.push_back((Real)p.dscale[i]);
+            zs.push_back((Real)p.zsec[i]);
+        }
+        std::vector<Real> tab;
+        source_table<Real>(specs[k], p.ph, p.num_steps, tab);
+        srct.insert(srct.end(), tab.begin(), tab.end());
+        const double q = specs[k].tm_q > 0 ? specs[k].tm_q : 1.0;
+        const double ps = specs[k].tm_ps > 0 ? specs[k].tm_ps : (double)686.46f;
+        twomass_init(tms[k], q, ps);
+    }
+    // largest tubes first
+    std::vector<int> order(h->n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        return (long long)h->desc[a].H * h->desc[a].W > (long long)h->desc[c].H * h->desc[c].W;
+    });
+    h->tm0 = tms;
+    hipStream_t st = h->stream;
+    VS_HIP(hipMemcpyAsync(b.cum, cum.data(), cum.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    VS_HIP(hipMemcpyAsync(b.diam, diam.data(), diam.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    VS_HIP(hipMemcpyAsync(b.minor, minor.data(), minor.size() * sizeof(Real), hipMemcpyHostToDevice, st));
+    VS_HIP(hipMemcpyAsync(b.dscale, dsc.data(), dsc.size() * sizeof(Real), hipMemcpyHostToDevice, st));
+    VS_HIP(hipMemcpyAsync(b.zsec, zs.data(), zs.size() * sizeof(Real), hipMemcpyHostToDevice, st));
+    VS_HIP(hipMemcpyAsync(b.source, srct.data(), srct.size() * sizeof(Real), hipMemcpyHostToDevice, st));
+    VS_HIP(hipMemcpyAsync(b.tm, tms.data(), tms.size() * sizeof(TwoMassState), hipMemcpyHostToDevice, st));
+    VS_HIP(hipMemcpyAsync(h->d_desc, h->desc.data(), h->n * sizeof(TubeDesc), hipMemcpyHostToDevice, st));
+    VS_HIP(hipMemcpyAsync(h->d_order, order.data(), h->n * sizeof(int), hipMemcpyHostToDevice, st));
+    // setup kernels; depth scratch is one maxHW slice per tube
+    long long maxHW = 0;
+    for (auto& d : h->desc) maxHW = std::max(maxHW, (long long)d.H * d.W);
+    Real* tmp = nullptr;
+    VS_HIP(hipMalloc(&tmp, sizeof(Real) * (size_t)maxHW * h->n * 4));
+    const size_t slab = (size_t)maxHW * h->n;
+    hipError_t e = launch_setup<Real>(h->d_desc, h->desc.data(), h->n, b, tmp, tmp + slab, tmp + 2 * slab,
+                                      tmp + 3 * slab, st);
+    hipError_t e2 = hipStreamSynchronize(st);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("setup launch: ") + hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(VS_E_DEVICE, std::string("setup: ") + hipGetErrorString(e2));
+    unsigned status = 0;
+    VS_HIP(hipMemcpy(&status, b.status, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (status & kStatusSqrtDomain) return fail(VS_E_GEOMETRY, "depth sqrt domain error");
+    return VS_OK;
+}
+
+int check_handle(const vs_handle* h) {
+    if (!h) return fail(VS_E_INVALID, "null handle");
+    return VS_OK;
+}
+
+template <typename Real>
+int run_steps(vs_handle* h, int n_steps) {
+    Bufs<Real>& b = bufs_of<Real>(h);
+    const int start = h->steps_done;
+    const int end = std::min(h->max_steps, start + n_steps);
+    int launches = 0;
+    double launch_ms = 0;
+    std::vector<hipEvent_t> evs;
+    VS_HIP(hipEventRecord(h->ev0, h->stream));
+    for (int t0 = start; t0 < end; t0 += h->steps_per_launch) {
+        const int ns = std::min(h->steps_per_launch, end - t0);
+        hipEvent_t a = nullptr, c = nullptr;
+        if (h->timing) {
+            VS_HIP(hipEventCreate(&a)); VS_HIP(hipEventCreate(&c));
+            VS_HIP(hipEventRecord(a, h->stream));
+        }
+        hipError_t e = launch_step<Real>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, h->stream);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("step launch: ") + hipGetErrorString(e));
+        if (h->timing) { VS_HIP(hipEventRecord(c, h->stream)); evs.push_back(a); evs.push_back(c); }
+        ++launches;
+    }
+    VS_HIP(hipEventRecord(h->ev1, h->stream));
+    h->steps_done = end;
+    h->last_launches = launches;
+    if (h->timing) {
+        VS_HIP(hipEventSynchronize(h->ev1));
+        for (size_t i = 0; i < evs.size(); i += 2) {
+            float ms = 0;
+            VS_HIP(hipEventElapsedTime(&ms, evs[i], evs[i + 1]));
+            launch_ms += ms;
+            (void)hipEventDestroy(evs[i]); (void)hipEventDestroy(evs[i + 1]);
+        }
+        h->last_mean_launch_ms = launches ? launch_ms / launches : 0;
+    }
+    return VS_OK;
+}
+
+size_t elem_size(const vs_handle* h) { return h->precision == 64 ? 8 : 4; }
+
+// Field location: device pointer, element count and element size.
+int locate(vs_handle* h, int tube, int field, void** ptr, size_t* count, size_t* esz) {
+    const TubeDesc& d = h->desc[tube];
+    const size_t HW = (size_t)d.H * d.W;
+    const bool f64 = h->precision == 64;
+    auto R = [&](void* base_f, void* base_d, long long off, size_t n) {
+        *esz = f64 ? 8 : 4;
+        *ptr = f64 ? (void*)((double*)base_d + off) : (void*)((float*)base_f + off);
+        *count = n;
+        return VS_OK;
+    };
+    switch (field) {
+    case VS_F_CELL_TYPE: return fail(VS_E_INVALID, "cell types are derived on copy");
+    case VS_F_BOUNDARY_SEG:
+        *ptr = (f64 ? h->bd.bst : h->bf.bst) + d.cell_off; *count = HW; *esz = 4; return VS_OK;
+    case VS_F_CELL_CLASS:
+        *ptr = (f64 ? h->bd.cls : h->bf.cls) + d.cell_off; *count = HW; *esz = 1; return VS_OK;
+    case VS_F_DEPTH_P: return R(h->bf.dP, h->bd.dP, d.cell_off, HW);
+    case VS_F_DEPTH_X: return R(h->bf.dX, h->bd.dX, d.cell_off, HW);
+    case VS_F_DEPTH_Y: return R(h->bf.dY, h->bd.dY, d.cell_off, HW);
+    case VS_F_PRESSURE: return R(h->bf.P, h->bd.P, d.cell_off, HW);
+    case VS_F_VX: return R(h->bf.Vx, h->bd.Vx, d.cell_off, HW);
+    case VS_F_VY: return R(h->bf.Vy, h->bd.Vy, d.cell_off, HW);
+    case VS_F_SOURCE: return R(h->bf.source, h->bd.source, d.src_off, d.num_steps);
+    case VS_F_UPPER_WALL:
+        *ptr = (f64 ? h->bd.upper : h->bf.upper) + d.wall_off; *count = d.L; *esz = 4; return VS_OK;
+    case VS_F_LOWER_WALL:
+        *ptr = (f64 ? h->bd.lower : h->bf.lower) + d.wall_off; *count = d.L; *esz = 4; return VS_OK;
+    case VS_F_SEGMENT:
+        *ptr = (f64 ? h->bd.seg : h->bf.seg) + d.wall_off; *count = d.L; *esz = 4; return VS_OK;
+    default: break;
+    }
+    return fail(VS_E_INVALID, "unknown field id " + std::to_string(field));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vs_last_error(void) { return g_last_error.c_str(); }
+int vs_abi_version(void) { return VS_ABI_VERSION; }
+
+int vs_create(const vs_tube_spec* specs, int n_tubes, int precision, vs_handle** out) {
+    g_last_error.clear();
+    if (!specs || !out || n_tubes <= 0) return fail(VS_E_INVALID, "vs_create: need specs, n_tubes > 0 and an output handle");
+    if (precision != VS_FP32 && precision != VS_FP64) return fail(VS_E_INVALID, "precision must be 32 or 64");
+    *out = nullptr;
+    std::unique_ptr<vs_handle> h(new vs_handle());
+    h->n = n_tubes;
+    h->precision = precision;
+    VS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    VS_HIP(hipEventCreate(&h->ev0));
+    VS_HIP(hipEventCreate(&h->ev1));
+    const int rc = precision == VS_FP64 ? build<double>(h.get(), specs) : build<float>(h.get(), specs);
+    if (rc != VS_OK) return rc;
+    *out = h.release();
+    return VS_OK;
+}
+
+int vs_destroy(vs_handle* h) {
+    if (!h) return VS_OK;
+    (void)hipStreamSynchronize(h->stream);
+    delete h;
+    return VS_OK;
+}
+
+int vs_num_tubes(const vs_handle* h, int* n) {
+    if (check_handle(h) || !n) return fail(VS_E_INVALID, "vs_num_tubes: bad argument");
+    *n = h->n;
+    return VS_OK;
+}
+
+int vs_get_dims(const vs_handle* h, int tube, vs_dims* out) {
+    if (check_handle(h) || !out) return fail(VS_E_INVALID, "vs_get_dims: bad argument");
+    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+    *out = h->dims[tube];
+    return VS_OK;
+}
+
+int vs_field_ptr(vs_handle* h, int tube, int field, void** dptr, size_t* bytes) {
+    if (check_handle(h) || !dptr) return fail(VS_E_INVALID, "vs_field_ptr: bad argument");
+    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+    size_t count = 0, esz = 0;
+    int rc = locate(h, tube, field, dptr, &count, &esz);
+    if (rc != VS_OK) return rc;
+    if (bytes) *bytes = count * esz;
+    return VS_OK;
+}
+
+int vs_get_field(vs_handle* h, int tube, int field, void* dst, size_t bytes) {
+    if (check_handle(h) || !dst) return fail(VS_E_INVALID, "vs_get_field: bad argument");
+    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+    const TubeDesc& d = h->desc[tube];
+    const size_t HW = (size_t)d.H * d.W;
+    if (field == VS_F_CELL_TYPE) {
+        if (bytes < HW * sizeof(int)) return fail(VS_E_RANGE, "destination too small");
+        std::vector<uint8_t> cls(HW);
+        VS_HIP(hipStreamSynchronize(h->stream));
+        const uint8_t* src = (h->precision == 64 ? h->bd.cls : h->bf.cls) + d.cell_off;
+        VS_HIP(hipMemcpy(cls.data(), src, HW, hipMemcpyDeviceToHost));
+        int* o = static_cast<int*>(dst);
+        for (size_t i = 0; i < HW; ++i) o[i] = cls[i] & kTypeMask;
+        return VS_OK;
+    }
+    if (field == VS_F_Z_INV) {
+        const size_t es = elem_size(h);
+        if (bytes < HW * es) return fail(VS_E_RANGE, "destination too small");
+        std::vector<uint8_t> cls(HW);
+        VS_HIP(hipStreamSynchronize(h->stream));
+        const uint8_t* src = (h->precision == 64 ? h->bd.cls : h->bf.cls) + d.cell_off;
+        VS_HIP(hipMemcpy(cls.data(), src, HW, hipMemcpyDeviceToHost));
+        std::vector<double> zc(d.W);
+        if (h->precision == 64) {
+            VS_HIP(hipMemcpy(zc.data(), h->bd.zcol + d.col_off, d.W * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> zf(d.W);
+            VS_HIP(hipMemcpy(zf.data(), h->bf.zcol + d.col_off, d.W * 4, hipMemcpyDeviceToHost));
+            for (int c = 0; c < d.W; ++c) zc[c] = zf[c];
+        }
+        for (size_t i = 0; i < HW; ++i) {
+            const double z = (cls[i] & kSegWallBit) ? zc[i % d.W] : 0.0;
+            if (h->precision == 64) static_cast<double*>(dst)[i] = z;
+            else static_cast<float*>(dst)[i] = (float)z;
+        }
+        return VS_OK;
+    }
+    if (field >= VS_F_COEFF0 && field < VS_F_COEFF0 + 17) {
+        const size_t ni = (size_t)(d.H - 2) * (d.W - 2);
+        const size_t es = elem_size(h);
+        if (bytes < ni * es) return fail(VS_E_RANGE, "destination too small");
+        void* tmp = nullptr;
+        VS_HIP(hipMalloc(&tmp, ni * 17 * es));
+        hipError_t e = h->precision == 64
+            ? launch_coeffs<double>(h->d_desc, &d, tube, h->bd, (double*)tmp, h->stream)
+            : launch_coeffs<float>(h->d_desc, &d, tube, h->bf, (float*)tmp, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess)
+            e = hipMemcpy(dst, (char*)tmp + (size_t)(field - VS_F_COEFF0) * ni * es, ni * es, hipMemcpyDeviceToHost);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("coefficients: ") + hipGetErrorString(e));
+        return VS_OK;
+    }
+    void* p = nullptr;
+    size_t count = 0, esz = 0;
+    int rc = locate(h, tube, field, &p, &count, &esz);
+    if (rc != VS_OK) return rc;
+    if (bytes < count * esz) return fail(VS_E_RANGE, "destination too small");
+    VS_HIP(hipStreamSynchronize(h->stream));
+    VS_HIP(hipMemcpy(dst, p, count * esz, hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+int vs_run(vs_handle* h, int n_steps) {
+    if (check_handle(h)) return VS_E_INVALID;
+    if (n_steps < 0) return fail(VS_E_INVALID, "n_steps must be >= 0");
+    return h->precision == 64 ? run_steps<double>(h, n_steps) : run_steps<float>(h, n_steps);
+}
+
+int vs_sync(vs_handle* h) {
+    if (check_handle(h)) return VS_E_INVALID;
+    VS_HIP(hipStreamSynchronize(h->stream));
+    float ms = 0;
+    VS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    return VS_OK;
+}
+
+int vs_steps_done(const vs_handle* h, int* steps) {
+    if (check_handle(h) || !steps) return fail(VS_E_INVALID, "vs_steps_done: bad argument");
+    *steps = h->steps_done;
+    return VS_OK;
+}
+
+int vs_reset(vs_handle* h) {
+    if (check_handle(h)) return VS_E_INVALID;
+    const size_t es = elem_size(h);
+    void* P = h->precision == 64 ? (void*)h->bd.P : (void*)h->bf.P;
+    void* Vx = h->precision == 64 ? (void*)h->bd.Vx : (void*)h->bf.Vx;
+    void* Vy = h->precision == 64 ? (void*)h->bd.Vy : (void*)h->bf.Vy;
+    VS_HIP(hipMemsetAsync(P, 0, h->total_cells * es, h->stream));
+    VS_HIP(hipMemsetAsync(Vx, 0, h->total_cells * es, h->stream));
+    VS_HIP(hipMemsetAsync(Vy, 0, h->total_cells * es, h->stream));
+    double* pexc = h->precision == 64 ? h->bd.pexc : h->bf.pexc;
+    VS_HIP(hipMemsetAsync(pexc, 0, h->n * sizeof(double), h->stream));
+    TwoMassState* tm = h->precision == 64 ? h->bd.tm : h->bf.tm;
+    VS_HIP(hipMemcpyAsync(tm, h->tm0.data(), h->n * sizeof(TwoMassState), hipMemcpyHostToDevice, h->stream));
+    VS_HIP(hipStreamSynchronize(h->stream));
+    h->steps_done = 0;
+    return VS_OK;
+}
+
+int vs_get_probe(vs_handle* h, int tube, double* dst, size_t n) {
+    if (check_handle(h) || !dst) return fail(VS_E_INVALID, "vs_get_probe: bad argument");
+    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+    const TubeDesc& d = h->desc[tube];
+    const size_t m = std::min<size_t>(n, (size_t)d.num_steps);
+    double* p = (h->precision == 64 ? h->bd.probe : h->bf.probe) + d.src_off;
+    VS_HIP(hipStreamSynchronize(h->stream));
+    VS_HIP(hipMemcpy(dst, p, m * sizeof(double), hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+int vs_get_source_trace(vs_handle* h, int tube, double* dst, size_t n) {
+    if (check_handle(h) || !dst) return fail(VS_E_INVALID, "vs_get_source_trace: bad argument");
+    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+    const TubeDesc& d = h->desc[tube];
+    const size_t m = std::min<size_t>(n, (size_t)d.num_steps);
+    double* p = (h->precision == 64 ? h->bd.strace : h->bf.strace) + d.src_off;
+    VS_HIP(hipStreamSynchronize(h->stream));
+    VS_HIP(hipMemcpy(dst, p, m * sizeof(double), hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+int vs_set_option(vs_handle* h, int option, int value) {
+    if (check_handle(h)) return VS_E_INVALID;
+    switch (option) {
+    case VS_OPT_STEPS_PER_LAUNCH:
+        if (value < 1) return fail(VS_E_INVALID, "steps per launch must be >= 1");
+        h->steps_per_launch = value; return VS_OK;
+    case VS_OPT_THREADS:
+        if (value < 64 || value > 1024 || value % 64) return fail(VS_E_INVALID, "threads must be a multiple of 64 in [64, 1024]");
+        h->threads = value; return VS_OK;
+    case VS_OPT_USE_GRAPH: h->use_graph = value ? 1 : 0; return VS_OK;
+    case VS_OPT_KERNEL:
+        if (value != VS_K_EXACT && value != VS_K_FAST) return fail(VS_E_INVALID, "unknown kernel family");
+        h->kernel = value; return VS_OK;
+    case VS_OPT_TIMING: h->timing = value ? 1 : 0; return VS_OK;
+    default: return fail(VS_E_INVALID, "unknown option");
+    }
+}
+
+int vs_get_option(const vs_handle* h, int option, int* value) {
+    if (check_handle(h) || !value) return fail(VS_E_INVALID, "vs_get_option: bad argument");
+    switch (option) {
+    case VS_OPT_STEPS_PER_LAUNCH: *value = h->steps_per_launch; return VS_OK;
+    case VS_OPT_THREADS: *value = h->threads; return VS_OK;
+    case VS_OPT_USE_GRAPH: *value = h->use_graph; return VS_OK;
+    case VS_OPT_KERNEL: *value = h->kernel; return VS_OK;
+    case VS_OPT_TIMING: *value = h->timing; return VS_OK;
+    default: return fail(VS_E_INVALID, "unknown option");
+    }
+}
+
+int vs_stream(vs_handle* h, void** stream) {
+    if (check_handle(h) || !stream) return fail(VS_E_INVALID, "vs_stream: bad argument");
+    *stream = (void*)h->stream;
+    return VS_OK;
+}
+
+int vs_last_run_timing(const vs_handle* h, double* total_ms, int* launches, double* mean_launch_ms) {
+    if (check_handle(h)) return VS_E_INVALID;
+    if (total_ms) *total_ms = h->last_ms;
+    if (launches) *launches = h->last_launches;
+    if (mean_launch_ms) *mean_launch_ms = h->last_mean_launch_ms;
+    return VS_OK;
+}
+
+int vs_cells_per_step(const vs_handle* h, long long* cells) {
+    if (check_handle(h) || !cells) return fail(VS_E_INVALID, "vs_cells_per_step: bad argument");
+    *cells = h->cells_per_step;
+    return VS_OK;
+}
+
+}  // extern "C"

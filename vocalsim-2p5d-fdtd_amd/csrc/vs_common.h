@@ -1,0 +1,197 @@
+// vs_common.h -- types shared by the host setup code and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace vs {
+
+// Cell classes: the reference's CellTypes enum (include/SolverParams.h:21-35).
+enum CellType : int {
+    kWall = 0, kAir = 1, kExcitation = 2, kPml0 = 3, kPml1, kPml2, kPml3, kPml4, kPml5,
+    kDynamic = 9, kDead = 10, kNoPressure = 11, kHead = 12, kNumTypes = 13
+};
+
+// Class byte stored per cell in HBM: low nibble = CellType, bit 4 = the cell
+// carries a boundary segment id (z_inv applies, FDTDSolver.cu:1068).
+constexpr uint8_t kTypeMask = 0x0f;
+constexpr uint8_t kSegWallBit = 0x10;
+
+// Status bits raised by device code (read back after setup / each run).
+constexpr unsigned kStatusSqrtDomain = 1u;   // negative depth sqrt argument
+constexpr unsigned kStatusNonFinite = 2u;    // non-finite probe sample
+
+// Physics constants and per-type tables of one tube, stored in double.
+// FP32 tubes hold the float32-evaluated values widened exactly.
+struct PhysTables {
+    double dt, ds, rho, c, K1, K2, rds;   // rds = rho * ds
+    double beta[16];
+    double sigma[16];
+    double air_normal[4];
+    double wall_normal[4];
+    double src_dir[4];
+};
+
+// Device descriptor of one tube inside a batch.
+struct TubeDesc {
+    int H, W, L, nsec;
+    int ts, te, midY;
+    int listener, excitation;     // flat indices inside the tube frame
+    int num_steps;
+    int source_type, dirichlet, variant, srate_mul;
+    int shape, solver_type;
+    float ds_f32;
+    int pml;                      // PML rings painted (pml_flag)
+    long long cell_off;           // start of the tube's H*W block
+    long long col_off;            // start of its W-long column arrays
+    long long wall_off;           // start of its L-long tube-column arrays
+    long long sec_off;            // start of its section arrays
+    long long src_off;            // start of its source table / traces (num_steps)
+    PhysTables ph;
+};
+
+// Two-mass vocal-fold state (Ishizaka-Flanagan, TwoMassModel.cpp:4-231),
+// evaluated in double on the device: one lane per tube per step.
+struct TwoMassState {
+    double mu, rho, q, gs, m1, m2, d1, d2, k1, k2, kc, etak1, etak2, h1, h2,
+        etah1, etah2, Ag01, Ag02, lg, r1_open, r2_open, r1_close, r2_close, A1,
+        ps, p1, ug_old, ug_curr, ug_next, x1_old, x1_curr, x2_old, x2_curr;
+};
+
+// Parameter set-up (TwoMassModel.cpp:4-87).  The float sub-expressions of the
+// reference (units are float32 constants) are kept as float.
+inline void twomass_init(TwoMassState& s, double q, double ps) {
+    const float CM = 1e-2f, GRAM = 1e-3f, DYN = 1e-5f, KG = 1.0f, M = 1.0f, SEC = 1.0f;
+    s.mu = 1.86e-4 * (DYN * SEC / (CM * CM));
+    s.rho = 1.14 * (KG / (M * M * M));
+    s.q = q;
+    s.gs = 1.0f;
+    s.m1 = (0.125f * GRAM) / s.q;
+    s.m2 = (0.025f * GRAM) / s.q;
+    s.d1 = (0.25f * CM) / s.q;
+    s.d2 = (0.05f * CM) / s.q;
+    s.k1 = 80000.0f * (DYN / CM) * s.q;
+    s.k2 = 8000.0f * (DYN / CM) * s.q;
+    s.kc = 25000.0f * (DYN / CM) * (s.q * s.q);
+    s.etak1 = 100 / (CM * CM);
+    s.etak2 = 100 / (CM * CM);
+    s.h1 = 3 * s.k1;
+    s.h2 = 3 * s.k2;
+    s.etah1 = 500.0f / (CM * CM);
+    s.etah2 = 500.0f / (CM * CM);
+    s.Ag01 = 0.05f * (CM * CM);
+    s.Ag02 = s.Ag01;
+    s.lg = 1.40f * CM;
+    const double z1o = 0.2f, z2o = 0.6f, z1c = 1.1f, z2c = 1.9f;
+    s.r1_open = (2 * z1o * __builtin_sqrt(s.k1 * s.m1)) / (s.gs * s.gs);
+    s.r2_open = (2 * z2o * __builtin_sqrt(s.k2 * s.m2)) / (s.gs * s.gs);
+    s.r1_close = (2 * z1c * __builtin_sqrt(s.k1 * s.m1)) / (s.gs * s.gs);
+    s.r2_close = (2 * z2c * __builtin_sqrt(s.k2 * s.m2)) / (s.gs * s.gs);
+    s.A1 = 3.0f * (CM * CM);
+    s.ps = ps;
+    s.p1 = 0;
+    s.ug_old = s.ug_curr = s.ug_next = 0;
+    s.x1_old = s.x1_curr = s.x2_old = s.x2_curr = 0;
+}
+
+// One step of the recurrence (TwoMassModel.cpp:90-231).  Rounds every
+// operation separately (no contraction) so host and device agree bitwise.
+__host__ __device__ inline double twomass_step(TwoMassState& s, int srate_mul) {
+#pragma clang fp contract(off)
+    const double sr = 44100 * static_cast<double>(srate_mul);
+    const double lg = s.lg, rho = s.rho, mu = s.mu, d1 = s.d1, d2 = s.d2;
+    const double x1 = s.x1_curr, x2 = s.x2_curr, ug = s.ug_curr;
+    double Ag1 = s.Ag01 + 2 * lg * x1;
+    double Ag2 = s.Ag02 + 2 * lg * x2;
+    const double Rv1 = (12 * mu * lg * lg * d1) / (Ag1 * Ag1 * Ag1);
+    const double Lg1 = (rho * d1) / (Ag1);
+    const double Rv2 = (12 * mu * lg * lg * d2) / (Ag2 * Ag2 * Ag2);
+    const double Lg2 = (rho * d2) / (Ag2);
+    const bool open1 = x1 > -s.Ag01 / (2 * lg);
+    const bool open2 = x2 > -s.Ag02 / (2 * lg);
+    double pm1;
+    if (open1 && open2)
+        pm1 = s.ps - 1.37 * (rho / 2) * (ug / Ag1) * (ug / Ag1) - 0.5 * (Rv1 * ug + Lg1 * sr * (ug - s.ug_old));
+    else
+        pm1 = s.ps;
+    const double f1 = pm1 * d1 * lg;
+    double pm2;
+    if (open1) {
+        if (open2)
+            pm2 = pm1 - (0.5 * (Rv1 + Rv2) * ug + (Lg1 + Lg2) * (ug - s.ug_old) * sr)
+                  - (rho / 2 * ug * ug) * (1 / (Ag2 * Ag2) - 1 / (Ag1 * Ag1));
+        else
+            pm2 = s.ps;
+    } else {
+        pm2 = 0;
+    }
+    const double f2 = pm2 * d2 * lg;
+    double h1u, r1u, h2u, r2u;
+    if (x1 + s.Ag01 / (2 * lg) < 0) { h1u = s.h1; r1u = s.r1_close; } else { h1u = 0.0f; r1u = s.r1_open; }
+    if (x2 + s.Ag02 / (2 * lg) < 0) { h2u = s.h2; r2u = s.r2_close; } else { h2u = 0.0f; r2u = s.r2_open; }
+    const double a11 = (s.k1 + h1u + s.kc) / (sr * sr) + r1u / sr + s.m1;
+    const double a12 = -s.kc / (sr * sr);
+    const double a21 = a12;
+    const double a22 = (s.k2 + h2u + s.kc) / (sr * sr) + r2u / sr + s.m2;
+    const double g1 = s.Ag01 / (2 * lg), g2 = s.Ag02 / (2 * lg);
+    const double s1p = s.k1 * s.etak1 * x1 * x1 * x1 + h1u * (g1 + s.etah1 * (g1 + x1) * (g1 + x1) * (g1 + x1));
+    const double s2p = s.k2 * s.etak2 * x2 * x2 * x2 + h2u * (g2 + s.etah2 * (g2 + x2) * (g2 + x2) * (g2 + x2));
+    const double b1 = (2 * s.m1 + r1u / sr) * x1 - s.m1 * s.x1_old - s1p / (sr * sr) + f1 / (sr * sr);
+    const double b2 = (2 * s.m2 + r2u / sr) * x2 - s.m2 * s.x2_old - s2p / (sr * sr) + f2 / (sr * sr);
+    double det = a11 * a22 - a21 * a12;
+    if (det == 0) det = 1;
+    const double x1n = (a22 * b1 - a12 * b2) / det;
+    const double x2n = (a11 * b2 - a21 * b1) / det;
+    s.x1_old = x1; s.x1_curr = x1n;
+    s.x2_old = x2; s.x2_curr = x2n;
+    Ag1 = s.Ag01 + 2 * lg * x1n;
+    Ag2 = s.Ag02 + 2 * lg * x2n;
+    double ugn;
+    if (Ag1 < 0 || Ag2 < 0) {
+        ugn = 0;
+    } else {
+        const double Rtot = (rho / 2) * __builtin_fabs(ug) * ((0.37 / (Ag1 * Ag1)) + (1 - 2 * (Ag2 / s.A1) * (1 - Ag2 / s.A1)) / (Ag2 * Ag2))
+                            + (12 * mu * lg * lg * (d1 / (Ag1 * Ag1 * Ag1) + d2 / (Ag2 * Ag2 * Ag2)));
+        const double Ltot = rho * (d1 / Ag1 + d2 / Ag2);
+        ugn = ((s.ps - s.p1) / sr + Ltot * ug) / (Rtot / sr + Ltot);
+    }
+    s.ug_old = ug;
+    s.ug_curr = ugn;
+    s.ug_next = ugn;
+    return ugn;
+}
+
+// Device buffers of a batch (packed over tubes).
+template <typename Real>
+struct Bufs {
+    Real* P; Real* Vx; Real* Vy;
+    Real* dP; Real* dX; Real* dY;
+    Real* zcol;          // z_inv of the wall cells of each column
+    uint8_t* cls;        // class byte per cell
+    int* bst;            // boundary segment id per cell (setup / export only)
+    int* upper; int* lower; int* seg;   // per tube column
+    float* cum;          // per section: float32 cumulative length [m]
+    int* diam;           // per section: diameter in cells
+    Real* minor;         // per section: ellipse minor axis [m]
+    Real* dscale;        // per section: depth scale
+    Real* zsec;          // per section: z_inv
+    Real* source;        // per step: precomputed excitation
+    double* probe;       // per step: listener pressure
+    double* strace;      // per step: injected source velocity
+    TwoMassState* tm;    // per tube
+    double* pexc;        // per tube: P at the excitation cell after the last step
+    unsigned* status;    // device status word
+};
+
+// Kernel launchers (vs_kernels.hip).
+template <typename Real>
+hipError_t launch_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<Real> b,
+                        Real* tmpA, Real* tmpB, Real* tmpC, Real* tmpD, hipStream_t st);
+template <typename Real>
+hipError_t launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<Real> b,
+                       int t0, int nsteps, int threads, int kernel, hipStream_t st);
+template <typename Real>
+hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tube, Bufs<Real> b,
+                         Real* out17, hipStream_t st);
+
+}  // namespace vs

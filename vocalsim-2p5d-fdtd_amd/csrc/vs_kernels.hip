@@ -1,0 +1,444 @@
+// vs_kernels.hip -- HIP kernels for gfx950: setup (walls, cell classes,
+// depth maps) and the FDTD time step.
+//
+// Every kernel here rounds each floating-point operation separately
+// (-ffp-contract=off for the translation unit) so that the FP64 path
+// reproduces the oracle's operation order bit for bit.
+
+#include "vs_common.h"
+
+namespace vs {
+
+namespace {
+
+constexpr int kSetupThreads = 256;
+
+__device__ inline float sqrt_r(float x) { return __builtin_sqrtf(x); }
+__device__ inline double sqrt_r(double x) { return __builtin_sqrt(x); }
+
+// ---------------------------------------------------------------------------
+// Setup 1: wall rows per tube column.  One lane per column: segment lookup
+// over the float32 cumulative lengths, then the wall rows mid -/+ (r + 1)
+// (TubeGeometry.cu:443-484).  Also fills the per-column z_inv of the walls.
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void k_walls(const TubeDesc* __restrict__ tubes, Bufs<Real> b) {
+    const TubeDesc& T = tubes[blockIdx.y];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= T.L) return;
+    const float ds = T.ds_f32;
+    const float curr_len = (idx + 1) * ds;
+    const float half = ds * 0.5f;
+    const float* cum = b.cum + T.sec_off;
+    int k = 0;
+    while (k < T.nsec - 1 && curr_len > cum[k] + half) ++k;
+    const int radius = (b.diam[T.sec_off + k] - 1) / 2;
+    b.upper[T.wall_off + idx] = T.midY - radius - 1;
+    b.lower[T.wall_off + idx] = T.midY + radius + 1;
+    b.seg[T.wall_off + idx] = k;
+    b.zcol[T.col_off + T.ts + idx] = b.zsec[T.sec_off + k];
+}
+
+// ---------------------------------------------------------------------------
+// Setup 2: cell class of every cell in closed form.  Reproduces the result
+// of the reference's ordered writes: walls and gap fill
+// (TubeGeometry.cu:433-569), excitation column (:571-594), Dirichlet column
+// (:596-612), dead ring (FDTDSolver.cu:816-835) and PML rings painted
+// horizontally then vertically (FDTDSolver.cu:371-464), whose net effect is
+// pml5 - min(ring_row, ring_col).
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void k_classify(const TubeDesc* __restrict__ tubes, Bufs<Real> b) {
+    const TubeDesc& T = tubes[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int H = T.H, W = T.W;
+    if (i >= H * W) return;
+    const int r = i / W, c = i - r * W;
+    const int* up = b.upper + T.wall_off;
+    const int* lo = b.lower + T.wall_off;
+    const int* sg = b.seg + T.wall_off;
+    const int L = T.L;
+    int type = kAir, bst = 0;
+    if (c >= T.ts && c <= T.te) {
+        const int j = c - T.ts;
+        const int uj = up[j], lj = lo[j];
+        if (r == uj || r == lj) { type = kWall; bst = sg[j] + 1; }
+        // gap fill while this column is the current one
+        if (j >= 1 && j <= L - 2) {
+            const int d = uj - up[j - 1];
+            if (d > 1 && ((r >= up[j - 1] && r <= uj) || (r >= lj && r <= lo[j - 1]))) {
+                type = kWall; bst = sg[j] + 1;
+            }
+        }
+        // gap fill while this column is the previous one
+        if (j + 1 >= 1 && j + 1 <= L - 2) {
+            const int d = up[j + 1] - uj;
+            if (d < -1 && ((r >= up[j + 1] && r <= uj) || (r >= lj && r <= lo[j + 1]))) {
+                type = kWall; bst = sg[j] + 1;
+            }
+        }
+    } else if (c == T.ts - 1) {
+        if (r > up[0] && r < lo[0]) type = kExcitation;
+        else if (r == up[0] || r == lo[0]) type = kWall;
+    } else if (T.dirichlet && c == T.te + 1) {
+        if (r >= up[L - 1] && r <= lo[L - 1]) type = kNoPressure;
+    }
+    if (r == 0 || r == H - 1 || c == 0 || c == W - 1) {
+        type = kDead;
+    } else if (T.pml) {
+        const int kr = min(r - 1, H - 2 - r), kc = min(c - 1, W - 2 - c);
+        const int k = min(kr, kc);
+        if (k <= 5) type = kPml5 - k;
+    }
+    b.cls[T.cell_off + i] = (uint8_t)(type | (bst != 0 ? kSegWallBit : 0));
+    b.bst[T.cell_off + i] = bst;
+}
+
+// ---------------------------------------------------------------------------
+// Setup 3: raw depth maps (TubeDepth.cu:313-490) in closed form per cell.
+// Rows above the centre take the "upper" branch, rows below the "lower"
+// branch; the centre row keeps the lower branch's depthY, which the
+// reference writes last.  Cells outside the tube keep the open-space value
+// 0.5 (TubeDepth.cu:108-110).
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void k_depth_raw(const TubeDesc* __restrict__ tubes, Bufs<Real> b, Real* X, Real* Y,
+                            long long tmp_stride) {
+    const TubeDesc& T = tubes[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int H = T.H, W = T.W;
+    if (i >= H * W) return;
+    X += (long long)blockIdx.y * tmp_stride;
+    Y += (long long)blockIdx.y * tmp_stride;
+    const int r = i / W, c = i - r * W;
+    Real x = (Real)(1.0 * (Real)0.50f), y = x;
+    if (c >= T.ts && c <= T.te) {
+        const int j = c - T.ts;
+        const int up = b.upper[T.wall_off + j], lo = b.lower[T.wall_off + j];
+        if (r == up || r == lo) {
+            x = 0; y = 0;
+        } else if (r > up && r < lo) {
+            const Real ds = (Real)T.ph.ds;
+            const int air = lo - up - 1;
+            const int half = (int)ceilf(air * 0.5f) - 1;
+            const int mid = up + half + 1;
+            const int sec = b.seg[T.wall_off + j];
+            const Real scale = b.dscale[T.sec_off + sec];
+            int k;
+            bool upper_side;
+            if (r < mid) { k = up + half + 1 - r; upper_side = true; }
+            else { k = r - mid; upper_side = false; }
+            const Real d = static_cast<Real>(k) * ds;
+            const Real dy = upper_side ? (static_cast<Real>(k) + 0.5f) * ds : (static_cast<Real>(k) - 0.5f) * ds;
+            Real sx, sy;
+            if (T.shape == 1) {
+                const Real radius = (static_cast<Real>(air) / 2.0f) * ds;
+                const Real r2 = radius * radius;
+                sx = r2 - d * d;
+                sy = (upper_side && k == half) ? (Real)0.0f : r2 - dy * dy;
+            } else {
+                const Real a = (air * ds) / 2.0f;
+                const Real bb = b.minor[T.sec_off + sec] / 2.0f;
+                const Real sa2 = a * a, sb2 = bb * bb;
+                sx = sb2 * (1.0f - ((d * d) / sa2));
+                sy = (upper_side && k == half) ? (Real)0.0f : sb2 * (1 - ((dy * dy) / sa2));
+            }
+            if (sx < 0 || sy < 0) {
+                atomicOr(b.status, kStatusSqrtDomain);
+                sx = sx < 0 ? (Real)0 : sx;
+                sy = sy < 0 ? (Real)0 : sy;
+            }
+            x = 2 * sqrt_r(sx) * scale;
+            y = 2 * sqrt_r(sy) * scale;
+        }
+    }
+    X[i] = x;
+    Y[i] = y;
+}
+
+// Setup 4: open-space reset + half-cell smoothing (TubeDepth.cu:642-712).
+template <typename Real>
+__global__ void k_depth_smooth(const TubeDesc* __restrict__ tubes, const Real* X, const Real* Y,
+                               Real* Xs, Real* Ys, long long tmp_stride) {
+    const TubeDesc& T = tubes[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int H = T.H, W = T.W;
+    if (i >= H * W) return;
+    const long long o = (long long)blockIdx.y * tmp_stride;
+    X += o; Y += o; Xs += o; Ys += o;
+    const int r = i / W, c = i - r * W;
+    const int mid = T.midY * W + T.te;
+    const Real open = (Real)0.50f;
+    const Real mx = X[mid], my = Y[mid];
+    Real xc = X[i]; if (xc == open) xc = mx;
+    Real yc = Y[i]; if (yc == open) yc = my;
+    Real xr = mx;
+    if (c < W - 1) { xr = X[i + 1]; if (xr == open) xr = mx; }
+    Real ya = my;
+    if (r > 0) { ya = Y[i - W]; if (ya == open) ya = my; }
+    Xs[i] = 0.5f * (xc + xr);
+    Ys[i] = 0.5f * (yc + ya);
+}
+
+// Setup 5: depthP average and min-depth clamp (TubeDepth.cu:714-762).
+template <typename Real>
+__global__ void k_depth_final(const TubeDesc* __restrict__ tubes, Bufs<Real> b, const Real* Xs,
+                              const Real* Ys, long long tmp_stride) {
+    const TubeDesc& T = tubes[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int H = T.H, W = T.W;
+    if (i >= H * W) return;
+    const long long o = (long long)blockIdx.y * tmp_stride;
+    Xs += o; Ys += o;
+    const int r = i / W, c = i - r * W;
+    const int mid = T.midY * W + T.te;
+    const Real cX = Xs[i], cY = Ys[i];
+    const Real lX = (c > 0) ? Xs[i - 1] : Xs[mid];
+    const Real dY = (r < H - 1) ? Ys[i + W] : Ys[mid];
+    Real p = 0.25f * (cX + cY + lX + dY);
+    const Real mn = (Real)0.001f;
+    Real x = cX, y = cY;
+    if (p < mn) p = mn;
+    if (x < mn) x = mn;
+    if (y < mn) y = mn;
+    if (T.solver_type == 0) { p = 1.0; x = 1.0; y = 1.0; }   // 2D: unit depth (FDTDSolver.cu:632-635)
+    b.dP[T.cell_off + i] = p;
+    b.dX[T.cell_off + i] = x;
+    b.dY[T.cell_off + i] = y;
+}
+
+// ---------------------------------------------------------------------------
+// Per-face coefficients of a (cur, right, top) type triple, in the reference's
+// arithmetic (FDTDSolver.cu:860-1056).
+// ---------------------------------------------------------------------------
+template <typename Real>
+struct Tables {
+    Real beta[16], sigma[16], airN[4], wallN[4], dir[4];
+    Real dt, rds, K1;
+};
+
+template <typename Real>
+__device__ inline void coeffs_of(const Tables<Real>& t, int tc, int tr, int tt, Real* o) {
+    const Real bc = t.beta[tc], br = t.beta[tr], bt = t.beta[tt];
+    const int ibr = (int)br, ibt = (int)bt;
+    const int lut = (ibr == kAir && ibt == kAir) ? 0 : (ibr == kAir && ibt == kWall) ? 1
+                  : (ibr == kWall && ibt == kAir) ? 2 : 3;
+    const Real sc = t.sigma[tc], sr = t.sigma[tr], st = t.sigma[tt];
+    const int ec = tc == kExcitation, er = tr == kExcitation, et = tt == kExcitation;
+    const Real mbx = (br < bc) ? br : bc;
+    const Real mby = (bt < bc) ? bt : bc;
+    o[0] = t.airN[lut] * bc;
+    o[1] = t.wallN[lut] * (1 - bc);
+    o[2] = mbx;
+    o[3] = mby;
+    o[4] = (sc < sr) ? sr : sc;
+    o[5] = (sc < st) ? st : sc;
+    o[6] = (mbx * mbx * t.dt) / t.rds;
+    o[7] = (mby * mby * t.dt) / t.rds;
+    o[8] = sc;
+    o[9] = (Real)((1 - ec) * (1 - er));
+    o[10] = (Real)((1 - ec) * (1 - et));
+    o[11] = (ec * t.dir[2]) + (er * t.dir[0]);
+    o[12] = (ec * t.dir[3]) + (et * t.dir[1]);
+    o[13] = br * (1 - bc);
+    o[14] = bc * (1 - br);
+    o[15] = bt * (1 - bc);
+    o[16] = bc * (1 - bt);
+}
+
+template <typename Real>
+__device__ inline void load_tables(const TubeDesc& T, Tables<Real>& t) {
+    for (int k = 0; k < 16; ++k) { t.beta[k] = (Real)T.ph.beta[k]; t.sigma[k] = (Real)T.ph.sigma[k]; }
+    for (int k = 0; k < 4; ++k) {
+        t.airN[k] = (Real)T.ph.air_normal[k];
+        t.wallN[k] = (Real)T.ph.wall_normal[k];
+        t.dir[k] = (Real)T.ph.src_dir[k];
+    }
+    t.dt = (Real)T.ph.dt;
+    t.rds = (Real)T.ph.rds;
+    t.K1 = (Real)T.ph.K1;
+}
+
+// Materialises the 17 interior coefficient grids of one tube (export only:
+// the step kernel derives the same values from the class bytes on the fly).
+template <typename Real>
+__global__ void k_coeffs(const TubeDesc* __restrict__ tubes, int tube, Bufs<Real> b, Real* out) {
+    const TubeDesc& T = tubes[tube];
+    const int H = T.H, W = T.W, Wi = W - 2;
+    const int n = (H - 2) * Wi;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Tables<Real> t;
+    load_tables(T, t);
+    const int r = 1 + i / Wi, c = 1 + i % Wi;
+    const uint8_t* cls = b.cls + T.cell_off;
+    const int q = r * W + c;
+    Real o[17];
+    coeffs_of(t, cls[q] & kTypeMask, cls[q + 1] & kTypeMask, cls[q - W] & kTypeMask, o);
+    for (int k = 0; k < 17; ++k) out[(size_t)k * n + i] = o[k];
+}
+
+// ---------------------------------------------------------------------------
+// Time step, reference operation order.  One work-group owns one tube for a
+// block of steps; P and V are updated in place with a work-group barrier
+// between the pressure and velocity phases (a cell's pressure update reads
+// only its own P, a cell's velocity update only its own V, so in-place
+// updates equal the reference's double buffer).  Lane 0 runs the source
+// (table lookup or the two-mass recurrence in double, fed by the previous
+// step's P at the excitation cell) and records the probe.
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict__ tubes,
+                                                     const int* __restrict__ order, Bufs<Real> b,
+                                                     int t0, int nsteps) {
+    const int tube = order[blockIdx.x];
+    const TubeDesc& T = tubes[tube];
+    __shared__ Tables<Real> tab;
+    __shared__ Real s_exeV;
+    const int tid = threadIdx.x, B = blockDim.x;
+    if (tid == 0) load_tables(T, tab);
+    const int H = T.H, W = T.W, Wi = W - 2, NI = (H - 2) * Wi;
+    const int tend = min(t0 + nsteps, T.num_steps);
+    Real* __restrict__ P = b.P + T.cell_off;
+    Real* __restrict__ Vx = b.Vx + T.cell_off;
+    Real* __restrict__ Vy = b.Vy + T.cell_off;
+    const Real* __restrict__ dP = b.dP + T.cell_off;
+    const Real* __restrict__ dX = b.dX + T.cell_off;
+    const Real* __restrict__ dY = b.dY + T.cell_off;
+    const uint8_t* __restrict__ cls = b.cls + T.cell_off;
+    const Real* __restrict__ zc = b.zcol + T.col_off;
+    const Real* __restrict__ src = b.source + T.src_off;
+    double* probe = b.probe + T.src_off;
+    double* strace = b.strace + T.src_off;
+    const bool twomass = T.source_type == 2;
+    const bool dirichlet = T.dirichlet != 0;
+    const bool eigen = T.variant == 1;
+    TwoMassState tm;
+    double pexc = 0;
+    if (tid == 0) {
+        pexc = b.pexc[tube];
+        if (twomass) tm = b.tm[tube];
+    }
+    __syncthreads();
+    const Real K1 = tab.K1;
+    // per-thread walk over interior cells i = tid + k * B in row-major order
+    const int dr = B / Wi, dc = B - dr * Wi;
+    const int r0 = tid / Wi, c0 = tid - r0 * Wi;
+    for (int t = t0; t < tend; ++t) {
+        if (tid == 0) {
+            double e;
+            if (twomass) { tm.p1 = pexc; e = twomass_step(tm, T.srate_mul); }
+            else e = (double)src[t];
+            s_exeV = (Real)e;
+            strace[t] = (double)(Real)e;
+        }
+        // ---- pressure phase (FDTDEngineCPU.cpp:316-339) ----
+        {
+            int r = r0 + 1, c = c0 + 1;
+            for (int i = tid; i < NI; i += B) {
+                const int q = r * W + c;
+                const Real cxvx = Vx[q] * dX[q] - Vx[q - 1] * dX[q - 1];
+                const Real cyvy = Vy[q] * dY[q] - Vy[q + W] * dY[q + W];
+                const Real num = (P[q] * dP[q]) - (K1 * (cxvx + cyvy));
+                const int tc = cls[q] & kTypeMask;
+                const Real sp = tab.sigma[tc];
+                const Real den = eigen ? (sp * dP[q] + dP[q]) : ((1 + sp) * dP[q]);
+                Real v = num / den;
+                if (dirichlet && tc == kNoPressure) v = 0;
+                P[q] = v;
+                r += dr; c += dc;
+                if (c > Wi) { c -= Wi; ++r; }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            probe[t] = (double)P[T.listener];
+            pexc = (double)P[T.excitation];
+        }
+        const Real ev = s_exeV;
+        // ---- velocity phase (FDTDEngineCPU.cpp:364-410) ----
+        {
+            int r = r0 + 1, c = c0 + 1;
+            for (int i = tid; i < NI; i += B) {
+                const int q = r * W + c;
+                const uint8_t cc = cls[q];
+                Real o[17];
+                coeffs_of(tab, cc & kTypeMask, cls[q + 1] & kTypeMask, cls[q - W] & kTypeMask, o);
+                const Real z = (cc & kSegWallBit) ? zc[c] : (Real)0;
+                const Real pc = P[q], pr = P[q + 1], pt = P[q - W];
+                const Real cxp = pr - pc;
+                const Real cyp = pt - pc;
+                Real vx = (o[2] * Vx[q]) - (o[6] * cxp);
+                Real vy = (o[3] * Vy[q]) - (o[7] * cyp);
+                vx = vx + (ev * o[11] * o[4]);
+                vy = vy + (ev * o[12] * o[5]);
+                Real bx = (o[14] * pc * o[0]) - (o[13] * pr * o[1]);
+                Real by = (o[16] * pc * o[0]) - (o[15] * pt * o[1]);
+                bx = bx * o[9] * z;
+                by = by * o[10] * z;
+                vx += (o[4] * bx);
+                vy += (o[5] * by);
+                vx /= (o[2] + o[4]);
+                vy /= (o[3] + o[5]);
+                Vx[q] = vx;
+                Vy[q] = vy;
+                r += dr; c += dc;
+                if (c > Wi) { c -= Wi; ++r; }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        b.pexc[tube] = pexc;
+        if (twomass) b.tm[tube] = tm;
+    }
+}
+
+}  // namespace
+
+template <typename Real>
+hipError_t launch_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<Real> b,
+                        Real* tmpA, Real* tmpB, Real* tmpC, Real* tmpD, hipStream_t st) {
+    int maxL = 0, maxHW = 0;
+    for (int k = 0; k < n; ++k) {
+        maxL = max(maxL, h_tubes[k].L);
+        maxHW = max(maxHW, h_tubes[k].H * h_tubes[k].W);
+    }
+    const long long stride = maxHW;   // raw / smoothed depth scratch: one slice per tube
+    dim3 gw((maxL + 63) / 64, n), gc((maxHW + kSetupThreads - 1) / kSetupThreads, n);
+    hipLaunchKernelGGL(k_walls<Real>, gw, dim3(64), 0, st, d_tubes, b);
+    hipLaunchKernelGGL(k_classify<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, b);
+    hipLaunchKernelGGL(k_depth_raw<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, b, tmpA, tmpB, stride);
+    hipLaunchKernelGGL(k_depth_smooth<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, tmpA, tmpB, tmpC, tmpD, stride);
+    hipLaunchKernelGGL(k_depth_final<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, b, tmpC, tmpD, stride);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<Real> b,
+                       int t0, int nsteps, int threads, int kernel, hipStream_t st) {
+    (void)kernel;
+    hipLaunchKernelGGL(k_step_exact<Real>, dim3(n), dim3(threads), 0, st, d_tubes, d_order, b, t0, nsteps);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tube, Bufs<Real> b,
+                         Real* out17, hipStream_t st) {
+    const int n = (h_tube->H - 2) * (h_tube->W - 2);
+    hipLaunchKernelGGL(k_coeffs<Real>, dim3((n + 255) / 256), dim3(256), 0, st, d_tubes, tube, b, out17);
+    return hipGetLastError();
+}
+
+#define VS_INSTANTIATE(Real)                                                                        \
+    template hipError_t launch_setup<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>, Real*, \
+                                           Real*, Real*, Real*, hipStream_t);                       \
+    template hipError_t launch_step<Real>(const TubeDesc*, const int*, int, Bufs<Real>, int, int,    \
+                                          int, int, hipStream_t);                                   \
+    template hipError_t launch_coeffs<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,       \
+                                            Real*, hipStream_t);
+
+VS_INSTANTIATE(float)
+VS_INSTANTIATE(double)
+
+}  // namespace vs

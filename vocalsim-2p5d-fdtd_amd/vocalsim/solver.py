@@ -1,0 +1,144 @@
+"""Batch solver handle over the C ABI (one handle = n >= 1 tubes on the GPU)."""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, load
+
+
+class Solver:
+    """Owns a vs_handle: a batch of tubes with their device state.
+
+    ``precision`` is 64 (the plain FP64 path) or 32 (the batched FP32 path).
+    Setup (walls, cell classes, depth maps, z_inv, sources) runs on the GPU in
+    the constructor.
+    """
+
+    def __init__(self, specs, precision: int = 64, threads: int = 256,
+                 steps_per_launch: int = 256, kernel: int = _abi.K_EXACT):
+        if not isinstance(specs, (list, tuple)):
+            specs = [specs]
+        L = load()
+        arr = (_abi.TubeSpecC * len(specs))()
+        self._keep = []
+        for i, s in enumerate(specs):
+            c, keep = _abi.spec_to_c(s)
+            arr[i] = c
+            self._keep.append(keep)
+        h = ctypes.c_void_p()
+        check(L.vs_create(arr, len(specs), precision, ctypes.byref(h)))
+        self.h = h
+        self.n = len(specs)
+        self.precision = precision
+        self.dims: List[Dict[str, int]] = []
+        for i in range(self.n):
+            d = _abi.Dims()
+            check(L.vs_get_dims(self.h, i, ctypes.byref(d)))
+            self.dims.append(d.as_dict())
+        self.set_option(_abi.OPT_THREADS, threads)
+        self.set_option(_abi.OPT_STEPS_PER_LAUNCH, steps_per_launch)
+        self.set_option(_abi.OPT_KERNEL, kernel)
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            load().vs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- options -----------------------------------------------------------
+    def set_option(self, opt: int, value: int):
+        check(load().vs_set_option(self.h, opt, int(value)))
+
+    def get_option(self, opt: int) -> int:
+        v = ctypes.c_int()
+        check(load().vs_get_option(self.h, opt, ctypes.byref(v)))
+        return v.value
+
+    # -- stepping ----------------------------------------------------------
+    def run(self, n_steps: int = None, sync: bool = True):
+        if n_steps is None:
+            n_steps = max(d["num_steps"] for d in self.dims)
+        check(load().vs_run(self.h, int(n_steps)))
+        if sync:
+            self.sync()
+
+    def sync(self):
+        check(load().vs_sync(self.h))
+
+    def reset(self):
+        check(load().vs_reset(self.h))
+
+    @property
+    def steps_done(self) -> int:
+        v = ctypes.c_int()
+        check(load().vs_steps_done(self.h, ctypes.byref(v)))
+        return v.value
+
+    def timing(self):
+        ms, mean = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int()
+        check(load().vs_last_run_timing(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(mean)))
+        return ms.value, n.value, mean.value
+
+    @property
+    def cells_per_step(self) -> int:
+        v = ctypes.c_longlong()
+        check(load().vs_cells_per_step(self.h, ctypes.byref(v)))
+        return v.value
+
+    # -- outputs -----------------------------------------------------------
+    def probe(self, tube: int = 0) -> np.ndarray:
+        n = self.dims[tube]["num_steps"]
+        out = np.zeros(n, dtype=np.float64)
+        check(load().vs_get_probe(self.h, tube, out.ctypes.data_as(_abi._D), n))
+        return out[: min(n, self.steps_done)]
+
+    def source_trace(self, tube: int = 0) -> np.ndarray:
+        n = self.dims[tube]["num_steps"]
+        out = np.zeros(n, dtype=np.float64)
+        check(load().vs_get_source_trace(self.h, tube, out.ctypes.data_as(_abi._D), n))
+        return out[: min(n, self.steps_done)]
+
+    def field(self, name: str, tube: int = 0) -> np.ndarray:
+        d = self.dims[tube]
+        H, W, L = d["frameH"], d["frameW"], d["tube_len_cells"]
+        real = np.float64 if self.precision == 64 else np.float32
+        table = {
+            "cell_type": (_abi.F_CELL_TYPE, (H, W), np.int32),
+            "boundary_seg": (_abi.F_BOUNDARY_SEG, (H, W), np.int32),
+            "cell_class": (_abi.F_CELL_CLASS, (H, W), np.uint8),
+            "depth_p": (_abi.F_DEPTH_P, (H, W), real),
+            "depth_x": (_abi.F_DEPTH_X, (H, W), real),
+            "depth_y": (_abi.F_DEPTH_Y, (H, W), real),
+            "z_inv": (_abi.F_Z_INV, (H, W), real),
+            "pressure": (_abi.F_PRESSURE, (H, W), real),
+            "vx": (_abi.F_VX, (H, W), real),
+            "vy": (_abi.F_VY, (H, W), real),
+            "source": (_abi.F_SOURCE, (d["num_steps"],), real),
+            "upper_wall": (_abi.F_UPPER_WALL, (L,), np.int32),
+            "lower_wall": (_abi.F_LOWER_WALL, (L,), np.int32),
+            "segment": (_abi.F_SEGMENT, (L,), np.int32),
+        }
+        if name in _abi.COEFFS:
+            fid, shape, dt = _abi.F_COEFF0 + _abi.COEFFS.index(name), (H - 2, W - 2), real
+        else:
+            fid, shape, dt = table[name]
+        out = np.zeros(shape, dtype=dt)
+        check(load().vs_get_field(self.h, tube, fid, out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
+        return out
