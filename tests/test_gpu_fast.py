@@ -1,0 +1,92 @@
+"""GPU checks of the FP32 flux-form kernel (VS_K_FAST) through the C ABI.
+
+The flux form reassociates the reference update (coefficients precomputed in
+double, rounded once to float), so it is compared within FP32 tolerances:
+against the FP32 oracle (relative L2 of the probe <= 1e-4) and against the
+FP64 oracle (relative L2 <= 1e-4 over the runs below).
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from vocalsim import Solver, tubes
+from vocalsim import _abi
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+CASES = {
+    "uniform": tubes.uniform_tube(num_steps=1500),
+    "vowel3_gauss": tubes.vowel_tube(3, num_steps=1500, source=tubes.SOURCE_GAUSSIAN),
+    "vowel0_twomass": tubes.vowel_tube(0, num_steps=1500),
+    "batch_tube5": tubes.batch_tubes(8, seed=2, num_steps=1500)[5],
+    "open_mouth": tubes.uniform_tube(num_steps=1500).replace(mouth_dirichlet=0),
+    "no_pml": tubes.vowel_tube(1, num_steps=1500, source=tubes.SOURCE_GAUSSIAN).replace(pml=0),
+}
+
+
+@pytest.mark.parametrize("threads", [1024, 512])
+@pytest.mark.parametrize("name", list(CASES))
+def test_fast_vs_oracles(hip_device, name, threads):
+    spec = CASES[name]
+    print(f"[fast] case={name} threads={threads}", flush=True)
+    p64, s64 = orc.OracleTube(spec, 64).run()
+    p32, _ = orc.OracleTube(spec, 32).run()
+    with Solver(spec, precision=32, threads=threads, kernel=_abi.K_FAST) as s:
+        s.run()
+        p = s.probe()
+        src = s.source_trace()
+    e32, e64 = rel_l2(p, p32), rel_l2(p, p64)
+    print(f"  rel L2 vs FP32 oracle {e32:.2e}, vs FP64 oracle {e64:.2e}, FP32 oracle vs FP64 {rel_l2(p32, p64):.2e}")
+    assert np.all(np.isfinite(p))
+    assert e64 <= 1e-4 and e32 <= 1e-4
+    assert rel_l2(src, s64) <= 1e-4
+
+
+def test_fast_batch_matches_single(hip_device):
+    specs = tubes.batch_tubes(16, seed=4, num_steps=800)
+    print("[fast batch] 16 tubes", flush=True)
+    with Solver(specs, precision=32, threads=1024, kernel=_abi.K_FAST) as s:
+        s.run()
+        probes = [s.probe(i) for i in range(len(specs))]
+    for i in (0, 7, 8, 15):
+        with Solver(specs[i], precision=32, threads=1024, kernel=_abi.K_FAST) as s1:
+            s1.run()
+            assert np.array_equal(s1.probe(), probes[i]), i
+        p64, _ = orc.OracleTube(specs[i], 64).run()
+        assert rel_l2(probes[i], p64) <= 1e-4
+
+
+def test_fast_blocks_equal_one_run(hip_device):
+    spec = tubes.vowel_tube(5, num_steps=900)
+    with Solver(spec, precision=32, threads=1024, kernel=_abi.K_FAST, steps_per_launch=900) as s:
+        s.run()
+        a = s.probe()
+    with Solver(spec, precision=32, threads=1024, kernel=_abi.K_FAST, steps_per_launch=64) as s:
+        for n in (100, 300, 500):
+            s.run(n)
+        b = s.probe()
+    assert np.array_equal(a, b)
+
+
+def test_fast_state_export_is_velocity(hip_device):
+    spec = tubes.vowel_tube(2, num_steps=400, source=tubes.SOURCE_GAUSSIAN)
+    o = orc.OracleTube(spec, 64)
+    o.run()
+    P, Vx, Vy = o.state()
+    with Solver(spec, precision=32, threads=1024, kernel=_abi.K_FAST) as s:
+        s.run()
+        gP, gVx, gVy = s.field("pressure"), s.field("vx"), s.field("vy")
+    assert rel_l2(gP, P) < 1e-4 and rel_l2(gVx, Vx) < 1e-4 and rel_l2(gVy, Vy) < 1e-4
+
+
+def test_fast_rejects_fp64(hip_device):
+    spec = tubes.uniform_tube(num_steps=10)
+    with Solver(spec, precision=64) as s:
+        with pytest.raises(_abi.VocalSimError):
+            s.set_option(_abi.OPT_KERNEL, _abi.K_FAST)

@@ -257,6 +257,12 @@ struct vs_handle {
     double last_ms = 0, last_mean_launch_ms = 0;
     int last_launches = 0;
     std::vector<TwoMassState> tm0;   // initial two-mass states (for vs_reset)
+    // FP32 flux-form path
+    FastBufs fb{};
+    bool fast_ready = false;   // coefficients built
+    bool flux_valid = false;   // Uxf/Uyf hold the current state
+    int max_interior = 0;      // largest (H-2)*(W-2) of the batch
+    int lds_limit = 0;
 
     ~vs_handle() {
         for (void* p : allocs) (void)hipFree(p);
@@ -317,6 +323,7 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
         col += p.W; wall += p.L; sec += s.num_sections; src += p.num_steps;
         h->max_steps = std::max(h->max_steps, p.num_steps);
         h->cells_per_step += (long long)(p.H - 2) * (p.W - 2);
+        h->max_interior = std::max(h->max_interior, (p.H - 2) * (p.W - 2));
         vs_dims& dm = h->dims[k];
         dm.frameH = p.H; dm.frameW = p.W; dm.domainH = p.domainH; dm.domainW = p.domainW;
         dm.listenerX = p.listenerX; dm.listenerY = p.listenerY;
@@ -396,9 +403,63 @@ int check_handle(const vs_handle* h) {
     return VS_OK;
 }
 
+// Lazily builds the per-cell coefficients of the FP32 flux-form kernel.
+int prepare_fast(vs_handle* h) {
+    if (h->precision != VS_FP32) return fail(VS_E_INVALID, "the flux-form kernel is FP32 only");
+    if (h->threads < 128) return fail(VS_E_INVALID, "the on-chip kernel needs at least 128 threads");
+    const int K = (h->max_interior + h->threads - 64 - 1) / (h->threads - 64);
+    if (K > 24) return fail(VS_E_INVALID, "tube too large for the on-chip kernel at this work-group size");
+    if ((long long)(fast_lds_bytes(K, h->threads) + fast_lds_static_bytes()) > h->lds_limit)
+        return fail(VS_E_INVALID, "on-chip kernel needs " + std::to_string(fast_lds_bytes(K, h->threads)) +
+                    " bytes of LDS, device allows " + std::to_string(h->lds_limit));
+    if (!h->fast_ready) {
+        FastBufs& f = h->fb;
+        const long long n = h->total_cells;
+        int rc;
+        if ((rc = h->alloc(&f.kP, n)) || (rc = h->alloc(&f.gX, n)) || (rc = h->alloc(&f.gY, n)) ||
+            (rc = h->alloc(&f.wx, n)) || (rc = h->alloc(&f.wy, n)) || (rc = h->alloc(&f.ex, n)) ||
+            (rc = h->alloc(&f.Uxf, n)) || (rc = h->alloc(&f.Uyf, n)) || (rc = h->alloc(&f.code, n)))
+            return rc;
+        const PhysTables& ph = h->desc[0].ph;
+        f.lut[0] = 1.0f;
+        f.lut[1] = 0.0f;
+        f.lut[2] = (float)(1.0 / (1.0 + ph.dt));
+        for (int k = 1; k <= 5; ++k) f.lut[2 + k] = (float)(1.0 / (1.0 + ph.sigma[kPml0 + k]));
+        for (const TubeDesc& d : h->desc)
+            if (d.srate_mul != h->desc[0].srate_mul) return fail(VS_E_INVALID, "one srate_mul per batch on the flux-form kernel");
+        h->fast_ready = true;
+    }
+    if (!h->flux_valid) {
+        hipError_t e = launch_fast_setup(h->d_desc, h->desc.data(), h->n, h->bf, h->fb, h->stream);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("flux setup: ") + hipGetErrorString(e));
+        h->flux_valid = true;
+    }
+    return VS_OK;
+}
+
+template <typename Real>
+hipError_t launch_any(vs_handle* h, Bufs<Real>& b, int t0, int ns);
+template <>
+hipError_t launch_any<double>(vs_handle* h, Bufs<double>& b, int t0, int ns) {
+    return launch_step<double>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, h->stream);
+}
+template <>
+hipError_t launch_any<float>(vs_handle* h, Bufs<float>& b, int t0, int ns) {
+    if (h->kernel == VS_K_FAST) {
+        const int K = (h->max_interior + h->threads - 64 - 1) / (h->threads - 64);
+        return launch_fast_step(K, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, h->stream);
+    }
+    h->flux_valid = false;
+    return launch_step<float>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, h->stream);
+}
+
 template <typename Real>
 int run_steps(vs_handle* h, int n_steps) {
     Bufs<Real>& b = bufs_of<Real>(h);
+    if (h->kernel == VS_K_FAST) {
+        const int rc = prepare_fast(h);
+        if (rc != VS_OK) return rc;
+    }
     const int start = h->steps_done;
     const int end = std::min(h->max_steps, start + n_steps);
     int launches = 0;
@@ -412,7 +473,7 @@ int run_steps(vs_handle* h, int n_steps) {
             VS_HIP(hipEventCreate(&a)); VS_HIP(hipEventCreate(&c));
             VS_HIP(hipEventRecord(a, h->stream));
         }
-        hipError_t e = launch_step<Real>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, h->stream);
+        hipError_t e = launch_any<Real>(h, b, t0, ns);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("step launch: ") + hipGetErrorString(e));
         if (h->timing) { VS_HIP(hipEventRecord(c, h->stream)); evs.push_back(a); evs.push_back(c); }
         ++launches;
@@ -488,6 +549,13 @@ int vs_create(const vs_tube_spec* specs, int n_tubes, int precision, vs_handle**
     VS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     VS_HIP(hipEventCreate(&h->ev0));
     VS_HIP(hipEventCreate(&h->ev1));
+    {
+        int dev = 0;
+        VS_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        VS_HIP(hipGetDeviceProperties(&prop, dev));
+        h->lds_limit = (int)std::max(prop.sharedMemPerBlockOptin, prop.sharedMemPerBlock);
+    }
     const int rc = precision == VS_FP64 ? build<double>(h.get(), specs) : build<float>(h.get(), specs);
     if (rc != VS_OK) return rc;
     *out = h.release();
@@ -619,6 +687,7 @@ int vs_reset(vs_handle* h) {
     VS_HIP(hipMemsetAsync(Vy, 0, h->total_cells * es, h->stream));
     double* pexc = h->precision == 64 ? h->bd.pexc : h->bf.pexc;
     VS_HIP(hipMemsetAsync(pexc, 0, h->n * sizeof(double), h->stream));
+    h->flux_valid = false;
     TwoMassState* tm = h->precision == 64 ? h->bd.tm : h->bf.tm;
     VS_HIP(hipMemcpyAsync(tm, h->tm0.data(), h->n * sizeof(TwoMassState), hipMemcpyHostToDevice, h->stream));
     VS_HIP(hipStreamSynchronize(h->stream));
@@ -660,6 +729,8 @@ int vs_set_option(vs_handle* h, int option, int value) {
     case VS_OPT_USE_GRAPH: h->use_graph = value ? 1 : 0; return VS_OK;
     case VS_OPT_KERNEL:
         if (value != VS_K_EXACT && value != VS_K_FAST) return fail(VS_E_INVALID, "unknown kernel family");
+        if (value == VS_K_FAST && h->precision != VS_FP32) return fail(VS_E_INVALID, "the flux-form kernel is FP32 only");
+        if (value != h->kernel && value == VS_K_FAST) h->flux_valid = false;
         h->kernel = value; return VS_OK;
     case VS_OPT_TIMING: h->timing = value ? 1 : 0; return VS_OK;
     default: return fail(VS_E_INVALID, "unknown option");

@@ -183,7 +183,22 @@ struct Bufs {
     unsigned* status;    // device status word
 };
 
-// Kernel launchers (vs_kernels.hip).
+// Per-cell coefficients of the FP32 flux-form kernel (vs_fast.hip).
+struct FastBufs {
+    float* kP; float* gX; float* gY;     // pressure and face coefficients
+    float* wx; float* wy; float* ex;     // special faces: wall admittance, source
+    float* Uxf; float* Uyf;              // flux state U = V * depth
+    uint16_t* code;                      // damping codes + special bit
+    float lut[8];                        // damping factor by code
+};
+
+// Kernel launchers (vs_kernels.hip, vs_fast.hip).
+size_t fast_lds_bytes(int K, int threads);
+size_t fast_lds_static_bytes();
+hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<float> b,
+                             FastBufs f, hipStream_t st);
+hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
+                            FastBufs f, int t0, int nsteps, int threads, hipStream_t st);
 template <typename Real>
 hipError_t launch_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<Real> b,
                         Real* tmpA, Real* tmpB, Real* tmpC, Real* tmpD, hipStream_t st);

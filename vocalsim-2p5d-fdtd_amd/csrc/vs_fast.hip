@@ -1,0 +1,356 @@
+// vs_fast.hip -- FP32 throughput path: one work-group per tube with the
+// tube's state held on chip for a whole block of steps.
+//
+// Layout inside a work-group of T threads.  The tube interior (Hi x Wi cells,
+// row-major, j = r*Wi + c) is cut into T runs of K consecutive cells; thread t
+// owns cells j = t*K + k, k < K.
+//   LDS   P[k][t], Uy[k][t]   pressure and y-flux of every cell, stored
+//                             slot-major so that every lane-uniform access
+//                             (own cell, cell above, cell below, next cell)
+//                             is a contiguous, bank-conflict-free row;
+//         UxEnd[t]            x-flux of each run's last cell (left neighbour
+//                             of the next run);
+//         lut[8]              damping factors 1/(1 + sigma) by code.
+//   VGPRs Ux[K], kP[K], gX[K], gY[K] and the packed per-cell codes.
+//
+// Update in flux form (U = V * depth), algebraically equal to the reference
+// step (FDTDEngineCPU.cpp:316-405) with every per-cell coefficient
+// precomputed at setup:
+//   P'  = aP * P - kP * (Ux - Ux_left + Uy - Uy_below)
+//   Ux' = ax * Ux + gX * (P' - P'_right)
+//   Uy' = ay * Uy + gY * (P' - P'_top)
+// aP = m/(1+sigma_c), kP = m*K1/((1+sigma_c)*dP) (m = 0 on Dirichlet cells);
+// ax = mbx/(mbx+Mx), gX = dX*Cx/(mbx+Mx) (Cx = mbx^2 dt/(rho ds)), same in y.
+// Cells whose face terms differ from that form ("special": walls carrying an
+// admittance, whose face velocity is -z*Nin*P_neighbour, and the excitation
+// column, whose x-face carries the source velocity) override the result with
+// coefficients read from HBM.
+
+#include "vs_common.h"
+
+namespace vs {
+
+namespace {
+
+constexpr uint32_t kCodeSpecial = 1u << 9;
+
+// ---------------------------------------------------------------------------
+// Setup of the per-cell coefficients of the flux form (evaluated in double
+// from the FP32 depth maps, then rounded once to float).
+// damping code: 0 -> sigma 0, 1 -> zero (Dirichlet cell / blocked face),
+//               2 -> sigma dt, 2 + k -> PML ring sigma k/5 * 0.5 (k = 1..5).
+// ---------------------------------------------------------------------------
+__device__ inline int sigma_code(int type) {
+    switch (type) {
+    case kWall: case kExcitation: case kHead: return 2;
+    case kPml1: return 3; case kPml2: return 4; case kPml3: return 5; case kPml4: return 6; case kPml5: return 7;
+    case kDead: return 8;
+    default: return 0;   // air, pml0, dynamic, noPressure
+    }
+}
+
+__global__ void k_fast_setup(const TubeDesc* __restrict__ tubes, Bufs<float> b, FastBufs f) {
+    const TubeDesc& T = tubes[blockIdx.y];
+    const int H = T.H, W = T.W, Wi = W - 2;
+    const int n = (H - 2) * Wi;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = 1 + i / Wi, c = 1 + i % Wi;
+    const long long q = T.cell_off + (long long)r * W + c;
+    const uint8_t* cls = b.cls + T.cell_off;
+    const int qq = r * W + c;
+    const int tc = cls[qq] & kTypeMask, tr = cls[qq + 1] & kTypeMask, tt = cls[qq - W] & kTypeMask;
+    const bool segwall = (cls[qq] & kSegWallBit) != 0;
+    const double dt = T.ph.dt, C = dt / T.ph.rds, K1 = T.ph.K1;
+    const double bc = T.ph.beta[tc], br = T.ph.beta[tr], bt = T.ph.beta[tt];
+    const double sc = T.ph.sigma[tc], sr = T.ph.sigma[tr], st = T.ph.sigma[tt];
+    const double dP = b.dP[q], dX = b.dX[q], dY = b.dY[q];
+    const bool nop = T.dirichlet && tc == kNoPressure;
+    // pressure
+    const double m = nop ? 0.0 : 1.0;
+    f.kP[q] = (float)(m * K1 / ((1.0 + sc) * dP));
+    int codeP = nop ? 1 : (sigma_code(tc) == 8 ? 1 : sigma_code(tc));
+    // faces
+    const double mbx = br < bc ? br : bc, mby = bt < bc ? bt : bc;
+    const double Mx = sc < sr ? sr : sc, My = sc < st ? st : sc;
+    const int mcx = max(sigma_code(tc), sigma_code(tr)), mcy = max(sigma_code(tc), sigma_code(tt));
+    const int codeX = (mbx == 0.0 || mcx == 8) ? 1 : mcx;
+    const int codeY = (mby == 0.0 || mcy == 8) ? 1 : mcy;
+    f.gX[q] = mbx == 0.0 ? 0.0f : (float)(dX * (mbx * mbx * C) / (mbx + Mx));
+    f.gY[q] = mby == 0.0 ? 0.0f : (float)(dY * (mby * mby * C) / (mby + My));
+    uint32_t code = (uint32_t)codeP | ((uint32_t)codeX << 3) | ((uint32_t)codeY << 6);
+    // special faces
+    float wx = 0.0f, wy = 0.0f, e = 0.0f;
+    if (tc == kExcitation) {
+        // (exeV * eWx * Mx) / (0 + Mx): the x-face carries the source velocity
+        e = (float)dX;
+        code |= kCodeSpecial;
+    } else if (segwall) {
+        const int ibr = (int)br, ibt = (int)bt;
+        const int lut = (ibr == kAir && ibt == kAir) ? 0 : (ibr == kAir && ibt == kWall) ? 1
+                      : (ibr == kWall && ibt == kAir) ? 2 : 3;
+        const double nin = T.ph.wall_normal[lut] * (1.0 - bc);
+        const double z = b.zcol[T.col_off + c];
+        const int ec = 0, er = tr == kExcitation, et = tt == kExcitation;
+        const double nex = (double)((1 - ec) * (1 - er)), ney = (double)((1 - ec) * (1 - et));
+        // (Mx * z * nE * (-xor1 * P_r * Nin)) / (0 + Mx), xor1 = beta_r (beta_c = 0 on walls)
+        wx = (float)(dX * z * nex * br * nin);
+        wy = (float)(dY * z * ney * bt * nin);
+        code |= kCodeSpecial;
+    }
+    f.code[q] = (uint16_t)code;
+    f.wx[q] = wx;
+    f.wy[q] = wy;
+    f.ex[q] = e;
+}
+
+// Flux <-> velocity conversions of the state (start / end of the fast path).
+__global__ void k_vel_to_flux(const TubeDesc* __restrict__ tubes, Bufs<float> b, FastBufs f) {
+    const TubeDesc& T = tubes[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T.H * T.W) return;
+    const long long q = T.cell_off + i;
+    f.Uxf[q] = b.Vx[q] * b.dX[q];
+    f.Uyf[q] = b.Vy[q] * b.dY[q];
+}
+
+template <int K>
+__device__ inline uint32_t code_of(const uint32_t* cw, int k) {
+    return (k & 1) ? (cw[k >> 1] >> 16) : (cw[k >> 1] & 0xffffu);
+}
+
+template <int K>
+__global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__ tubes,
+                                                    const int* __restrict__ order, Bufs<float> b,
+                                                    FastBufs f, int t0, int nsteps) {
+    extern __shared__ float lds[];
+    // threads [0, T) own cells; the last wave is the control wave (source,
+    // two-mass recurrence, probe).  Keeping the double-precision recurrence
+    // out of the cell waves keeps their register budget for the cell arrays.
+    const int T = blockDim.x - 64, t = threadIdx.x;
+    float* Pb = lds;
+    float* Uyb = lds + K * T;
+    float* UxE = lds + 2 * K * T;
+    float* lut = UxE + T;
+    double* sexe = reinterpret_cast<double*>(lut + 8);   // [2], 8-byte aligned (T even)
+    __shared__ TwoMassState tm;
+
+    const int tube = order[blockIdx.x];
+    const TubeDesc& D = tubes[tube];
+    const int H = D.H, W = D.W, Hi = H - 2, Wi = W - 2, N = Hi * Wi;
+    const int tend = min(t0 + nsteps, D.num_steps);
+    const long long co = D.cell_off;
+
+    if (t >= T) {
+        // ------------------------- control wave -------------------------
+        const int lane = t - T;
+        const bool twomass = D.source_type == 2;
+        const int jL = (D.listener / W - 1) * Wi + (D.listener % W - 1);
+        const int jE = (D.excitation / W - 1) * Wi + (D.excitation % W - 1);
+        const int aL = (jL % K) * T + jL / K, aE = (jE % K) * T + jE / K;
+        double pexc = 0.0;
+        if (lane < 8) lut[lane] = f.lut[lane];
+        if (lane == 0) {
+            pexc = b.pexc[tube];
+            if (twomass) tm = b.tm[tube];
+            if (t0 < tend) {
+                double e;
+                if (twomass) { tm.p1 = pexc; e = twomass_step(tm, D.srate_mul); }
+                else e = (double)b.source[D.src_off + t0];
+                sexe[t0 & 1] = (double)(float)e;
+            }
+        }
+        __syncthreads();
+        for (int ts = t0; ts < tend; ++ts) {
+            __syncthreads();   // pressure phase done
+            if (lane == 0) {
+                b.probe[D.src_off + ts] = (double)Pb[aL];
+                pexc = (double)Pb[aE];
+                b.strace[D.src_off + ts] = sexe[ts & 1];
+                if (ts + 1 < tend) {
+                    double e;
+                    if (twomass) { tm.p1 = pexc; e = twomass_step(tm, D.srate_mul); }
+                    else e = (double)b.source[D.src_off + ts + 1];
+                    sexe[(ts + 1) & 1] = (double)(float)e;
+                }
+            }
+            __syncthreads();   // velocity phase done
+        }
+        if (lane == 0) {
+            b.pexc[tube] = pexc;
+            if (twomass) b.tm[tube] = tm;
+        }
+        return;
+    }
+
+    // --------------------------- cell waves ---------------------------
+    float Ux[K], kP[K], gX[K], gY[K];
+    uint32_t cw[(K + 1) / 2];
+    const int jb = t * K;
+    const int r0 = jb / Wi, c0 = jb - r0 * Wi;
+    {
+        int r = r0, c = c0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int j = jb + k;
+            float p = 0.f, uy = 0.f, ux = 0.f, kp = 0.f, gx = 0.f, gy = 0.f;
+            uint32_t cd = 1u | (1u << 3) | (1u << 6);
+            if (j < N) {
+                const long long q = co + (long long)(r + 1) * W + (c + 1);
+                p = b.P[q]; uy = f.Uyf[q]; ux = f.Uxf[q];
+                kp = f.kP[q]; gx = f.gX[q]; gy = f.gY[q]; cd = f.code[q];
+            }
+            Pb[k * T + t] = p;
+            Uyb[k * T + t] = uy;
+            Ux[k] = ux; kP[k] = kp; gX[k] = gx; gY[k] = gy;
+            if (k & 1) cw[k >> 1] |= cd << 16; else cw[k >> 1] = cd;
+            if (++c == Wi) { c = 0; ++r; }
+        }
+    }
+    UxE[t] = Ux[K - 1];
+    __syncthreads();
+
+    for (int ts = t0; ts < tend; ++ts) {
+        // Launder the lane coordinates once per step so the compiler
+        // recomputes per-cell addresses and masks (a few integer ops) instead
+        // of keeping K copies of them live across the time loop.
+        int tl = t, r0l = r0, c0l = c0;
+        asm volatile("" : "+v"(tl), "+v"(r0l), "+v"(c0l));
+#pragma unroll
+        for (int k = 0; k < (K + 1) / 2; ++k) asm volatile("" : "+v"(cw[k]));
+        // ---------------- pressure phase ----------------
+        {
+            int r = r0l, c = c0l;
+            const float uxl0 = (tl > 0) ? UxE[tl - 1] : 0.f;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int j = jb + k;
+                if (jb + k < N) {
+                    const float p = Pb[k * T + tl];
+                    const float uy = Uyb[k * T + tl];
+                    float uyb = 0.f;
+                    if (r < Hi - 1) {
+                        const int d = k + Wi;
+                        uyb = Uyb[(d % K) * T + tl + d / K];
+                    }
+                    const float left = (c > 0) ? (k > 0 ? Ux[k - 1] : uxl0) : 0.f;
+                    const float dv = (Ux[k] - left) + (uy - uyb);
+                    const float aP = lut[code_of<K>(cw, k) & 7u];
+                    Pb[k * T + tl] = aP * p - kP[k] * dv;
+                }
+                if (++c == Wi) { c = 0; ++r; }
+            }
+        }
+        __syncthreads();
+        const float ev = (float)sexe[ts & 1];
+        // ---------------- velocity phase ----------------
+        {
+            asm volatile("" : "+v"(tl), "+v"(r0l), "+v"(c0l));
+#pragma unroll
+            for (int k = 0; k < (K + 1) / 2; ++k) asm volatile("" : "+v"(cw[k]));
+            int r = r0l, c = c0l;
+            float pn = Pb[tl];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const float p = pn;
+                if (k < K - 1) pn = Pb[(k + 1) * T + tl];
+                else pn = (tl + 1 < T) ? Pb[tl + 1] : 0.f;
+                if (tl * K + k < N) {
+                    const float pr = (c < Wi - 1) ? pn : 0.f;
+                    float pa = 0.f;
+                    if (r > 0) {
+                        const int d = k - Wi;
+                        const int tt = d >= 0 ? d / K : -((-d + K - 1) / K);   // floor(d / K)
+                        const int kk = d - tt * K;
+                        pa = Pb[kk * T + tl + tt];
+                    }
+                    const uint32_t cd = code_of<K>(cw, k);
+                    const float ax = lut[(cd >> 3) & 7u], ay = lut[(cd >> 6) & 7u];
+                    float ux = ax * Ux[k] + gX[k] * (p - pr);
+                    float uy = ay * Uyb[k * T + tl] + gY[k] * (p - pa);
+                    if (cd & kCodeSpecial) {
+                        const long long q = co + (long long)(r + 1) * W + (c + 1);
+                        ux = f.ex[q] * ev - f.wx[q] * pr;
+                        uy = -f.wy[q] * pa;
+                    }
+                    Ux[k] = ux;
+                    Uyb[k * T + tl] = uy;
+                }
+                if (++c == Wi) { c = 0; ++r; }
+            }
+            UxE[tl] = Ux[K - 1];
+        }
+        __syncthreads();
+    }
+    // ---------------- write back ----------------
+    {
+        // fresh lane coordinates: the addresses of the initial loads must not
+        // stay live across the time loop
+        int r0w = r0, c0w = c0;
+        asm volatile("" : "+v"(r0w), "+v"(c0w));
+        int r = r0w, c = c0w;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int j = jb + k;
+            if (j < N) {
+                const long long q = co + (long long)(r + 1) * W + (c + 1);
+                const float uy = Uyb[k * T + t];
+                b.P[q] = Pb[k * T + t];
+                f.Uxf[q] = Ux[k];
+                f.Uyf[q] = uy;
+                b.Vx[q] = Ux[k] / b.dX[q];
+                b.Vy[q] = uy / b.dY[q];
+            }
+            if (++c == Wi) { c = 0; ++r; }
+        }
+    }
+}
+
+template <int K>
+hipError_t launch_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
+                    int t0, int nsteps, int threads, hipStream_t st) {
+    const size_t lds = fast_lds_bytes(K, threads);
+    static bool configured = false;
+    if (!configured) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_fast_step<K>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024 - (int)fast_lds_static_bytes());
+        if (e != hipSuccess) return e;
+        configured = true;
+    }
+    hipLaunchKernelGGL(k_fast_step<K>, dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0, nsteps);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t fast_lds_bytes(int K, int threads) {
+    const int T = threads - 64;   // cell threads
+    return (size_t)(2 * K * T + T + 8) * sizeof(float) + 2 * sizeof(double);
+}
+
+size_t fast_lds_static_bytes() { return sizeof(TwoMassState) + 64; }
+
+hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<float> b,
+                             FastBufs f, hipStream_t st) {
+    int maxHW = 0;
+    for (int k = 0; k < n; ++k) maxHW = max(maxHW, h_tubes[k].H * h_tubes[k].W);
+    dim3 g((maxHW + 255) / 256, n);
+    hipLaunchKernelGGL(k_fast_setup, g, dim3(256), 0, st, d_tubes, b, f);
+    hipLaunchKernelGGL(k_vel_to_flux, g, dim3(256), 0, st, d_tubes, b, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
+                            FastBufs f, int t0, int nsteps, int threads, hipStream_t st) {
+    switch (K) {
+#define VS_K(k) case k: return launch_k<k>(d_tubes, d_order, n, b, f, t0, nsteps, threads, st);
+        VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
+        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(19) VS_K(20)
+        VS_K(21) VS_K(22) VS_K(23) VS_K(24)
+#undef VS_K
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace vs

@@ -313,7 +313,7 @@ __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict_
     const bool twomass = T.source_type == 2;
     const bool dirichlet = T.dirichlet != 0;
     const bool eigen = T.variant == 1;
-    TwoMassState tm;
+    __shared__ TwoMassState tm;   // lane 0 only; LDS keeps it out of every lane's VGPRs
     double pexc = 0;
     if (tid == 0) {
         pexc = b.pexc[tube];
