@@ -57,8 +57,8 @@ def main():
     total_blocks = args.warmup + args.steps
     nsteps = max(19845, total_blocks * BLOCK)
     specs = tubes.batch_tubes(args.tubes, seed=0, num_steps=nsteps)
-    kernel = args.kernel if args.kernel >= 0 else _abi.K_EXACT
-    threads = args.threads or 256
+    kernel = args.kernel if args.kernel >= 0 else _abi.K_FAST
+    threads = args.threads or 1024
     log(f"[bench] building {args.tubes} tubes (FP32), kernel={kernel} threads={threads}")
     s = vocalsim.Solver(specs, precision=32, threads=threads, steps_per_launch=BLOCK, kernel=kernel)
     cells = s.cells_per_step
@@ -86,7 +86,7 @@ def main():
 
     # ---------------- plain FP64 single tube ----------------
     spec64 = tubes.vowel_tube(0, num_steps=33075)
-    s64 = vocalsim.Solver(spec64, precision=64, threads=256, steps_per_launch=BLOCK)
+    s64 = vocalsim.Solver(spec64, precision=64, threads=1024, steps_per_launch=BLOCK)
     c64 = s64.cells_per_step
     s64.run(BLOCK)
     torch.cuda.synchronize()

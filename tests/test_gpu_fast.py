@@ -30,21 +30,27 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("threads", [1024, 512])
+@pytest.mark.parametrize("threads", [1024, 768])
 @pytest.mark.parametrize("name", list(CASES))
 def test_fast_vs_oracles(hip_device, name, threads):
     spec = CASES[name]
     print(f"[fast] case={name} threads={threads}", flush=True)
     p64, s64 = orc.OracleTube(spec, 64).run()
     p32, _ = orc.OracleTube(spec, 32).run()
-    with Solver(spec, precision=32, threads=threads, kernel=_abi.K_FAST) as s:
+    with Solver(spec, precision=32, threads=threads) as s:
+        d = s.dims[0]
+        if (d["frameH"] - 2) * (d["frameW"] - 2) > 24 * (threads - 64):
+            pytest.skip("tube larger than the on-chip kernel holds at this work-group size")
+        s.set_option(_abi.OPT_KERNEL, _abi.K_FAST)
         s.run()
         p = s.probe()
         src = s.source_trace()
     e32, e64 = rel_l2(p, p32), rel_l2(p, p64)
     print(f"  rel L2 vs FP32 oracle {e32:.2e}, vs FP64 oracle {e64:.2e}, FP32 oracle vs FP64 {rel_l2(p32, p64):.2e}")
     assert np.all(np.isfinite(p))
-    assert e64 <= 1e-4 and e32 <= 1e-4
+    # the FP32 oracle runs the two-mass recurrence in float; the device runs it in
+    # double, so the FP32 oracle can be the farther one from the FP64 oracle
+    assert e64 <= 1e-4 and e32 <= max(1e-4, 2 * rel_l2(p32, p64))
     assert rel_l2(src, s64) <= 1e-4
 
 
@@ -83,6 +89,13 @@ def test_fast_state_export_is_velocity(hip_device):
         s.run()
         gP, gVx, gVy = s.field("pressure"), s.field("vx"), s.field("vy")
     assert rel_l2(gP, P) < 1e-4 and rel_l2(gVx, Vx) < 1e-4 and rel_l2(gVy, Vy) < 1e-4
+
+
+def test_fast_rejects_oversized_tube(hip_device):
+    spec = tubes.vowel_tube(0, num_steps=10)
+    with Solver(spec, precision=32, threads=256, kernel=_abi.K_FAST) as s:
+        with pytest.raises(_abi.VocalSimError):
+            s.run()
 
 
 def test_fast_rejects_fp64(hip_device):
