@@ -169,6 +169,17 @@ int vs_stream(vs_handle* h, void** stream);
  * mean duration of one launch. */
 int vs_last_run_timing(const vs_handle* h, double* total_ms, int* launches, double* mean_launch_ms);
 
+/* FP64 discrete-time Fourier transform X(f) = sum_n x[n] w[n] e^{-i 2 pi f n / fs}
+ * at nf arbitrary frequencies [Hz] (w = Hann window when hann != 0, else 1).
+ * Host arrays in and out; computed on the GPU.  Replaces the np.fft.fft
+ * spectrum of results/formants_bandwidths.py:34-36 and
+ * results/plt_transfer_function.py:21-23 with an evaluation at any frequency,
+ * which the sub-bin formant estimates need. */
+int vs_dtft(const double* x, int n, double fs, int hann, const double* freqs, int nf, double* re, double* im);
+/* Same transform of a tube's device-resident probe waveform (kind 0) or
+ * injected source velocity (kind 1) over the steps advanced so far. */
+int vs_probe_dtft(vs_handle* h, int tube, int kind, int hann, const double* freqs, int nf, double* re, double* im);
+
 /* Interior cell updates of one step summed over the batch:
  * sum (frameH-2)(frameW-2)  (FDTDEngineCPU.cpp:316-317). */
 int vs_cells_per_step(const vs_handle* h, long long* cells);

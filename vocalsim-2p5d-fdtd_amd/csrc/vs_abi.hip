@@ -763,6 +763,50 @@ int vs_last_run_timing(const vs_handle* h, double* total_ms, int* launches, doub
     return VS_OK;
 }
 
+namespace {
+int dtft_device(const double* d_x, int n, double fs, int hann, const double* freqs, int nf, double* re,
+                double* im, hipStream_t st) {
+    double *d_f = nullptr, *d_re = nullptr, *d_im = nullptr;
+    VS_HIP(hipMalloc(&d_f, sizeof(double) * nf));
+    VS_HIP(hipMalloc(&d_re, sizeof(double) * nf));
+    VS_HIP(hipMalloc(&d_im, sizeof(double) * nf));
+    hipError_t e = hipMemcpyAsync(d_f, freqs, sizeof(double) * nf, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_dtft(d_x, 0, 1, n, fs, hann, d_f, nf, d_re, d_im, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(re, d_re, sizeof(double) * nf, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(im, d_im, sizeof(double) * nf, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_f); (void)hipFree(d_re); (void)hipFree(d_im);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("dtft: ") + hipGetErrorString(e));
+    return VS_OK;
+}
+}  // namespace
+
+int vs_dtft(const double* x, int n, double fs, int hann, const double* freqs, int nf, double* re, double* im) {
+    if (!x || !freqs || !re || !im || n <= 0 || nf <= 0 || !(fs > 0))
+        return fail(VS_E_INVALID, "vs_dtft: bad argument");
+    double* d_x = nullptr;
+    VS_HIP(hipMalloc(&d_x, sizeof(double) * n));
+    hipError_t e = hipMemcpy(d_x, x, sizeof(double) * n, hipMemcpyHostToDevice);
+    int rc = e == hipSuccess ? dtft_device(d_x, n, fs, hann, freqs, nf, re, im, nullptr)
+                             : fail(VS_E_DEVICE, std::string("vs_dtft copy: ") + hipGetErrorString(e));
+    (void)hipFree(d_x);
+    return rc;
+}
+
+int vs_probe_dtft(vs_handle* h, int tube, int kind, int hann, const double* freqs, int nf, double* re, double* im) {
+    if (check_handle(h) || !freqs || !re || !im || nf <= 0) return fail(VS_E_INVALID, "vs_probe_dtft: bad argument");
+    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+    if (kind != 0 && kind != 1) return fail(VS_E_INVALID, "kind must be 0 (probe) or 1 (source)");
+    const TubeDesc& d = h->desc[tube];
+    const int n = std::min(h->steps_done, d.num_steps);
+    if (n <= 0) return fail(VS_E_STATE, "no steps advanced yet");
+    const double* base = kind == 0 ? (h->precision == 64 ? h->bd.probe : h->bf.probe)
+                                    : (h->precision == 64 ? h->bd.strace : h->bf.strace);
+    const double fs = 1.0 / d.ph.dt;
+    VS_HIP(hipStreamSynchronize(h->stream));
+    return dtft_device(base + d.src_off, n, fs, hann, freqs, nf, re, im, h->stream);
+}
+
 int vs_cells_per_step(const vs_handle* h, long long* cells) {
     if (check_handle(h) || !cells) return fail(VS_E_INVALID, "vs_cells_per_step: bad argument");
     *cells = h->cells_per_step;

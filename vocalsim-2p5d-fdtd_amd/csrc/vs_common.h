@@ -193,6 +193,8 @@ struct FastBufs {
 };
 
 // Kernel launchers (vs_kernels.hip, vs_fast.hip).
+hipError_t launch_dtft(const double* d_x, long long stride, int nsig, int n, double fs, int hann,
+                       const double* d_f, int nf, double* d_re, double* d_im, hipStream_t st);
 size_t fast_lds_bytes(int K, int threads);
 size_t fast_lds_static_bytes();
 hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<float> b,

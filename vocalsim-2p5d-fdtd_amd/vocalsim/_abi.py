@@ -32,7 +32,7 @@ EXPORTS = [
     "vs_last_error", "vs_abi_version", "vs_create", "vs_destroy", "vs_num_tubes", "vs_get_dims",
     "vs_get_field", "vs_run", "vs_sync", "vs_steps_done", "vs_reset", "vs_get_probe",
     "vs_get_source_trace", "vs_set_option", "vs_get_option", "vs_field_ptr", "vs_stream",
-    "vs_last_run_timing", "vs_cells_per_step",
+    "vs_last_run_timing", "vs_cells_per_step", "vs_dtft", "vs_probe_dtft",
 ]
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -101,6 +101,8 @@ def load():
     L.vs_stream.argtypes = [c_void, ctypes.POINTER(c_void)]
     L.vs_last_run_timing.argtypes = [c_void, _D, ctypes.POINTER(c_int), _D]
     L.vs_cells_per_step.argtypes = [c_void, ctypes.POINTER(ctypes.c_longlong)]
+    L.vs_dtft.argtypes = [_D, c_int, ctypes.c_double, c_int, _D, c_int, _D, _D]
+    L.vs_probe_dtft.argtypes = [c_void, c_int, c_int, c_int, _D, c_int, _D, _D]
     for name in EXPORTS:
         if name not in ("vs_last_error", "vs_abi_version"):
             getattr(L, name).restype = c_int
