@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--fp64-blocks", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     return ap.parse_args()
 
 
@@ -98,6 +99,33 @@ def main():
     fp64_rate = c64 * args.fp64_blocks * BLOCK / w64
     log(f"[bench] plain FP64 single tube ({c64} cells): {fp64_rate:.3e} cell-updates/s")
 
+    # ---------------- FP32 precision: F1-F4 vs FP64 (17 tubes, 50 ms) ----------------
+    precision = None
+    if not args.no_precision:
+        from vocalsim import analysis
+        pspecs = [tubes.vowel_tube(k, num_steps=33075, source=tubes.SOURCE_GAUSSIAN) for k in range(17)]
+        with vocalsim.Solver(pspecs, precision=64, threads=1024) as a64:
+            a64.run()
+            q64 = [(a64.probe(i), a64.source_trace(i)) for i in range(17)]
+        with vocalsim.Solver(pspecs, precision=32, threads=1024, kernel=_abi.K_FAST) as a32:
+            a32.run()
+            q32 = [(a32.probe(i), a32.source_trace(i)) for i in range(17)]
+        fs = 661500.0
+        dF, dB, rl2 = [], [], []
+        for (y64, u64), (y32, u32) in zip(q64, q32):
+            F64, B64 = analysis.formants(y64, u64, fs)
+            F32, B32 = analysis.formants(y32, u32, fs)
+            dF.append(np.abs(F32 - F64))
+            dB.append(np.abs(B32 - B64))
+            rl2.append(float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64)))
+        dF, dB = np.array(dF), np.array(dB)
+        precision = {"workload": "17 seeded 44-section vowel tubes, Gaussian pulse, 50 ms (33075 steps)",
+                     "max_abs_dF1_4_hz": float(np.nanmax(dF)), "mean_abs_dF1_4_hz": float(np.nanmean(dF)),
+                     "max_abs_dB1_4_hz": float(np.nanmax(dB)), "max_probe_rel_l2": float(max(rl2)),
+                     "reference": "FP64 exact GPU kernel (bit-exact to the CPU oracle)"}
+        log(f"[bench] FP32 vs FP64: max |dF1-4| = {precision['max_abs_dF1_4_hz']:.4f} Hz, "
+            f"max probe rel L2 = {precision['max_probe_rel_l2']:.2e}")
+
     # ---------------- CPU baseline (oracle, one core) ----------------
     cpu = None
     if not args.no_cpu:
@@ -139,6 +167,7 @@ def main():
         "fp64_plain": {"value": fp64_rate, "cells_per_step": c64,
                        "workload": "44-section vowel tube, two-mass 800 Pa, FP64, one tube"},
         "ratio_batched_fp32_over_plain_fp64": value / fp64_rate,
+        "precision_fp32_vs_fp64": precision,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_step (batched FP32)", "mean_launch_ms": mean_launch_ms,
