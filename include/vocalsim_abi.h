@@ -102,6 +102,7 @@ enum vs_field {
     VS_F_VX,              /* Real  [H*W] current Vx                                */
     VS_F_VY,              /* Real  [H*W] current Vy                                */
     VS_F_CELL_CLASS,      /* uint8 [H*W] packed class byte (type | segment-wall bit) */
+    VS_F_AS_F32 = 0x1000, /* flag for vs_field_ptr: float32 copy of a Real field (FP64 handles) */
     VS_F_COEFF0 = 100     /* + k, k in [0,17): Real [(H-2)*(W-2)] coefficient grid k
                              in getter order N_out, N_in, minVxBeta, minVyBeta,
                              maxVxSigmaPrimedt, maxVySigmaPrimedt,
@@ -116,7 +117,8 @@ enum vs_option {
     VS_OPT_THREADS = 1,          /* work-group size of the step kernel (64..1024) */
     VS_OPT_USE_GRAPH = 2,        /* 1: replay step blocks from a captured hipGraph */
     VS_OPT_KERNEL = 3,           /* step kernel family, see vs_kernel */
-    VS_OPT_TIMING = 4            /* 1: time every launch with HIP events */
+    VS_OPT_TIMING = 4,           /* 1: time every launch with HIP events */
+    VS_OPT_ENGINE_VARIANT = 5    /* 0: FDTDEngineCPU rounding, 1: FDTDEngineEigen (all tubes) */
 };
 
 enum vs_kernel {
@@ -160,7 +162,9 @@ int vs_set_option(vs_handle* h, int option, int value);
 int vs_get_option(const vs_handle* h, int option, int* value);
 
 /* Device pointer of a field (same ids as vs_get_field, tube's slice), and the
- * handle's stream as an opaque pointer (hipStream_t). */
+ * handle's stream as an opaque pointer (hipStream_t).  Coefficient grids,
+ * cell types and z_inv are materialised on first request into buffers the
+ * handle owns (the step kernels derive them on the fly). */
 int vs_field_ptr(vs_handle* h, int tube, int field, void** dptr, size_t* bytes);
 int vs_stream(vs_handle* h, void** stream);
 
@@ -168,6 +172,17 @@ int vs_stream(vs_handle* h, void** stream);
  * milliseconds, number of step-kernel launches and (with VS_OPT_TIMING) the
  * mean duration of one launch. */
 int vs_last_run_timing(const vs_handle* h, double* total_ms, int* launches, double* mean_launch_ms);
+
+/* Autotuner (no reference counterpart; the reference has no GPU time step).
+ * Times the step kernel over the work-group shapes that fit the batch (for
+ * the on-chip FP32 kernel: every multiple of 64 threads whose cells-per-lane
+ * count K and LDS footprint fit; for the exact kernel 256..1024 threads) and
+ * steps per launch {128, 256, 512}, each with a short warm-up and a timed
+ * block measured with HIP events, then keeps the fastest and resets the
+ * state (vs_reset).  Results are cached per (batch shape, precision, kernel)
+ * for the life of the process.  Requires steps_done == 0.  Writes a JSON
+ * report of the candidates into report (if non-NULL). */
+int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len);
 
 /* FP64 discrete-time Fourier transform X(f) = sum_n x[n] w[n] e^{-i 2 pi f n / fs}
  * at nf arbitrary frequencies [Hz] (w = Hann window when hann != 0, else 1).

@@ -1,0 +1,87 @@
+"""The C++ drop-in API (include/vocalsim/*.h, libvocalsim_dropin.so) driven by
+vs_demo, a driver written like the reference's main() (src/main.cu:65-146).
+
+CPU tests cover the host-side error paths; GPU tests compare the probe file
+written by FDTDEngineCPU / FDTDEngineEigen::startSolverEngine (6 significant
+digits, FDTDEngineCPU.cpp:435-450) with the oracle on the reference's own
+area-function files."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from vocalsim import tubes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DEMO = os.path.join(ROOT, "vocalsim-2p5d-fdtd_amd", "vs_demo")
+DATA = tubes.reference_data_dir() + "/"
+
+
+def run_demo(args, tmp_path, env_extra=None):
+    env = dict(os.environ, VOCALSIM_DATA_DIR=DATA)
+    if env_extra:
+        env.update(env_extra)
+    out = tmp_path / "recorded-pressure.txt"
+    cmd = [DEMO] + [str(a) for a in args] + [str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=300)
+    return r, out
+
+
+def test_demo_built():
+    assert os.path.exists(DEMO), "vs_demo missing: run __graft_entry__.build()"
+
+
+def test_regular_tube_flag_selects_no_file(tmp_path):
+    # TubeGeometry.cu:58-71: with tube_geometry = 0 no area file is chosen
+    r, _ = run_demo([1, 1, 0, 1, 1, 1, 15, 0.02, 1, 1, 0, 64, 100], tmp_path)
+    assert r.returncode == 1 and "Unable to open file" in r.stderr
+
+
+def test_open_space_rejected(tmp_path):
+    r, _ = run_demo([1, 0, 1, 1, 1, 1, 15, 0.02, 1, 1, 0, 64, 100], tmp_path)
+    assert r.returncode == 1 and "open-space" in r.stderr
+
+
+def _oracle_probe(vowel_file, shape, source, steps, variant=0, prec=64):
+    spec = tubes.read_area_file(DATA + vowel_file).replace(
+        shape=shape, source_type=source, num_steps=steps, pml=1, engine_variant=variant)
+    return orc.OracleTube(spec, prec).run()[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("vowel,fname,shape,source,engine", [
+    (1, "vocaltract_a.txt", 1, 1, 0),
+    (2, "vocaltract_i.txt", 2, 1, 1),
+    (3, "vocaltract_u.txt", 1, 2, 0),
+    (7, "cylindrical-tube.txt", 2, 0, 0),
+])
+def test_dropin_probe_matches_oracle(hip_device, tmp_path, vowel, fname, shape, source, engine):
+    steps = 800
+    print(f"[dropin] vowel={vowel} shape={shape} source={source} engine={engine}", flush=True)
+    r, out = run_demo([1, 1, 1, vowel, shape, 1, 15, 0.02, source, 1, engine, 64, steps], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "Elapsed time" in r.stdout
+    got = np.loadtxt(out)
+    ref = _oracle_probe(fname, shape, source, steps, variant=engine)
+    assert got.shape == ref.shape
+    # the text file keeps 6 significant digits
+    np.testing.assert_allclose(got, ref, rtol=2e-6, atol=1e-6 * np.abs(ref).max())
+
+
+@pytest.mark.gpu
+def test_dropin_fp32(hip_device, tmp_path):
+    r, out = run_demo([1, 1, 1, 1, 2, 1, 15, 0.02, 1, 1, 0, 32, 600], tmp_path)
+    assert r.returncode == 0, r.stderr
+    got = np.loadtxt(out)
+    ref = _oracle_probe("vocaltract_a.txt", 2, 1, 600, prec=32)
+    assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
+def test_dropin_float_step_count(hip_device, tmp_path):
+    # num_steps from the float-accumulated time vector (SolverParams.cu:59-66)
+    r, out = run_demo([1, 1, 1, 1, 1, 1, 15, 0.002, 1, 1, 0, 64, 0], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert np.loadtxt(out).size == orc.float_num_steps(0.002)

@@ -263,8 +263,17 @@ struct vs_handle {
     bool flux_valid = false;   // Uxf/Uyf hold the current state
     int max_interior = 0;      // largest (H-2)*(W-2) of the batch
     int lds_limit = 0;
+    // materialised export buffers (coefficients, types, z_inv) per tube
+    std::vector<void*> coeff_dev, types_dev, zinv_dev;
+    std::vector<std::pair<long long, void*>> f32_dev;   // (tube << 32 | field) -> float32 copy
+    // captured step blocks
+    int graph_blocks = 4;
+    int* d_t0 = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    long long graph_key = -1;
 
     ~vs_handle() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (void* p : allocs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -284,6 +293,11 @@ struct vs_handle {
 };
 
 namespace {
+
+long long graph_key(const vs_handle* h) {
+    return ((long long)h->kernel << 40) ^ ((long long)h->threads << 24) ^ ((long long)h->steps_per_launch << 8) ^
+           (long long)h->graph_blocks;
+}
 
 template <typename Real>
 Bufs<Real>& bufs_of(vs_handle* h);
@@ -429,6 +443,11 @@ int prepare_fast(vs_handle* h) {
             if (d.srate_mul != h->desc[0].srate_mul) return fail(VS_E_INVALID, "one srate_mul per batch on the flux-form kernel");
         h->fast_ready = true;
     }
+    {
+        const int K = (h->max_interior + h->threads - 64 - 1) / (h->threads - 64);
+        hipError_t e = configure_fast(K);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("configure: ") + hipGetErrorString(e));
+    }
     if (!h->flux_valid) {
         hipError_t e = launch_fast_setup(h->d_desc, h->desc.data(), h->n, h->bf, h->fb, h->stream);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("flux setup: ") + hipGetErrorString(e));
@@ -438,19 +457,47 @@ int prepare_fast(vs_handle* h) {
 }
 
 template <typename Real>
-hipError_t launch_any(vs_handle* h, Bufs<Real>& b, int t0, int ns);
+hipError_t launch_any(vs_handle* h, Bufs<Real>& b, int t0, int ns, const int* d_t0 = nullptr);
 template <>
-hipError_t launch_any<double>(vs_handle* h, Bufs<double>& b, int t0, int ns) {
-    return launch_step<double>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, h->stream);
+hipError_t launch_any<double>(vs_handle* h, Bufs<double>& b, int t0, int ns, const int* d_t0) {
+    return launch_step<double>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, d_t0, h->stream);
 }
 template <>
-hipError_t launch_any<float>(vs_handle* h, Bufs<float>& b, int t0, int ns) {
+hipError_t launch_any<float>(vs_handle* h, Bufs<float>& b, int t0, int ns, const int* d_t0) {
     if (h->kernel == VS_K_FAST) {
         const int K = (h->max_interior + h->threads - 64 - 1) / (h->threads - 64);
-        return launch_fast_step(K, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, h->stream);
+        return launch_fast_step(K, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, d_t0, h->stream);
     }
     h->flux_valid = false;
-    return launch_step<float>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, h->stream);
+    return launch_step<float>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, d_t0, h->stream);
+}
+
+// Captures G step blocks (step kernel + counter bump) into one graph.
+template <typename Real>
+int build_graph(vs_handle* h, Bufs<Real>& b) {
+    if (h->graph_exec) {
+        (void)hipGraphExecDestroy(h->graph_exec);
+        h->graph_exec = nullptr;
+    }
+    if (!h->d_t0) {
+        int rc = h->alloc(&h->d_t0, 1);
+        if (rc != VS_OK) return rc;
+    }
+    hipGraph_t g = nullptr;
+    VS_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < h->graph_blocks && e == hipSuccess; ++k) {
+        e = launch_any<Real>(h, b, 0, h->steps_per_launch, h->d_t0);
+        if (e == hipSuccess) e = launch_bump(h->d_t0, h->steps_per_launch, h->stream);
+    }
+    hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(VS_E_DEVICE, std::string("graph capture end: ") + hipGetErrorString(e2));
+    e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    h->graph_key = graph_key(h);
+    return VS_OK;
 }
 
 template <typename Real>
@@ -460,12 +507,39 @@ int run_steps(vs_handle* h, int n_steps) {
         const int rc = prepare_fast(h);
         if (rc != VS_OK) return rc;
     }
+    bool graph_ran = false;
+    if (h->use_graph && !h->timing) {
+        // whole graph replays (graph_blocks x steps_per_launch steps each), then
+        // the remainder with ordinary launches
+        const int per = h->graph_blocks * h->steps_per_launch;
+        const int start = h->steps_done;
+        const int end = std::min(h->max_steps, start + n_steps);
+        const int reps = (end - start) / per;
+        if (reps > 0) {
+            if (!h->graph_exec || h->graph_key != graph_key(h)) {
+                const int rc = build_graph<Real>(h, b);
+                if (rc != VS_OK) return rc;
+            }
+            VS_HIP(hipMemcpyAsync(h->d_t0, &h->steps_done, sizeof(int), hipMemcpyHostToDevice, h->stream));
+            VS_HIP(hipEventRecord(h->ev0, h->stream));
+            for (int k = 0; k < reps; ++k) VS_HIP(hipGraphLaunch(h->graph_exec, h->stream));
+            VS_HIP(hipEventRecord(h->ev1, h->stream));
+            if (h->kernel != VS_K_FAST) h->flux_valid = false;
+            h->steps_done = start + reps * per;
+            h->last_launches = reps * h->graph_blocks;
+            graph_ran = true;
+            // keep the counter in host memory alive until the copy has run
+            VS_HIP(hipStreamSynchronize(h->stream));
+        }
+        if (end - h->steps_done <= 0) return VS_OK;
+        n_steps = end - h->steps_done;
+    }
     const int start = h->steps_done;
     const int end = std::min(h->max_steps, start + n_steps);
     int launches = 0;
     double launch_ms = 0;
     std::vector<hipEvent_t> evs;
-    VS_HIP(hipEventRecord(h->ev0, h->stream));
+    if (!graph_ran) VS_HIP(hipEventRecord(h->ev0, h->stream));
     for (int t0 = start; t0 < end; t0 += h->steps_per_launch) {
         const int ns = std::min(h->steps_per_launch, end - t0);
         hipEvent_t a = nullptr, c = nullptr;
@@ -473,14 +547,14 @@ int run_steps(vs_handle* h, int n_steps) {
             VS_HIP(hipEventCreate(&a)); VS_HIP(hipEventCreate(&c));
             VS_HIP(hipEventRecord(a, h->stream));
         }
-        hipError_t e = launch_any<Real>(h, b, t0, ns);
+        hipError_t e = launch_any<Real>(h, b, t0, ns, nullptr);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("step launch: ") + hipGetErrorString(e));
         if (h->timing) { VS_HIP(hipEventRecord(c, h->stream)); evs.push_back(a); evs.push_back(c); }
         ++launches;
     }
     VS_HIP(hipEventRecord(h->ev1, h->stream));
     h->steps_done = end;
-    h->last_launches = launches;
+    h->last_launches = launches + (graph_ran ? h->last_launches : 0);
     if (h->timing) {
         VS_HIP(hipEventSynchronize(h->ev1));
         for (size_t i = 0; i < evs.size(); i += 2) {
@@ -585,6 +659,77 @@ int vs_get_dims(const vs_handle* h, int tube, vs_dims* out) {
 int vs_field_ptr(vs_handle* h, int tube, int field, void** dptr, size_t* bytes) {
     if (check_handle(h) || !dptr) return fail(VS_E_INVALID, "vs_field_ptr: bad argument");
     if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+    if (field & VS_F_AS_F32) {
+        field &= ~VS_F_AS_F32;
+        const bool real_field = field == VS_F_DEPTH_P || field == VS_F_DEPTH_X || field == VS_F_DEPTH_Y ||
+                                field == VS_F_Z_INV || field == VS_F_SOURCE || field == VS_F_PRESSURE ||
+                                field == VS_F_VX || field == VS_F_VY ||
+                                (field >= VS_F_COEFF0 && field < VS_F_COEFF0 + 17);
+        if (!real_field) return fail(VS_E_INVALID, "VS_F_AS_F32 applies to floating-point fields only");
+        if (h->precision == 64) {
+            const long long key = ((long long)tube << 32) | field;
+            for (auto& kv : h->f32_dev)
+                if (kv.first == key) {
+                    *dptr = kv.second;
+                    return VS_OK;
+                }
+            void* p64 = nullptr;
+            size_t nb = 0;
+            int rc = vs_field_ptr(h, tube, field, &p64, &nb);
+            if (rc != VS_OK) return rc;
+            const size_t n = nb / sizeof(double);
+            std::vector<double> host(n);
+            VS_HIP(hipStreamSynchronize(h->stream));
+            VS_HIP(hipMemcpy(host.data(), p64, nb, hipMemcpyDeviceToHost));
+            std::vector<float> f(host.begin(), host.end());
+            float* p = nullptr;
+            if ((rc = h->alloc(&p, n)) != VS_OK) return rc;
+            VS_HIP(hipMemcpy(p, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+            h->f32_dev.push_back({key, p});
+            *dptr = p;
+            if (bytes) *bytes = n * sizeof(float);
+            return VS_OK;
+        }
+    }
+    const TubeDesc& d = h->desc[tube];
+    const size_t es = elem_size(h), HW = (size_t)d.H * d.W, ni = (size_t)(d.H - 2) * (d.W - 2);
+    if (h->coeff_dev.empty()) {
+        h->coeff_dev.assign(h->n, nullptr);
+        h->types_dev.assign(h->n, nullptr);
+        h->zinv_dev.assign(h->n, nullptr);
+    }
+    if (field >= VS_F_COEFF0 && field < VS_F_COEFF0 + 17) {
+        if (!h->coeff_dev[tube]) {
+            void* p = nullptr;
+            int rc = h->alloc((char**)&p, ni * 17 * es);
+            if (rc != VS_OK) return rc;
+            hipError_t e = h->precision == 64
+                ? launch_coeffs<double>(h->d_desc, &d, tube, h->bd, (double*)p, h->stream)
+                : launch_coeffs<float>(h->d_desc, &d, tube, h->bf, (float*)p, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("coefficients: ") + hipGetErrorString(e));
+            h->coeff_dev[tube] = p;
+        }
+        *dptr = (char*)h->coeff_dev[tube] + (size_t)(field - VS_F_COEFF0) * ni * es;
+        if (bytes) *bytes = ni * es;
+        return VS_OK;
+    }
+    if (field == VS_F_CELL_TYPE || field == VS_F_Z_INV) {
+        std::vector<void*>& slot = field == VS_F_CELL_TYPE ? h->types_dev : h->zinv_dev;
+        const size_t nb = HW * (field == VS_F_CELL_TYPE ? sizeof(int) : es);
+        if (!slot[tube]) {
+            std::vector<char> host(nb);
+            int rc = vs_get_field(h, tube, field, host.data(), nb);
+            if (rc != VS_OK) return rc;
+            void* p = nullptr;
+            if ((rc = h->alloc((char**)&p, nb)) != VS_OK) return rc;
+            VS_HIP(hipMemcpy(p, host.data(), nb, hipMemcpyHostToDevice));
+            slot[tube] = p;
+        }
+        *dptr = slot[tube];
+        if (bytes) *bytes = nb;
+        return VS_OK;
+    }
     size_t count = 0, esz = 0;
     int rc = locate(h, tube, field, dptr, &count, &esz);
     if (rc != VS_OK) return rc;
@@ -733,6 +878,13 @@ int vs_set_option(vs_handle* h, int option, int value) {
         if (value != h->kernel && value == VS_K_FAST) h->flux_valid = false;
         h->kernel = value; return VS_OK;
     case VS_OPT_TIMING: h->timing = value ? 1 : 0; return VS_OK;
+    case VS_OPT_ENGINE_VARIANT: {
+        if (value != 0 && value != 1) return fail(VS_E_INVALID, "engine variant must be 0 or 1");
+        for (TubeDesc& d : h->desc) d.variant = value;
+        VS_HIP(hipMemcpyAsync(h->d_desc, h->desc.data(), h->n * sizeof(TubeDesc), hipMemcpyHostToDevice, h->stream));
+        VS_HIP(hipStreamSynchronize(h->stream));
+        return VS_OK;
+    }
     default: return fail(VS_E_INVALID, "unknown option");
     }
 }
@@ -745,6 +897,7 @@ int vs_get_option(const vs_handle* h, int option, int* value) {
     case VS_OPT_USE_GRAPH: *value = h->use_graph; return VS_OK;
     case VS_OPT_KERNEL: *value = h->kernel; return VS_OK;
     case VS_OPT_TIMING: *value = h->timing; return VS_OK;
+    case VS_OPT_ENGINE_VARIANT: *value = h->desc.empty() ? 0 : h->desc[0].variant; return VS_OK;
     default: return fail(VS_E_INVALID, "unknown option");
     }
 }
@@ -805,6 +958,89 @@ int vs_probe_dtft(vs_handle* h, int tube, int kind, int hann, const double* freq
     const double fs = 1.0 / d.ph.dt;
     VS_HIP(hipStreamSynchronize(h->stream));
     return dtft_device(base + d.src_off, n, fs, hann, freqs, nf, re, im, h->stream);
+}
+
+namespace {
+struct TuneKey {
+    int n, max_interior, precision, kernel;
+    long long cells;
+    bool operator==(const TuneKey& o) const {
+        return n == o.n && max_interior == o.max_interior && precision == o.precision && kernel == o.kernel &&
+               cells == o.cells;
+    }
+};
+std::vector<std::pair<TuneKey, std::pair<int, int>>> g_tune_cache;
+}  // namespace
+
+int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) {
+    if (check_handle(h)) return VS_E_INVALID;
+    if (h->steps_done != 0) return fail(VS_E_STATE, "vs_autotune needs a handle with no steps advanced");
+    if (timed_steps <= 0) timed_steps = 64;
+    const TuneKey key{h->n, h->max_interior, h->precision, h->kernel, h->cells_per_step};
+    std::string rep = "{\"candidates\": [";
+    for (auto& kv : g_tune_cache)
+        if (kv.first == key) {
+            h->threads = kv.second.first;
+            h->steps_per_launch = kv.second.second;
+            if (report && report_len) {
+                std::string r = "{\"cached\": true, \"threads\": " + std::to_string(h->threads) +
+                                ", \"steps_per_launch\": " + std::to_string(h->steps_per_launch) + "}";
+                std::snprintf(report, report_len, "%s", r.c_str());
+            }
+            return VS_OK;
+        }
+    std::vector<int> threads;
+    if (h->kernel == VS_K_FAST) {
+        for (int t = 1024; t >= 128; t -= 64) {
+            const int K = (h->max_interior + t - 64 - 1) / (t - 64);
+            if (K <= 24 && (long long)(fast_lds_bytes(K, t) + fast_lds_static_bytes()) <= h->lds_limit)
+                threads.push_back(t);
+        }
+    } else {
+        threads = {1024, 768, 512, 256};
+    }
+    if (threads.empty()) return fail(VS_E_INVALID, "no work-group shape fits this batch");
+    const int spls[3] = {128, 256, 512};
+    const int warm = 16;
+    const int old_timing = h->timing, old_graph = h->use_graph;
+    h->timing = 0;
+    h->use_graph = 0;
+    double best = 1e300;
+    int best_t = h->threads, best_s = h->steps_per_launch;
+    bool first = true;
+    for (int t : threads) {
+        for (int spl : spls) {
+            if (spl > 2 * timed_steps && spl != 128) continue;
+            h->threads = t;
+            h->steps_per_launch = spl;
+            int rc = vs_reset(h);
+            if (rc == VS_OK) rc = vs_run(h, warm);
+            if (rc == VS_OK) rc = vs_sync(h);
+            if (rc == VS_OK) rc = vs_run(h, timed_steps);
+            if (rc == VS_OK) rc = vs_sync(h);
+            if (rc != VS_OK) {
+                h->timing = old_timing; h->use_graph = old_graph;
+                return rc;
+            }
+            const int ran = h->steps_done - warm;
+            if (ran <= 0) continue;
+            const double us = h->last_ms * 1e3 / ran;
+            rep += std::string(first ? "" : ", ") + "{\"threads\": " + std::to_string(t) +
+                   ", \"steps_per_launch\": " + std::to_string(spl) + ", \"us_per_step\": " + std::to_string(us) + "}";
+            first = false;
+            if (us < best) { best = us; best_t = t; best_s = spl; }
+        }
+    }
+    h->threads = best_t;
+    h->steps_per_launch = best_s;
+    h->timing = old_timing;
+    h->use_graph = old_graph;
+    const int rc = vs_reset(h);
+    g_tune_cache.push_back({key, {best_t, best_s}});
+    rep += "], \"threads\": " + std::to_string(best_t) + ", \"steps_per_launch\": " + std::to_string(best_s) +
+           ", \"us_per_step\": " + std::to_string(best) + "}";
+    if (report && report_len) std::snprintf(report, report_len, "%s", rep.c_str());
+    return rc;
 }
 
 int vs_cells_per_step(const vs_handle* h, long long* cells) {

@@ -200,13 +200,15 @@ size_t fast_lds_static_bytes();
 hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<float> b,
                              FastBufs f, hipStream_t st);
 hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
-                            FastBufs f, int t0, int nsteps, int threads, hipStream_t st);
+                            FastBufs f, int t0, int nsteps, int threads, const int* d_t0, hipStream_t st);
+hipError_t launch_bump(int* d_t0, int s, hipStream_t st);
+hipError_t configure_fast(int K);
 template <typename Real>
 hipError_t launch_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<Real> b,
                         Real* tmpA, Real* tmpB, Real* tmpC, Real* tmpD, hipStream_t st);
 template <typename Real>
 hipError_t launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<Real> b,
-                       int t0, int nsteps, int threads, int kernel, hipStream_t st);
+                       int t0, int nsteps, int threads, int kernel, const int* d_t0, hipStream_t st);
 template <typename Real>
 hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tube, Bufs<Real> b,
                          Real* out17, hipStream_t st);

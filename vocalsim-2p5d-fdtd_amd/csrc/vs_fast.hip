@@ -122,7 +122,8 @@ __device__ inline uint32_t code_of(const uint32_t* cw, int k) {
 template <int K>
 __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__ tubes,
                                                     const int* __restrict__ order, Bufs<float> b,
-                                                    FastBufs f, int t0, int nsteps) {
+                                                    FastBufs f, int t0_arg, int nsteps,
+                                                    const int* __restrict__ d_t0) {
     extern __shared__ float lds[];
     // threads [0, T) own cells; the last wave is the control wave (source,
     // two-mass recurrence, probe).  Keeping the double-precision recurrence
@@ -138,6 +139,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
     const int tube = order[blockIdx.x];
     const TubeDesc& D = tubes[tube];
     const int H = D.H, W = D.W, Hi = H - 2, Wi = W - 2, N = Hi * Wi;
+    // graph replays read the first step from device memory
+    const int t0 = d_t0 ? *d_t0 : t0_arg;
     const int tend = min(t0 + nsteps, D.num_steps);
     const long long co = D.cell_off;
 
@@ -307,9 +310,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
 }
 
 template <int K>
-hipError_t launch_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
-                    int t0, int nsteps, int threads, hipStream_t st) {
-    const size_t lds = fast_lds_bytes(K, threads);
+hipError_t configure_k() {
     static bool configured = false;
     if (!configured) {
         hipError_t e = hipFuncSetAttribute((const void*)k_fast_step<K>,
@@ -318,7 +319,16 @@ hipError_t launch_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<flo
         if (e != hipSuccess) return e;
         configured = true;
     }
-    hipLaunchKernelGGL(k_fast_step<K>, dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0, nsteps);
+    return hipSuccess;
+}
+
+template <int K>
+hipError_t launch_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
+                    int t0, int nsteps, int threads, const int* d_t0, hipStream_t st) {
+    const size_t lds = fast_lds_bytes(K, threads);
+    hipError_t e = configure_k<K>();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fast_step<K>, dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0, nsteps, d_t0);
     return hipGetLastError();
 }
 
@@ -341,10 +351,21 @@ hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, i
     return hipGetLastError();
 }
 
-hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
-                            FastBufs f, int t0, int nsteps, int threads, hipStream_t st) {
+hipError_t configure_fast(int K) {
     switch (K) {
-#define VS_K(k) case k: return launch_k<k>(d_tubes, d_order, n, b, f, t0, nsteps, threads, st);
+#define VS_K(k) case k: return configure_k<k>();
+        VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
+        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(19) VS_K(20)
+        VS_K(21) VS_K(22) VS_K(23) VS_K(24)
+#undef VS_K
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
+                            FastBufs f, int t0, int nsteps, int threads, const int* d_t0, hipStream_t st) {
+    switch (K) {
+#define VS_K(k) case k: return launch_k<k>(d_tubes, d_order, n, b, f, t0, nsteps, threads, d_t0, st);
         VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
         VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(19) VS_K(20)
         VS_K(21) VS_K(22) VS_K(23) VS_K(24)

@@ -290,8 +290,10 @@ __global__ void k_coeffs(const TubeDesc* __restrict__ tubes, int tube, Bufs<Real
 template <typename Real>
 __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict__ tubes,
                                                      const int* __restrict__ order, Bufs<Real> b,
-                                                     int t0, int nsteps) {
+                                                     int t0_arg, int nsteps, const int* __restrict__ d_t0) {
     const int tube = order[blockIdx.x];
+    // graph replays read the first step from device memory
+    const int t0 = d_t0 ? *d_t0 : t0_arg;
     const TubeDesc& T = tubes[tube];
     __shared__ Tables<Real> tab;
     __shared__ Real s_exeV;
@@ -416,9 +418,21 @@ hipError_t launch_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n,
 
 template <typename Real>
 hipError_t launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<Real> b,
-                       int t0, int nsteps, int threads, int kernel, hipStream_t st) {
+                       int t0, int nsteps, int threads, int kernel, const int* d_t0, hipStream_t st) {
     (void)kernel;
-    hipLaunchKernelGGL(k_step_exact<Real>, dim3(n), dim3(threads), 0, st, d_tubes, d_order, b, t0, nsteps);
+    hipLaunchKernelGGL(k_step_exact<Real>, dim3(n), dim3(threads), 0, st, d_tubes, d_order, b, t0, nsteps, d_t0);
+    return hipGetLastError();
+}
+
+namespace {
+// Advances the device-resident first step of the next graph replay.
+__global__ void k_bump(int* d_t0, int s) {
+    if (threadIdx.x == 0) *d_t0 += s;
+}
+}  // namespace
+
+hipError_t launch_bump(int* d_t0, int s, hipStream_t st) {
+    hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, st, d_t0, s);
     return hipGetLastError();
 }
 
@@ -434,7 +448,7 @@ hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tu
     template hipError_t launch_setup<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>, Real*, \
                                            Real*, Real*, Real*, hipStream_t);                       \
     template hipError_t launch_step<Real>(const TubeDesc*, const int*, int, Bufs<Real>, int, int,    \
-                                          int, int, hipStream_t);                                   \
+                                          int, int, const int*, hipStream_t);                       \
     template hipError_t launch_coeffs<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,       \
                                             Real*, hipStream_t);
 

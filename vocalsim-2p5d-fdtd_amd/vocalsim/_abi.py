@@ -25,14 +25,15 @@ COEFFS = ["N_out", "N_in", "minVxBeta", "minVyBeta", "maxVxSigmaPrimedt",
           "pressureSigmaPrimedt", "areWeNotExcitationVx", "areWeNotExcitationVy",
           "excitationWeightVx", "excitationWeightVy", "xor_val1", "xor_val2", "xor_val3", "xor_val4"]
 
-OPT_STEPS_PER_LAUNCH, OPT_THREADS, OPT_USE_GRAPH, OPT_KERNEL, OPT_TIMING = range(5)
+OPT_STEPS_PER_LAUNCH, OPT_THREADS, OPT_USE_GRAPH, OPT_KERNEL, OPT_TIMING, OPT_ENGINE_VARIANT = range(6)
+F_AS_F32 = 0x1000
 K_EXACT, K_FAST = 0, 1
 
 EXPORTS = [
     "vs_last_error", "vs_abi_version", "vs_create", "vs_destroy", "vs_num_tubes", "vs_get_dims",
     "vs_get_field", "vs_run", "vs_sync", "vs_steps_done", "vs_reset", "vs_get_probe",
     "vs_get_source_trace", "vs_set_option", "vs_get_option", "vs_field_ptr", "vs_stream",
-    "vs_last_run_timing", "vs_cells_per_step", "vs_dtft", "vs_probe_dtft",
+    "vs_last_run_timing", "vs_cells_per_step", "vs_dtft", "vs_probe_dtft", "vs_autotune",
 ]
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -103,6 +104,7 @@ def load():
     L.vs_cells_per_step.argtypes = [c_void, ctypes.POINTER(ctypes.c_longlong)]
     L.vs_dtft.argtypes = [_D, c_int, ctypes.c_double, c_int, _D, c_int, _D, _D]
     L.vs_probe_dtft.argtypes = [c_void, c_int, c_int, c_int, _D, c_int, _D, _D]
+    L.vs_autotune.argtypes = [c_void, c_int, ctypes.c_char_p, c_size]
     for name in EXPORTS:
         if name not in ("vs_last_error", "vs_abi_version"):
             getattr(L, name).restype = c_int

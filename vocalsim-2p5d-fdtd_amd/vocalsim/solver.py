@@ -70,6 +70,14 @@ class Solver:
         check(load().vs_get_option(self.h, opt, ctypes.byref(v)))
         return v.value
 
+    def autotune(self, timed_steps: int = 64) -> dict:
+        """Picks the fastest work-group shape / steps per launch for this batch
+        (vs_autotune) and resets the state.  Returns the JSON report."""
+        import json
+        buf = ctypes.create_string_buffer(16384)
+        check(load().vs_autotune(self.h, int(timed_steps), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     # -- stepping ----------------------------------------------------------
     def run(self, n_steps: int = None, sync: bool = True):
         if n_steps is None:
