@@ -19,6 +19,11 @@ BYTES_FP32 = 37   # P, Vx, Vy read+write, dP, dX, dY read (4 B each), 1 B class
 BYTES_FP64 = 73
 HBM_PEAK_GBS = 8000.0
 BLOCK = 256
+# HBM bytes per 256-step launch of k_fast_step<19> on the 256-tube batch at 1024
+# threads, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes, each scaled by
+# the same counter over a plain copy kernel of known size (FETCH x2.000,
+# WRITE x1.000); profiles/k_fast_step_batch256_fp32_pmc.md.
+PMC_TRAFFIC_BYTES = 1.515e9
 
 
 def log(msg):
@@ -37,11 +42,31 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="counter calibration only: 5 launches of a plain FP32 copy kernel over this many MiB")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """Plain elementwise copy (y = x * 1) of a known size, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of the step kernels."""
+    import torch
+    torch.cuda.set_device(0)
+    x = torch.rand(mib * (1 << 18), device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel_bytes_read": mib << 20, "copy_kernel_bytes_written": mib << 20,
+                      "launches": 5}), flush=True)
 
 
 def main():
     args = parse()
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(tubes would shard by batch; see DESIGN.md).", file=sys.stderr)
@@ -169,7 +194,9 @@ def main():
         "ratio_batched_fp32_over_plain_fp64": value / fp64_rate,
         "precision_fp32_vs_fp64": precision,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": PMC_TRAFFIC_BYTES if (kernel == _abi.K_FAST and threads == 1024
+                                                      and args.tubes == 256) else None,
                      "kernel": "k_step (batched FP32)", "mean_launch_ms": mean_launch_ms,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,

@@ -199,6 +199,37 @@ int vs_probe_dtft(vs_handle* h, int tube, int kind, int hann, const double* freq
  * sum (frameH-2)(frameW-2)  (FDTDEngineCPU.cpp:316-317). */
 int vs_cells_per_step(const vs_handle* h, long long* cells);
 
+/* Reverse-time (adjoint) gradients.  No reference counterpart: the reference
+ * only runs the forward time loop (FDTDEngineCPU.cpp:316-410); these entry
+ * points differentiate that loop, written in flux form U = V * depth, which
+ * is algebraically the same step.
+ *
+ * vs_adjoint_forward runs every tube of the handle from the zero state over
+ * its num_steps in the handle's precision, keeps a state checkpoint
+ * (P, Ux, Uy) every checkpoint_every steps in device memory, and writes the
+ * probe traces (tube-major, num_steps values per tube) into probe when probe
+ * is non-NULL.  It leaves the handle's ordinary state (vs_run) untouched.
+ * Table sources only (source_type 0 or 1): a two-mass source feeds back the
+ * pressure and is rejected with VS_E_INVALID.
+ *
+ * vs_adjoint_backward, after vs_adjoint_forward, evaluates the gradient of
+ * J = sum_tubes sum_n gprobe[n] * probe[n] (gprobe tube-major like probe)
+ * with respect to each section's area_cm2, depth_scale and wall admittance
+ * mu3d (outputs section-major per tube, tubes concatenated; n_sections_total
+ * = sum of num_sections).  Segments of checkpoint_every steps are recomputed
+ * from their checkpoint and reversed step by step on the GPU; the per-cell
+ * coefficient gradients are then chained on the host through the depth
+ * clamps, the depth-P average, the half-cell smoothing, the ellipse depth
+ * profile and the admittance map.  The integer geometry (diameters in cells)
+ * is held fixed: its dependence on the area is piecewise constant.  A
+ * circular cross-section's depth does not read the area, so its area
+ * gradient is 0; so are the area and depth-scale gradients of the 2D solver.
+ * Memory: checkpoints ceil(num_steps / checkpoint_every) * 3 * H * W values
+ * and a segment slab (checkpoint_every + 1) * 3 * H * W values per tube. */
+int vs_adjoint_forward(vs_handle* h, int checkpoint_every, double* probe, size_t probe_len);
+int vs_adjoint_backward(vs_handle* h, const double* gprobe, size_t gprobe_len, double* g_area,
+                        double* g_depth_scale, double* g_mu3d, size_t n_sections_total);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,4 +36,6 @@ def test_null_arguments_rejected_without_device():
     assert L.vs_create(None, 1, 64, ctypes.byref(h)) == -1
     assert b"specs" in L.vs_last_error()
     assert L.vs_run(None, 10) == -1
+    assert L.vs_adjoint_forward(None, 512, None, 0) == -1
+    assert L.vs_adjoint_backward(None, None, 0, None, None, None, 0) == -1
     assert L.vs_destroy(None) == 0

@@ -271,10 +271,20 @@ struct vs_handle {
     int* d_t0 = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     long long graph_key = -1;
+    // per-section inputs kept for the gradient chain (tube-major, sec_off)
+    std::vector<double> sec_area, sec_mu, sec_scale;
+    // reverse-time pass
+    AdjBufs<float> af{};
+    AdjBufs<double> ad{};
+    bool adj_ready = false;    // per-cell buffers allocated
+    int adj_S = 0;             // checkpoint spacing of the slabs held
+    bool adj_forward = false;  // checkpoints hold a completed forward pass
+    std::vector<void*> adj_slabs;
 
     ~vs_handle() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (void* p : allocs) (void)hipFree(p);
+        for (void* p : adj_slabs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -365,10 +375,15 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
         const Prelude& p = pre[k];
         cum.insert(cum.end(), p.cum.begin(), p.cum.end());
         diam.insert(diam.end(), p.diam.begin(), p.diam.end());
+        const float mu_def = specs[k].mu3d_default > 0 ? specs[k].mu3d_default
+                                                       : (specs[k].fdtd_solver_type == 1 ? 0.025f : 0.005f);
         for (int i = 0; i < specs[k].num_sections; ++i) {
             minor.push_back((Real)p.minor[i]);
             dsc.push_back((Real)p.dscale[i]);
             zs.push_back((Real)p.zsec[i]);
+            h->sec_area.push_back((double)specs[k].area_cm2[i]);
+            h->sec_mu.push_back((double)(specs[k].mu3d ? specs[k].mu3d[i] : mu_def));
+            h->sec_scale.push_back(p.dscale[i]);
         }
         std::vector<Real> tab;
         source_table<Real>(specs[k], p.ph, p.num_steps, tab);
@@ -1047,6 +1062,293 @@ int vs_cells_per_step(const vs_handle* h, long long* cells) {
     if (check_handle(h) || !cells) return fail(VS_E_INVALID, "vs_cells_per_step: bad argument");
     *cells = h->cells_per_step;
     return VS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reverse-time pass: gradients of J = sum_n g[n] probe[n] with respect to
+// each section's area, depth scale and wall admittance.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename Real>
+AdjBufs<Real>& adj_of(vs_handle* h);
+template <>
+AdjBufs<float>& adj_of<float>(vs_handle* h) { return h->af; }
+template <>
+AdjBufs<double>& adj_of<double>(vs_handle* h) { return h->ad; }
+
+// d z_inv / d mu of the admittance map z_inv(mu) (FDTDSolver.cu:613-617):
+// z = (1 - s) / (rho c (1 + s)), s = sqrt(1 - a), a = 1 / (1/2 + (m + 1/m) / 4),
+// m = mu pi / 1.84.
+double dz_dmu(double mu, double rho_c) {
+    const double kap = (2.0 * 0.5 * M_PI) / 1.84;
+    const double m2 = mu * kap;
+    const double a = 1.0 / (0.5 + 0.25 * (m2 + 1.0 / m2));
+    const double sq = std::sqrt(1.0 - a);
+    const double da_dm = -a * a * 0.25 * (1.0 - 1.0 / (m2 * m2));
+    const double ds_da = -0.5 / sq;
+    const double dz_ds = -2.0 / (rho_c * (1.0 + sq) * (1.0 + sq));
+    return dz_ds * ds_da * da_dm * kap;
+}
+
+// Allocates the per-cell buffers (once) and the checkpoint / segment slabs
+// for checkpoint spacing S (again when S changes).
+template <typename Real>
+int adj_prepare(vs_handle* h, int S) {
+    AdjBufs<Real>& a = adj_of<Real>(h);
+    Bufs<Real>& b = bufs_of<Real>(h);
+    int rc;
+    if (!h->adj_ready) {
+        const long long n = h->total_cells;
+        if ((rc = h->alloc(&a.aP, n)) || (rc = h->alloc(&a.kP, n)) || (rc = h->alloc(&a.ax, n)) ||
+            (rc = h->alloc(&a.ay, n)) || (rc = h->alloc(&a.cx, n)) || (rc = h->alloc(&a.cy, n)) ||
+            (rc = h->alloc(&a.fx, n)) || (rc = h->alloc(&a.fy, n)) || (rc = h->alloc(&a.P, n)) ||
+            (rc = h->alloc(&a.Ux, n)) || (rc = h->alloc(&a.Uy, n)) || (rc = h->alloc(&a.LP, n)) ||
+            (rc = h->alloc(&a.LX, n)) || (rc = h->alloc(&a.LY, n)) || (rc = h->alloc(&a.M, n)) ||
+            (rc = h->alloc(&a.GkP, n)) || (rc = h->alloc(&a.Gx, n)) || (rc = h->alloc(&a.Gy, n)) ||
+            (rc = h->alloc(&a.probe, h->total_src)) || (rc = h->alloc(&a.gprobe, h->total_src)) ||
+            (rc = h->alloc(&a.ck_off, h->n)) || (rc = h->alloc(&a.seg_off, h->n)))
+            return rc;
+        hipError_t e = launch_adj_coeffs<Real>(h->d_desc, h->desc.data(), h->n, b, a, h->stream);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint coefficients: ") + hipGetErrorString(e));
+        h->adj_ready = true;
+    }
+    if (h->adj_S != S) {
+        VS_HIP(hipStreamSynchronize(h->stream));
+        for (void* p : h->adj_slabs) (void)hipFree(p);
+        h->adj_slabs.clear();
+        a.ck = a.seg = nullptr;
+        h->adj_S = 0;
+        h->adj_forward = false;
+        std::vector<long long> cko(h->n), sgo(h->n);
+        long long ck = 0, sg = 0;
+        for (int k = 0; k < h->n; ++k) {
+            const TubeDesc& d = h->desc[k];
+            const long long st = 3LL * d.H * d.W;
+            cko[k] = ck;
+            sgo[k] = sg;
+            ck += (((long long)d.num_steps + S - 1) / S) * st;
+            sg += ((long long)S + 1) * st;
+        }
+        void* p1 = nullptr;
+        void* p2 = nullptr;
+        hipError_t e = hipMalloc(&p1, sizeof(Real) * (size_t)std::max(ck, 1LL));
+        if (e != hipSuccess)
+            return fail(VS_E_DEVICE, "checkpoint slab of " + std::to_string(sizeof(Real) * ck) + " bytes: " +
+                                         hipGetErrorString(e));
+        h->adj_slabs.push_back(p1);
+        e = hipMalloc(&p2, sizeof(Real) * (size_t)std::max(sg, 1LL));
+        if (e != hipSuccess)
+            return fail(VS_E_DEVICE, "segment slab of " + std::to_string(sizeof(Real) * sg) + " bytes: " +
+                                         hipGetErrorString(e));
+        h->adj_slabs.push_back(p2);
+        a.ck = static_cast<Real*>(p1);
+        a.seg = static_cast<Real*>(p2);
+        VS_HIP(hipMemcpyAsync(a.ck_off, cko.data(), h->n * sizeof(long long), hipMemcpyHostToDevice, h->stream));
+        VS_HIP(hipMemcpyAsync(a.seg_off, sgo.data(), h->n * sizeof(long long), hipMemcpyHostToDevice, h->stream));
+        VS_HIP(hipStreamSynchronize(h->stream));
+        a.S = S;
+        h->adj_S = S;
+    }
+    return VS_OK;
+}
+
+int adj_check(vs_handle* h) {
+    for (int k = 0; k < h->n; ++k)
+        if (h->desc[k].source_type == 2)
+            return fail(VS_E_INVALID, "tube " + std::to_string(k) +
+                                          ": the reverse-time pass needs a table source (source_type 0 or 1)");
+    if (h->threads < 64 || h->threads > 1024 || h->threads % 64)
+        return fail(VS_E_INVALID, "threads must be a multiple of 64 in [64, 1024]");
+    return VS_OK;
+}
+
+template <typename Real>
+int adj_forward(vs_handle* h, int S, double* probe, size_t len) {
+    int rc = adj_check(h);
+    if (rc != VS_OK) return rc;
+    if (probe && len < (size_t)h->total_src)
+        return fail(VS_E_RANGE, "probe buffer needs " + std::to_string(h->total_src) + " values");
+    if ((rc = adj_prepare<Real>(h, S)) != VS_OK) return rc;
+    AdjBufs<Real>& a = adj_of<Real>(h);
+    Bufs<Real>& b = bufs_of<Real>(h);
+    const size_t cb = sizeof(Real) * (size_t)h->total_cells;
+    VS_HIP(hipMemsetAsync(a.P, 0, cb, h->stream));
+    VS_HIP(hipMemsetAsync(a.Ux, 0, cb, h->stream));
+    VS_HIP(hipMemsetAsync(a.Uy, 0, cb, h->stream));
+    for (int t0 = 0; t0 < h->max_steps; t0 += S) {
+        hipError_t e = launch_adj_fwd<Real>(h->d_desc, h->d_order, h->n, b, a, t0, S, 1, -1, h->threads, h->stream);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint forward: ") + hipGetErrorString(e));
+    }
+    VS_HIP(hipStreamSynchronize(h->stream));
+    h->adj_forward = true;
+    if (probe)
+        VS_HIP(hipMemcpy(probe, a.probe, sizeof(double) * (size_t)h->total_src, hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+template <typename T>
+int fetch(std::vector<T>& dst, const T* src, long long n) {
+    dst.resize((size_t)n);
+    VS_HIP(hipMemcpy(dst.data(), src, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+template <typename Real>
+int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area, double* g_scale, double* g_mu,
+                 size_t nsec) {
+    if (!h->adj_forward) return fail(VS_E_STATE, "vs_adjoint_backward needs a completed vs_adjoint_forward");
+    if (!gprobe || glen != (size_t)h->total_src)
+        return fail(VS_E_RANGE, "gprobe needs " + std::to_string(h->total_src) + " values (tube-major)");
+    if (nsec != (size_t)h->total_secs)
+        return fail(VS_E_RANGE, "gradient buffers need " + std::to_string(h->total_secs) + " values");
+    AdjBufs<Real>& a = adj_of<Real>(h);
+    Bufs<Real>& b = bufs_of<Real>(h);
+    const int S = h->adj_S;
+    VS_HIP(hipMemcpyAsync(a.gprobe, gprobe, sizeof(double) * glen, hipMemcpyHostToDevice, h->stream));
+    const size_t cr = sizeof(Real) * (size_t)h->total_cells, cd = sizeof(double) * (size_t)h->total_cells;
+    VS_HIP(hipMemsetAsync(a.LP, 0, cr, h->stream));
+    VS_HIP(hipMemsetAsync(a.LX, 0, cr, h->stream));
+    VS_HIP(hipMemsetAsync(a.LY, 0, cr, h->stream));
+    VS_HIP(hipMemsetAsync(a.M, 0, cr, h->stream));
+    VS_HIP(hipMemsetAsync(a.GkP, 0, cd, h->stream));
+    VS_HIP(hipMemsetAsync(a.Gx, 0, cd, h->stream));
+    VS_HIP(hipMemsetAsync(a.Gy, 0, cd, h->stream));
+    const int nseg = (h->max_steps + S - 1) / S;
+    for (int k = nseg - 1; k >= 0; --k) {
+        hipError_t e = launch_adj_fwd<Real>(h->d_desc, h->d_order, h->n, b, a, k * S, S, 2, k, h->threads, h->stream);
+        if (e == hipSuccess) e = launch_adj_rev<Real>(h->d_desc, h->d_order, h->n, b, a, k * S, S, h->threads, h->stream);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint reverse: ") + hipGetErrorString(e));
+    }
+    // raw depth maps for the chain through smoothing and clamping
+    long long maxHW = 0;
+    for (auto& d : h->desc) maxHW = std::max(maxHW, (long long)d.H * d.W);
+    Real* raw = nullptr;
+    VS_HIP(hipMalloc(&raw, sizeof(Real) * (size_t)maxHW * h->n * 2));
+    hipError_t e = launch_depth_raw<Real>(h->d_desc, h->desc.data(), h->n, b, raw, raw + maxHW * h->n, maxHW, h->stream);
+    hipError_t e2 = hipStreamSynchronize(h->stream);
+    std::vector<Real> rawh;
+    int rc = VS_OK;
+    if (e == hipSuccess && e2 == hipSuccess) rc = fetch(rawh, raw, maxHW * h->n * 2);
+    (void)hipFree(raw);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("raw depth: ") + hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(VS_E_DEVICE, std::string("reverse pass: ") + hipGetErrorString(e2));
+    if (rc != VS_OK) return rc;
+    std::vector<double> GkP, Gx, Gy;
+    std::vector<Real> kP, cx, cy, dP, dX, dY, zcol, minor, dscale;
+    std::vector<uint8_t> cls;
+    std::vector<int> seg, upper, lower;
+    const long long C = h->total_cells;
+    if ((rc = fetch(GkP, a.GkP, C)) || (rc = fetch(Gx, a.Gx, C)) || (rc = fetch(Gy, a.Gy, C)) ||
+        (rc = fetch(kP, a.kP, C)) || (rc = fetch(cx, a.cx, C)) || (rc = fetch(cy, a.cy, C)) ||
+        (rc = fetch(dP, b.dP, C)) || (rc = fetch(dX, b.dX, C)) || (rc = fetch(dY, b.dY, C)) ||
+        (rc = fetch(cls, b.cls, C)) || (rc = fetch(zcol, b.zcol, h->total_cols)) ||
+        (rc = fetch(seg, b.seg, h->total_walls)) || (rc = fetch(upper, b.upper, h->total_walls)) ||
+        (rc = fetch(lower, b.lower, h->total_walls)) || (rc = fetch(minor, b.minor, h->total_secs)) ||
+        (rc = fetch(dscale, b.dscale, h->total_secs)))
+        return rc;
+    for (int k = 0; k < h->n; ++k) {
+        const TubeDesc& d = h->desc[k];
+        const int H = d.H, W = d.W, HW = H * W;
+        const long long base = d.cell_off;
+        const Real* X = rawh.data() + (size_t)k * maxHW;
+        const Real* Y = rawh.data() + (size_t)(maxHW * h->n) + (size_t)k * maxHW;
+        const int mid = d.midY * W + d.te;
+        const Real open = (Real)0.50f, mnR = (Real)0.001f;
+        const bool depth3d = d.solver_type == 1;
+        std::vector<double> gX(HW, 0.0), gY(HW, 0.0), gXs(HW, 0.0), gYs(HW, 0.0), gzc(W, 0.0);
+        // smoothed maps, in the arithmetic of the setup kernel
+        std::vector<Real> Xs(HW), Ys(HW);
+        for (int i = 0; i < HW; ++i) {
+            const int r = i / W, c = i - r * W;
+            const Real mx = X[mid], my = Y[mid];
+            Real xc = X[i]; if (xc == open) xc = mx;
+            Real yc = Y[i]; if (yc == open) yc = my;
+            Real xr = mx;
+            if (c < W - 1) { xr = X[i + 1]; if (xr == open) xr = mx; }
+            Real ya = my;
+            if (r > 0) { ya = Y[i - W]; if (ya == open) ya = my; }
+            Xs[i] = (Real)0.5f * (xc + xr);
+            Ys[i] = (Real)0.5f * (yc + ya);
+        }
+        for (int r = 1; r < H - 1; ++r)
+            for (int c = 1; c < W - 1; ++c) {
+                const int i = r * W + c;
+                const long long q = base + i;
+                if (depth3d) {
+                    const double gdp = -GkP[q] * (double)kP[q] / (double)dP[q];
+                    const double gdx = Gx[q] * (double)cx[q] / (double)dX[q];
+                    const double gdy = Gy[q] * (double)cy[q] / (double)dY[q];
+                    if (!(Xs[i] < mnR)) gXs[i] += gdx;
+                    if (!(Ys[i] < mnR)) gYs[i] += gdy;
+                    const Real lX = Xs[i - 1], dYv = Ys[i + W];
+                    const Real p = (Real)0.25f * (Xs[i] + Ys[i] + lX + dYv);
+                    if (!(p < mnR)) {
+                        const double g4 = 0.25 * gdp;
+                        gXs[i] += g4; gYs[i] += g4; gXs[i - 1] += g4; gYs[i + W] += g4;
+                    }
+                }
+                const int tc = cls[q] & kTypeMask;
+                if ((cls[q] & kSegWallBit) && tc != kExcitation) {
+                    const double z = (double)zcol[d.col_off + c];
+                    if (z != 0.0) gzc[c] += (Gx[q] * (double)cx[q] + Gy[q] * (double)cy[q]) / z;
+                }
+            }
+        // half-cell smoothing, transposed (open-space cells read the mouth-end cell)
+        for (int i = 0; i < HW && depth3d; ++i) {
+            const int r = i / W, c = i - r * W;
+            const double tx = 0.5 * gXs[i], ty = 0.5 * gYs[i];
+            gX[X[i] == open ? mid : i] += tx;
+            gX[(c < W - 1 && X[i + 1] != open) ? i + 1 : mid] += tx;
+            gY[Y[i] == open ? mid : i] += ty;
+            gY[(r > 0 && Y[i - W] != open) ? i - W : mid] += ty;
+        }
+        const long long so = d.sec_off;
+        std::vector<double> gminor(d.nsec, 0.0), gsc(d.nsec, 0.0), gz(d.nsec, 0.0);
+        for (int j = 0; j < d.L; ++j) {
+            const int c = d.ts + j, sc = seg[d.wall_off + j];
+            gz[sc] += gzc[c];
+            if (!depth3d) continue;
+            for (int r = upper[d.wall_off + j] + 1; r < lower[d.wall_off + j]; ++r) {
+                const int i = r * W + c;
+                const double v = gX[i] * (double)X[i] + gY[i] * (double)Y[i];
+                gsc[sc] += v / (double)dscale[so + sc];
+                gminor[sc] += v / (double)minor[so + sc];
+            }
+        }
+        const double rho_c = d.ph.rho * d.ph.c;
+        for (int s = 0; s < d.nsec; ++s) {
+            // minor axis = 2 sqrt(A / (q pi)): d minor / d A = minor / (2 A); the
+            // circular shape's depth does not read the minor axis
+            g_area[so + s] = (depth3d && d.shape == 2) ? gminor[s] * (double)minor[so + s] / (2.0 * h->sec_area[so + s]) : 0.0;
+            g_scale[so + s] = depth3d ? gsc[s] : 0.0;
+            g_mu[so + s] = gz[s] * dz_dmu(h->sec_mu[so + s], rho_c);
+        }
+    }
+    return VS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vs_adjoint_forward(vs_handle* h, int checkpoint_every, double* probe, size_t probe_len) {
+    if (int rc = check_handle(h)) return rc;
+    if (checkpoint_every < 1 || checkpoint_every > (1 << 20))
+        return fail(VS_E_INVALID, "checkpoint_every must be in [1, 2^20]");
+    return h->precision == VS_FP64 ? adj_forward<double>(h, checkpoint_every, probe, probe_len)
+                                   : adj_forward<float>(h, checkpoint_every, probe, probe_len);
+}
+
+int vs_adjoint_backward(vs_handle* h, const double* gprobe, size_t gprobe_len, double* g_area, double* g_depth_scale,
+                        double* g_mu3d, size_t n_sections_total) {
+    if (int rc = check_handle(h)) return rc;
+    if (!g_area || !g_depth_scale || !g_mu3d) return fail(VS_E_INVALID, "null gradient buffer");
+    return h->precision == VS_FP64
+               ? adj_backward<double>(h, gprobe, gprobe_len, g_area, g_depth_scale, g_mu3d, n_sections_total)
+               : adj_backward<float>(h, gprobe, gprobe_len, g_area, g_depth_scale, g_mu3d, n_sections_total);
 }
 
 }  // extern "C"

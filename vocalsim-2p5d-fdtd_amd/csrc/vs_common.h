@@ -192,7 +192,25 @@ struct FastBufs {
     float lut[8];                        // damping factor by code
 };
 
-// Kernel launchers (vs_kernels.hip, vs_fast.hip).
+// Buffers of the reverse-time (adjoint) pass (vs_adjoint.hip).  Per-cell
+// arrays share the cell layout of Bufs; ck and seg are per-tube slabs of
+// whole states (P, Ux, Uy; 3 * H * W values each).
+template <typename Real>
+struct AdjBufs {
+    Real *aP, *kP, *ax, *ay, *cx, *cy;   // flux-form coefficients
+    uint8_t *fx, *fy;                    // face kind: 0 ordinary, 1 wall admittance, 2 source
+    Real *P, *Ux, *Uy;                   // forward state
+    Real *LP, *LX, *LY, *M;              // adjoint state and the pressure adjoint of a step
+    double *GkP, *Gx, *Gy;               // accumulated d J / d (kP, cx, cy)
+    Real* ck;                            // checkpoints, every S steps
+    Real* seg;                           // the S + 1 states of the segment being reversed
+    long long *ck_off, *seg_off;         // per tube slab offsets
+    double* probe;                       // per step: listener pressure of the flux form
+    double* gprobe;                      // per step: d J / d probe
+    int S;
+};
+
+// Kernel launchers (vs_kernels.hip, vs_fast.hip, vs_adjoint.hip).
 hipError_t launch_dtft(const double* d_x, long long stride, int nsig, int n, double fs, int hann,
                        const double* d_f, int nf, double* d_re, double* d_im, hipStream_t st);
 size_t fast_lds_bytes(int K, int threads);
@@ -212,5 +230,17 @@ hipError_t launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<
 template <typename Real>
 hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tube, Bufs<Real> b,
                          Real* out17, hipStream_t st);
+template <typename Real>
+hipError_t launch_depth_raw(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<Real> b, Real* X,
+                            Real* Y, long long stride, hipStream_t st);
+template <typename Real>
+hipError_t launch_adj_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<Real> b,
+                             AdjBufs<Real> a, hipStream_t st);
+template <typename Real>
+hipError_t launch_adj_fwd(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<Real> b, AdjBufs<Real> a,
+                          int t0, int ns, int mode, int load, int threads, hipStream_t st);
+template <typename Real>
+hipError_t launch_adj_rev(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<Real> b, AdjBufs<Real> a,
+                          int s0, int ns, int threads, hipStream_t st);
 
 }  // namespace vs

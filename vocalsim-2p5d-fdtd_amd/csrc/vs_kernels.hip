@@ -416,6 +416,17 @@ hipError_t launch_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n,
     return hipGetLastError();
 }
 
+// Raw (unsmoothed) depth maps only, one stride-long slice per tube.
+template <typename Real>
+hipError_t launch_depth_raw(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<Real> b, Real* X,
+                            Real* Y, long long stride, hipStream_t st) {
+    int maxHW = 0;
+    for (int k = 0; k < n; ++k) maxHW = max(maxHW, h_tubes[k].H * h_tubes[k].W);
+    dim3 gc((maxHW + kSetupThreads - 1) / kSetupThreads, n);
+    hipLaunchKernelGGL(k_depth_raw<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, b, X, Y, stride);
+    return hipGetLastError();
+}
+
 template <typename Real>
 hipError_t launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<Real> b,
                        int t0, int nsteps, int threads, int kernel, const int* d_t0, hipStream_t st) {
@@ -450,7 +461,9 @@ hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tu
     template hipError_t launch_step<Real>(const TubeDesc*, const int*, int, Bufs<Real>, int, int,    \
                                           int, int, const int*, hipStream_t);                       \
     template hipError_t launch_coeffs<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,       \
-                                            Real*, hipStream_t);
+                                            Real*, hipStream_t);                                    \
+    template hipError_t launch_depth_raw<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,    \
+                                               Real*, Real*, long long, hipStream_t);
 
 VS_INSTANTIATE(float)
 VS_INSTANTIATE(double)

@@ -1,0 +1,73 @@
+"""Differentiable tube simulation for PyTorch.
+
+:class:`TubeProbe` is a ``torch.autograd.Function`` whose forward runs the
+differentiable forward pass of a batch of tubes on the GPU
+(``vs_adjoint_forward``: the FDTD time step of FDTDEngineCPU.cpp:316-410 in
+flux form, with a state checkpoint every ``checkpoint_every`` steps) and whose
+backward runs the hand-written reverse-time kernels
+(``vs_adjoint_backward``): each checkpointed segment is recomputed, reversed
+step by step, and the per-cell coefficient gradients are chained to the
+per-section inputs.
+
+Inputs are ``[B, S]`` tensors of section areas [cm^2], depth scales and wall
+admittances (mu3d); the output is the ``[B, N]`` probe pressure of every tube
+(all tubes of one call share S sections and N steps).  The library stores the
+inputs as float32 (the reference's float32 area function,
+TubeGeometry.cu:83-88), so the gradient is taken at the float32-rounded
+values.  The integer geometry (diameters in cells) is held fixed.
+"""
+from __future__ import annotations
+
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from .solver import Solver
+from .tubes import TubeSpec
+
+
+class TubeProbe(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, area, depth_scale, mu3d, specs: Sequence[TubeSpec], precision: int = 64,
+                checkpoint_every: int = 512):
+        if area.dim() != 2 or area.shape != depth_scale.shape or area.shape != mu3d.shape:
+            raise ValueError("area, depth_scale and mu3d must be [B, S] tensors of one shape")
+        B, S = area.shape
+        if len(specs) != B:
+            raise ValueError(f"need {B} tube specs, got {len(specs)}")
+        a = area.detach().to("cpu", torch.float64).numpy()
+        d = depth_scale.detach().to("cpu", torch.float64).numpy()
+        m = mu3d.detach().to("cpu", torch.float64).numpy()
+        run = []
+        for b, sp in enumerate(specs):
+            if sp.num_sections != S:
+                raise ValueError(f"tube {b} has {sp.num_sections} sections, expected {S}")
+            run.append(sp.replace(areas_cm2=a[b].astype(np.float32), depth_scale=d[b].astype(np.float32),
+                                  mu3d=m[b].astype(np.float32)))
+        solver = Solver(run, precision=precision)
+        probes = solver.adjoint_forward(checkpoint_every)
+        if len({p.size for p in probes}) != 1:
+            solver.close()
+            raise ValueError("all tubes of one call must have the same number of steps")
+        ctx.solver = solver
+        ctx.meta = (area.device, area.dtype)
+        return torch.from_numpy(np.stack(probes)).to(area.device)
+
+    @staticmethod
+    def backward(ctx, gprobe):
+        device, dtype = ctx.meta
+        g = gprobe.detach().to("cpu", torch.float64).contiguous().numpy()
+        ga, gs, gm = ctx.solver.adjoint_backward([g[b] for b in range(g.shape[0])])
+        ctx.solver.close()
+        ctx.solver = None
+        to = lambda x: torch.from_numpy(np.stack(x)).to(device=device, dtype=dtype)
+        return to(ga), to(gs), to(gm), None, None, None
+
+
+def tube_probe(area: torch.Tensor, depth_scale: torch.Tensor, mu3d: torch.Tensor, specs: Sequence[TubeSpec],
+               precision: int = 64, checkpoint_every: int = 512) -> torch.Tensor:
+    """Probe pressure [B, N] of the tubes ``specs`` with their area function,
+    depth scale and wall admittance replaced by the given [B, S] tensors;
+    differentiable with respect to all three."""
+    return TubeProbe.apply(area, depth_scale, mu3d, list(specs), precision, checkpoint_every)

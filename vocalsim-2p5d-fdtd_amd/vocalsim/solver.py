@@ -110,6 +110,42 @@ class Solver:
         check(load().vs_cells_per_step(self.h, ctypes.byref(v)))
         return v.value
 
+    # -- reverse-time gradients --------------------------------------------
+    def adjoint_forward(self, checkpoint_every: int = 512) -> List[np.ndarray]:
+        """Differentiable forward pass from the zero state (vs_adjoint_forward):
+        returns each tube's probe trace and keeps checkpoints for
+        :meth:`adjoint_backward`.  The ordinary state (run/reset) is untouched."""
+        total = sum(d["num_steps"] for d in self.dims)
+        out = np.zeros(total, dtype=np.float64)
+        check(load().vs_adjoint_forward(self.h, int(checkpoint_every), out.ctypes.data_as(_abi._D), total))
+        self._adj_total = total
+        return self._split_steps(out)
+
+    def adjoint_backward(self, gprobe: Sequence[np.ndarray]):
+        """Gradients of J = sum_tubes sum_n gprobe[t][n] probe[t][n] with respect to
+        every section's area_cm2, depth_scale and mu3d (vs_adjoint_backward).
+        Returns three lists of per-tube arrays."""
+        if len(gprobe) != self.n:
+            raise ValueError(f"need {self.n} gradient traces, got {len(gprobe)}")
+        parts = []
+        for t, g in enumerate(gprobe):
+            g = np.asarray(g, dtype=np.float64).ravel()
+            if g.size != self.dims[t]["num_steps"]:
+                raise ValueError(f"tube {t}: gradient trace needs {self.dims[t]['num_steps']} values")
+            parts.append(g)
+        g = np.ascontiguousarray(np.concatenate(parts))
+        ns = sum(d["num_sections"] for d in self.dims)
+        ga, gs, gm = np.zeros(ns), np.zeros(ns), np.zeros(ns)
+        check(load().vs_adjoint_backward(self.h, g.ctypes.data_as(_abi._D), g.size,
+                                         ga.ctypes.data_as(_abi._D), gs.ctypes.data_as(_abi._D),
+                                         gm.ctypes.data_as(_abi._D), ns))
+        split = np.cumsum([d["num_sections"] for d in self.dims])[:-1]
+        return np.split(ga, split), np.split(gs, split), np.split(gm, split)
+
+    def _split_steps(self, flat: np.ndarray) -> List[np.ndarray]:
+        split = np.cumsum([d["num_steps"] for d in self.dims])[:-1]
+        return np.split(flat, split)
+
     # -- outputs -----------------------------------------------------------
     def probe(self, tube: int = 0) -> np.ndarray:
         n = self.dims[tube]["num_steps"]
