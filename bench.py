@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-tubes", type=int, default=64)
+    ap.add_argument("--adj-steps", type=int, default=13230)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="counter calibration only: 5 launches of a plain FP32 copy kernel over this many MiB")
     return ap.parse_args()
@@ -60,6 +63,52 @@ def copy_calibration(mib):
     torch.cuda.synchronize()
     print(json.dumps({"copy_kernel_bytes_read": mib << 20, "copy_kernel_bytes_written": mib << 20,
                       "launches": 5}), flush=True)
+
+
+def adjoint_leg(args, np, torch, vocalsim, tubes, _abi):
+    """Loss = mean squared difference of ln|Y(f)| (0-5 kHz, 1024 bins, FP64 DFT
+    of the probe) against targets from other seeds; FP32 forward with 512-step
+    checkpoints, then the reverse-time pass to area, depth scale and
+    admittance of every section."""
+    from vocalsim.adjoint import dft_basis, log_magnitude, tube_probe
+    n, B = args.adj_steps, args.adj_tubes
+    fs = 44100.0 * tubes.SRATE_MUL
+
+    def spec_of(seed):
+        sp = tubes.vowel_tube(seed, num_steps=n, source=tubes.SOURCE_GAUSSIAN)
+        return sp.replace(depth_scale=np.ones(sp.num_sections, np.float32),
+                          mu3d=np.full(sp.num_sections, 0.005, np.float32))
+
+    specs = [spec_of(k) for k in range(B)]
+    basis = dft_basis(n, fs, np.linspace(0.0, 5000.0, 1024))
+    with vocalsim.Solver([spec_of(64 + k) for k in range(B)], precision=32, threads=1024,
+                         kernel=_abi.K_FAST) as s:
+        s.run()
+        target = log_magnitude(torch.tensor(np.stack([s.probe(i) for i in range(B)]), device="cuda"), basis)
+        cells = s.cells_per_step
+    dev = "cuda"
+    A = torch.tensor(np.stack([sp.areas_cm2 for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
+    D = torch.tensor(np.stack([sp.depth_scale for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
+    M = torch.tensor(np.stack([sp.mu3d for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p = tube_probe(A, D, M, specs, 32, 512)
+    loss = ((log_magnitude(p, basis) - target) ** 2).mean()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out = {"workload": f"{B} seeded 44-section vowel tubes, Gaussian pulse, {n} steps; loss = mean squared "
+                       f"ln|Y(f)| difference over 1024 bins 0-5 kHz against {B} other seeds",
+           "precision": "f32 state, f64 gradient accumulation", "checkpoint_every": 512,
+           "cells_per_step": cells, "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
+           "forward_cell_updates_per_s": cells * n / (t1 - t0),
+           "gradient_cell_updates_per_s": cells * n / (t2 - t0),
+           "loss": float(loss.detach()), "grad_norm_area": float(A.grad.norm())}
+    log(f"[bench] gradient workload: forward {out['forward_ms']:.0f} ms, backward {out['backward_ms']:.0f} ms "
+        f"({B} tubes x {n} steps)")
+    return out
 
 
 def main():
@@ -151,6 +200,11 @@ def main():
         log(f"[bench] FP32 vs FP64: max |dF1-4| = {precision['max_abs_dF1_4_hz']:.4f} Hz, "
             f"max probe rel L2 = {precision['max_probe_rel_l2']:.2e}")
 
+    # ---------------- gradients: log-spectrum loss over a batch of tubes ----------------
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(args, np, torch, vocalsim, tubes, _abi)
+
     # ---------------- CPU baseline (oracle, one core) ----------------
     cpu = None
     if not args.no_cpu:
@@ -200,6 +254,7 @@ def main():
                      "kernel": "k_step (batched FP32)", "mean_launch_ms": mean_launch_ms,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

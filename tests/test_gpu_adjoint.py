@@ -198,3 +198,49 @@ def test_torch_function_backward(hip_device):
     g = float(area.grad[0, sec])
     print(f"[torch adjoint] dL/dA[{sec}] adjoint={g:.10e} fd={fd:.10e}", flush=True)
     assert abs(g - fd) <= FD_REL * abs(fd)
+
+
+def test_log_spectrum_loss_gradient_four_tubes(hip_device):
+    """The gradient workload: log-magnitude spectra (0-5 kHz, 1024 bins, FP64
+    DFT of the probe) of 4 tubes against targets from 4 other seeds, FP64
+    adjoint, 512-step checkpoints; central FD on one parameter of each kind."""
+    import torch
+    from vocalsim.adjoint import dft_basis, log_magnitude, tube_probe
+
+    N = 2000
+    fs = 44100.0 * tubes.SRATE_MUL
+
+    def spec_of(seed):
+        sp = tubes.vowel_tube(seed, num_steps=N, source=tubes.SOURCE_GAUSSIAN)
+        return sp.replace(depth_scale=np.ones(sp.num_sections, np.float32),
+                          mu3d=np.full(sp.num_sections, 0.005, np.float32))
+
+    specs = [spec_of(k) for k in range(4)]
+    basis = dft_basis(N, fs, np.linspace(0.0, 5000.0, 1024))
+    with Solver([spec_of(64 + k) for k in range(4)], precision=64) as s:
+        s.run()
+        target = log_magnitude(torch.tensor(np.stack([s.probe(i) for i in range(4)]), device="cuda"), basis)
+
+    def loss_of(probe):
+        return ((log_magnitude(probe, basis) - target) ** 2).mean()
+
+    dev = "cuda"
+    A = torch.tensor(np.stack([sp.areas_cm2 for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
+    D = torch.tensor(np.stack([sp.depth_scale for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
+    M = torch.tensor(np.stack([sp.mu3d for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
+    loss = loss_of(tube_probe(A, D, M, specs, 64, 512))
+    loss.backward()
+
+    def L(perturbed_specs):
+        with Solver(perturbed_specs, precision=64) as s:
+            s.run()
+            p = torch.tensor(np.stack([s.probe(i) for i in range(4)]), device="cuda")
+        return float(loss_of(p))
+
+    for kind, grad, tube, sec in (("area", A.grad, 1, 10), ("scale", D.grad, 2, 30), ("mu", M.grad, 3, 5)):
+        sp_p, vp = perturbed(specs[tube], kind, sec, +1)
+        sp_m, vm = perturbed(specs[tube], kind, sec, -1)
+        fd = (L(specs[:tube] + [sp_p] + specs[tube + 1:]) - L(specs[:tube] + [sp_m] + specs[tube + 1:])) / (vp - vm)
+        g = float(grad[tube, sec])
+        print(f"[log-spectrum loss] {kind}[{tube},{sec}] adjoint={g:.10e} fd={fd:.10e}", flush=True)
+        assert abs(g - fd) <= FD_REL * abs(fd), (kind, g, fd)

@@ -30,7 +30,7 @@ from .tubes import TubeSpec
 class TubeProbe(torch.autograd.Function):
     @staticmethod
     def forward(ctx, area, depth_scale, mu3d, specs: Sequence[TubeSpec], precision: int = 64,
-                checkpoint_every: int = 512):
+                checkpoint_every: int = 512, threads: int = 1024):
         if area.dim() != 2 or area.shape != depth_scale.shape or area.shape != mu3d.shape:
             raise ValueError("area, depth_scale and mu3d must be [B, S] tensors of one shape")
         B, S = area.shape
@@ -45,7 +45,7 @@ class TubeProbe(torch.autograd.Function):
                 raise ValueError(f"tube {b} has {sp.num_sections} sections, expected {S}")
             run.append(sp.replace(areas_cm2=a[b].astype(np.float32), depth_scale=d[b].astype(np.float32),
                                   mu3d=m[b].astype(np.float32)))
-        solver = Solver(run, precision=precision)
+        solver = Solver(run, precision=precision, threads=threads)
         probes = solver.adjoint_forward(checkpoint_every)
         if len({p.size for p in probes}) != 1:
             solver.close()
@@ -62,12 +62,30 @@ class TubeProbe(torch.autograd.Function):
         ctx.solver.close()
         ctx.solver = None
         to = lambda x: torch.from_numpy(np.stack(x)).to(device=device, dtype=dtype)
-        return to(ga), to(gs), to(gm), None, None, None
+        return to(ga), to(gs), to(gm), None, None, None, None
 
 
 def tube_probe(area: torch.Tensor, depth_scale: torch.Tensor, mu3d: torch.Tensor, specs: Sequence[TubeSpec],
-               precision: int = 64, checkpoint_every: int = 512) -> torch.Tensor:
+               precision: int = 64, checkpoint_every: int = 512, threads: int = 1024) -> torch.Tensor:
     """Probe pressure [B, N] of the tubes ``specs`` with their area function,
     depth scale and wall admittance replaced by the given [B, S] tensors;
     differentiable with respect to all three."""
-    return TubeProbe.apply(area, depth_scale, mu3d, list(specs), precision, checkpoint_every)
+    return TubeProbe.apply(area, depth_scale, mu3d, list(specs), precision, checkpoint_every, threads)
+
+
+def dft_basis(n: int, fs: float, freqs, device="cuda"):
+    """cos / sin bases [n, F] in float64 for X(f) = sum_k x[k] exp(-i 2 pi f k / fs).
+    The phase f k is reduced modulo fs (one cycle) before the trigonometric
+    call; the reduction is exact when f is an integer number of Hz."""
+    f = torch.as_tensor(np.asarray(freqs, dtype=np.float64), device=device)
+    k = torch.arange(n, dtype=torch.float64, device=device)
+    ph = torch.remainder(k[:, None] * f[None, :], fs) * (2.0 * np.pi / fs)
+    return torch.cos(ph), torch.sin(ph)
+
+
+def log_magnitude(x: torch.Tensor, basis) -> torch.Tensor:
+    """ln |X(f)| of signals x [B, n] at the frequencies of ``basis`` (dft_basis)."""
+    c, s = basis
+    re = x @ c
+    im = x @ s
+    return 0.5 * torch.log(re * re + im * im)
