@@ -74,6 +74,11 @@ def lib():
         L.vso_float_num_steps.restype = ctypes.c_int
         L.vso_float_num_steps.argtypes = [ctypes.c_float, ctypes.c_int]
         L.vso_coeff_triple.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
+        L.vso_plane_contour.argtypes = [ctypes.c_float, _F, _F, ctypes.c_int, _D]
+        L.vso_polygon_area.restype = ctypes.c_float
+        L.vso_polygon_area.argtypes = [_F, _F, ctypes.c_int]
+        L.vso_vertical_chord.argtypes = [_F, _F, ctypes.c_int, ctypes.c_float, _F, _F]
+        L.vso_segment_cross.argtypes = [_F, _F, _F]
         _lib = L
     return _lib
 
@@ -176,3 +181,44 @@ def coeff_triple(cur: int, right: int, top: int, precision: int = 64, srate_mul:
     if lib().vso_coeff_triple(precision, srate_mul, cur, right, top, _dptr(out)) != 0:
         raise OracleError("bad cell type")
     return out
+
+
+def plane_contour(area_cm2: float):
+    """Aligned single-plane-symmetric contour of one section: (y, z, info) with
+    info = dict(minY, minZ, maxZ, diameter, area, steps)."""
+    info = np.zeros(6)
+    n = lib().vso_plane_contour(area_cm2, None, None, 0, _dptr(info))
+    if n < 0:
+        raise OracleError(lib().vso_last_error().decode())
+    y = np.zeros(n, dtype=np.float32)
+    z = np.zeros(n, dtype=np.float32)
+    lib().vso_plane_contour(area_cm2, _fptr(y), _fptr(z), n, _dptr(info))
+    keys = ["minY", "minZ", "maxZ", "diameter", "area", "steps"]
+    return y, z, dict(zip(keys, info.tolist()))
+
+
+def polygon_area(y, z) -> float:
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    z = np.ascontiguousarray(z, dtype=np.float32)
+    return float(lib().vso_polygon_area(_fptr(y), _fptr(z), y.size))
+
+
+def vertical_chord(y, z, y0: float):
+    """(lowest, highest) crossing of the vertical line y = y0 with the
+    polyline, or None."""
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    z = np.ascontiguousarray(z, dtype=np.float32)
+    lo = np.zeros(1, dtype=np.float32)
+    hi = np.zeros(1, dtype=np.float32)
+    if not lib().vso_vertical_chord(_fptr(y), _fptr(z), y.size, y0, _fptr(lo), _fptr(hi)):
+        return None
+    return float(lo[0]), float(hi[0])
+
+
+def segment_cross(p, q):
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    q = np.ascontiguousarray(q, dtype=np.float32)
+    out = np.zeros(2, dtype=np.float32)
+    if not lib().vso_segment_cross(_fptr(p), _fptr(q), _fptr(out)):
+        return None
+    return float(out[0]), float(out[1])

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -135,8 +136,172 @@ int float_num_steps(float audio_dur, float dt) {
 /* Geometry (float32 integerisation): TubeGeometry.cu:53-179, 182-237,  */
 /* 241-286, 316-364, 433-612                                             */
 /* ------------------------------------------------------------------ */
+/* ------------------------------------------------------------------ */
+/* Single-plane-symmetric contour (ShapeGenerator.cpp:35-318,           */
+/* ShapeAlignment.cu:11-152, Utils.cpp:59-122).  Boost.Geometry's        */
+/* linestring intersection and ring area are restated as the crossing of */
+/* two segments and the shoelace sum, both in double rounded to float.   */
+/* ------------------------------------------------------------------ */
+struct PlaneContour {
+    std::vector<float> y, z;
+    float minY = 0, minZ = 0, maxZ = 0, diameter = 0, area = 0;
+    int steps = 0;
+};
+
+std::vector<float> linspace(float a, float b, int n) {
+    std::vector<float> v;
+    if (n == 0) return v;
+    if (n == 1) { v.push_back(a); return v; }
+    float step = (b - a) / (n - 1);
+    for (int i = 0; i < n; ++i) v.push_back(a + step * i);
+    return v;
+}
+
+struct Pt { float y, z; };
+
+std::vector<Pt> ellipse_upper(float y0, float y1, float a, float b, int n, float yc) {
+    std::vector<float> ys = linspace(y0, y1, n);
+    std::vector<float> dd(n);
+    for (int i = 0; i < n; ++i) { float t = ys[i] - yc; dd[i] = t * t; }
+    float aa = a * a, bb = b * b;
+    dd[0] = aa;
+    for (int i = 0; i < n / 2; ++i) dd[n - i - 1] = dd[i];
+    std::vector<Pt> out(n);
+    for (int i = 0; i < n; ++i) out[i] = {ys[i], sqrtf(std::max(0.0f, bb * (1.0f - dd[i] / aa)))};
+    return out;
+}
+
+/* crossing point of segments p0p1 and q0q1, if any */
+bool seg_cross(Pt p0, Pt p1, Pt q0, Pt q1, Pt& out) {
+    double rx = (double)p1.y - p0.y, rz = (double)p1.z - p0.z;
+    double sx = (double)q1.y - q0.y, sz = (double)q1.z - q0.z;
+    double den = rx * sz - rz * sx;
+    if (den == 0.0) return false;
+    double qx = (double)q0.y - p0.y, qz = (double)q0.z - p0.z;
+    double u = (qx * sz - qz * sx) / den;
+    double v = (qx * rz - qz * rx) / den;
+    if (u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0) return false;
+    out = {(float)((double)p0.y + u * rx), (float)((double)p0.z + u * rz)};
+    return true;
+}
+
+/* first crossing of two polylines in the order of their segments */
+bool polyline_cross(const std::vector<Pt>& a, const std::vector<Pt>& b, Pt& out) {
+    for (size_t i = 0; i + 1 < a.size(); ++i)
+        for (size_t j = 0; j + 1 < b.size(); ++j) {
+            if (std::max(b[j].y, b[j + 1].y) < std::min(a[i].y, a[i + 1].y)) continue;
+            if (std::min(b[j].y, b[j + 1].y) > std::max(a[i].y, a[i + 1].y)) continue;
+            if (seg_cross(a[i], a[i + 1], b[j], b[j + 1], out)) return true;
+        }
+    return false;
+}
+
+float shoelace(const std::vector<Pt>& r) {
+    double acc = 0;
+    for (size_t i = 0; i < r.size(); ++i) {
+        const Pt& p = r[i];
+        const Pt& q = r[(i + 1) % r.size()];
+        acc += (double)p.y * q.z - (double)q.y * p.z;
+    }
+    return (float)std::fabs(acc * 0.5);
+}
+
+/* lowest and highest crossing of the vertical line y = y0 with a polyline */
+bool vertical_chord(const std::vector<float>& y, const std::vector<float>& z, float y0, float& lo, float& hi) {
+    lo = std::numeric_limits<float>::infinity();
+    hi = -std::numeric_limits<float>::infinity();
+    for (size_t k = 0; k + 1 < y.size(); ++k) {
+        float ya = y[k], yb = y[k + 1];
+        if (y0 < std::min(ya, yb) || y0 > std::max(ya, yb)) continue;
+        if (ya == yb) {
+            lo = std::min(lo, std::min(z[k], z[k + 1]));
+            hi = std::max(hi, std::max(z[k], z[k + 1]));
+            continue;
+        }
+        double t = ((double)y0 - ya) / ((double)yb - ya);
+        float zc = (float)((double)z[k] + ((double)z[k + 1] - z[k]) * t);
+        lo = std::min(lo, zc);
+        hi = std::max(hi, zc);
+    }
+    return lo <= hi;
+}
+
+std::vector<Pt> joined_contour(float a1, float b1, float a2, float b2, float c1, float c2, int nh, bool& ok) {
+    std::vector<Pt> e1 = ellipse_upper(c1 - a1, c1 + a1, a1, b1, nh, c1);
+    std::vector<Pt> e2 = ellipse_upper(c2 - a2, c2 + a2, a2, b2, nh, c2);
+    Pt ip{0, 0};
+    ok = polyline_cross(e1, e2, ip);
+    std::vector<Pt> out;
+    if (!ok) return out;
+    std::vector<float> ya = linspace(c1 - a1, ip.y, nh - 1), yb = linspace(ip.y, c2 + a2, nh - 1);
+    for (float y : ya) out.push_back({y, sqrtf(std::max(0.0f, (b1 * b1) * (1.0f - ((y - c1) * (y - c1)) / (a1 * a1))))});
+    out.push_back(ip);
+    for (float y : yb) out.push_back({y, sqrtf(std::max(0.0f, (b2 * b2) * (1.0f - ((y - c2) * (y - c2)) / (a2 * a2))))});
+    return out;
+}
+
+bool plane_contour(float area_cm2, PlaneContour& pc) {
+    const int nh = (1999 + 1) / 2;
+    float A = area_cm2 * fu::CENTIMETER * fu::CENTIMETER;
+    float m = 3.0f, x = A / (m + 1.0f), A1 = m * x, A2 = 1.0f * x;
+    float l1 = sqrtf(A1 / (M_PI * 3.0f * 1.0f)), l2 = sqrtf(A2 / (M_PI * 1.0f * 1.0f));
+    float a1 = l1 * 3.0f, b1 = l1 * 1.0f, a2 = l2 * 1.0f, b2 = l2 * 1.0f;
+    float c1 = 0.0f, c2 = c1 + a1 + 0.003f * fu::MILLIMETER;
+    bool ok = false;
+    std::vector<Pt> half = joined_contour(a1, b1, a2, b2, c1, c2, nh, ok);
+    if (!ok) { g_err = "contour shapes do not intersect"; return false; }
+    float area = 2 * shoelace(half);
+    float step = fu::MILLIMETER * 0.01f;
+    int steps = 0;
+    while (area < A) {
+        a1 = a1 + step; b1 = b1 + step; a2 = a2 + step; b2 = b2 + step;
+        half = joined_contour(a1, b1, a2, b2, c1, c2, nh, ok);
+        if (!ok) { g_err = "contour shapes do not intersect"; return false; }
+        area = 2 * shoelace(half);
+        ++steps;
+    }
+    half.front().z = 0.0f;
+    half.back().z = 0.0f;
+    std::vector<Pt> full = half;
+    for (int i = (int)half.size() - 2; i >= 0; --i) full.push_back({half[i].y, -1 * half[i].z});
+    /* maximum chord: first point attaining the largest distance, and its first farthest point */
+    const int n = (int)full.size();
+    float best = 0.0f;
+    int bi = 0, bj = 0;
+    for (int i = 0; i < n; ++i) {
+        float mi = 0.0f;
+        int mj = 0;
+        for (int j = 0; j < n; ++j) {
+            float dy = full[i].y - full[j].y, dz = full[i].z - full[j].z;
+            float d = sqrtf(dy * dy + dz * dz);
+            if (d > mi) { mi = d; mj = j; }
+        }
+        if (mi > best) { best = mi; bi = i; bj = mj; }
+    }
+    float theta = std::atan2(full[bj].z - full[bi].z, full[bj].y - full[bi].y);
+    float c = std::cos(-1 * theta), sn = std::sin(-1 * theta);
+    for (Pt& p : full) {
+        float y = p.y, z = p.z;
+        p.y = c * y - sn * z;
+        p.z = sn * y + c * z;
+    }
+    float mnY = full[0].y, mnZ = full[0].z;
+    for (const Pt& p : full) { mnY = std::min(mnY, p.y); mnZ = std::min(mnZ, p.z); }
+    float shY = (mnY < 0.0f ? -mnY : 0.0f) + 0.05f, shZ = (mnZ < 0.0f ? -mnZ : 0.0f) + 0.05f;
+    pc.y.clear(); pc.z.clear();
+    for (Pt& p : full) { pc.y.push_back(p.y + shY); pc.z.push_back(p.z + shZ); }
+    pc.minY = *std::min_element(pc.y.begin(), pc.y.end());
+    pc.minZ = *std::min_element(pc.z.begin(), pc.z.end());
+    pc.maxZ = *std::max_element(pc.z.begin(), pc.z.end());
+    pc.diameter = best;
+    pc.area = area;
+    pc.steps = steps;
+    return true;
+}
+
 struct Geometry {
     int nsec = 0;
+    std::vector<PlaneContour> plane;
     std::vector<float> len_m, width_m, minor_m;  /* sagittal width, ellipse minor axis */
     std::vector<int> diam_cells;
     float tube_length = 0;
@@ -187,8 +352,14 @@ bool build_geometry(const vso_spec& s, float ds, Geometry& g) {
             float lenX = sqrtf(a_m2 / (major * minor * M_PI));
             g.width_m.push_back(lenX * (major * 2));
             g.minor_m.push_back(lenX * (minor * 2));
+        } else if (s.cross_sectional_shape == 3) {
+            PlaneContour pc;
+            if (!plane_contour(s.area_cm2[i], pc)) return false;
+            g.width_m.push_back(pc.diameter);
+            g.minor_m.push_back(0.0f);
+            g.plane.push_back(pc);
         } else {
-            g_err = "cross_sectional_shape must be 1 (circular) or 2 (elliptical)"; return false;
+            g_err = "cross_sectional_shape must be 1 (circular), 2 (elliptical) or 3 (single-plane symmetric)"; return false;
         }
     }
     for (int i = 0; i < g.nsec; ++i) g.len_m.push_back(s.length_cm[i] * fu::CENTIMETER);
@@ -592,6 +763,30 @@ bool Tube<Real>::build_depth(const vso_spec& s) {
         int air = lo - up - 1;
         int half = (int)ceilf(air * 0.5f) - 1;
         Real scale = s.depth_scale ? (Real)s.depth_scale[g.seg[j]] : (Real)1;
+        if (s.cross_sectional_shape == 3) {
+            /* TubeDepth.cu:493-640: half the vertical chord of the section's
+             * contour, positions counted from the lower wall */
+            int maxseg = 0;
+            for (int y = 0; y < H; ++y) maxseg = std::max(maxseg, bst[(size_t)y * W + x]);
+            const PlaneContour& pc = g.plane[maxseg - 1];
+            for (int cc = 1; cc <= air; ++cc) {
+                float px = (cc - 0.5) * ds_f32;
+                float py = cc * ds_f32;
+                float zl, zh;
+                if (!vertical_chord(pc.y, pc.z, pc.minY + px, zl, zh)) { g_err = "depth: no contour crossing"; return false; }
+                Real vx = ((Real)zh - (Real)zl) / (Real)2.0f;
+                Real vy = 0;
+                if (cc < air) {
+                    if (!vertical_chord(pc.y, pc.z, pc.minY + py, zl, zh)) { g_err = "depth: no contour crossing"; return false; }
+                    vy = ((Real)zh - (Real)zl) / (Real)2.0f;
+                }
+                X[(size_t)(lo - cc) * W + x] = vx * scale;
+                Y[(size_t)(lo - cc) * W + x] = vy * scale;
+            }
+            X[(size_t)up * W + x] = 0; X[(size_t)lo * W + x] = 0;
+            Y[(size_t)up * W + x] = 0; Y[(size_t)lo * W + x] = 0;
+            continue;
+        }
         Real sa2 = 0, sb2 = 0, r2 = 0;
         if (s.cross_sectional_shape == 1) {
             Real radius = (static_cast<Real>(air) / 2.0f) * ds;
@@ -1021,6 +1216,39 @@ int vso_coeff_triple(int precision, int srate_mul, int cur, int right, int top, 
         for (int k = 0; k < 17; ++k) out17[k] = o[k];
     }
     return 0;
+}
+
+int vso_plane_contour(float area_cm2, float* y, float* z, int cap, double* info6) {
+    PlaneContour pc;
+    if (!plane_contour(area_cm2, pc)) return -1;
+    const int n = (int)pc.y.size();
+    if (y && z) {
+        if (cap < n) return -1;
+        for (int i = 0; i < n; ++i) { y[i] = pc.y[i]; z[i] = pc.z[i]; }
+    }
+    if (info6) {
+        info6[0] = pc.minY; info6[1] = pc.minZ; info6[2] = pc.maxZ;
+        info6[3] = pc.diameter; info6[4] = pc.area; info6[5] = pc.steps;
+    }
+    return n;
+}
+
+float vso_polygon_area(const float* y, const float* z, int n) {
+    std::vector<Pt> r(n);
+    for (int i = 0; i < n; ++i) r[i] = {y[i], z[i]};
+    return shoelace(r);
+}
+
+int vso_vertical_chord(const float* y, const float* z, int n, float y0, float* lo, float* hi) {
+    std::vector<float> yy(y, y + n), zz(z, z + n);
+    return vertical_chord(yy, zz, y0, *lo, *hi) ? 1 : 0;
+}
+
+int vso_segment_cross(const float* p4, const float* q4, float* out2) {
+    Pt o{0, 0};
+    if (!seg_cross({p4[0], p4[1]}, {p4[2], p4[3]}, {q4[0], q4[1]}, {q4[2], q4[3]}, o)) return 0;
+    out2[0] = o.y; out2[1] = o.z;
+    return 1;
 }
 
 }  // extern "C"

@@ -98,6 +98,18 @@ int   vso_float_num_steps(float audio_dur, int srate_mul);
 /* 17 solver coefficients of one (cur, right, top) type triple. */
 int   vso_coeff_triple(int precision, int srate_mul, int cur, int right, int top, double* out17);
 
+/* Single-plane-symmetric contour of one section (ShapeGenerator.cpp:35-197,
+ * ShapeAlignment.cu:11-113): writes the aligned closed contour (returns its
+ * point count, or -1) and info6 = {minY, minZ, maxZ, max chord, contour area,
+ * growth steps}.  y/z may be NULL to query the count. */
+int   vso_plane_contour(float area_cm2, float* y, float* z, int cap, double* info6);
+/* Geometry helpers of that path: shoelace |area| of a closed ring, lowest and
+ * highest crossing of a vertical line with a polyline (returns 1 if any), and
+ * the crossing of two segments p = (y0,z0,y1,z1), q likewise (returns 1 if any). */
+float vso_polygon_area(const float* y, const float* z, int n);
+int   vso_vertical_chord(const float* y, const float* z, int n, float y0, float* lo, float* hi);
+int   vso_segment_cross(const float* p4, const float* q4, float* out2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,11 @@ def _cases():
             shape=tubes.SHAPE_ELLIPTICAL, num_steps=600),
         "cylinder_file_circle": tubes.read_area_file(tubes.reference_data_dir() + "/cylindrical-tube.txt").replace(
             num_steps=600),
+        "vowel5_plane_symmetric": tubes.vowel_tube(5, num_steps=600, source=tubes.SOURCE_GAUSSIAN,
+                                                   n_sections=12).replace(shape=tubes.SHAPE_PLANE_SYMMETRIC,
+                                                                          aspect=None),
+        "vowel_u_file_plane_symmetric": tubes.read_area_file(tubes.reference_data_dir() + "/vocaltract_u.txt").replace(
+            shape=tubes.SHAPE_PLANE_SYMMETRIC, num_steps=400),
     }
     return cases
 

@@ -47,6 +47,7 @@ struct Prelude {
     std::vector<double> minor;     // ellipse minor axis [m] (Real-evaluated, stored in double)
     std::vector<double> zsec;      // z_inv per section
     std::vector<double> dscale;
+    std::vector<PlaneSection> plane;   // single-plane-symmetric contours (shape 3)
     PhysTables ph;
     float ds32;
 };
@@ -111,8 +112,8 @@ int prelude(const vs_tube_spec& s, bool fp64, Prelude& p) {
     if (s.num_sections <= 0 || !s.length_cm || !s.area_cm2) return fail(VS_E_INVALID, "empty area function");
     if (s.srate_mul <= 0) return fail(VS_E_INVALID, "srate_mul must be positive");
     if (s.fdtd_solver_type != 0 && s.fdtd_solver_type != 1) return fail(VS_E_INVALID, "fdtd_solver_type must be 0 or 1");
-    if (s.cross_sectional_shape != 1 && s.cross_sectional_shape != 2)
-        return fail(VS_E_INVALID, "cross_sectional_shape must be 1 (circular) or 2 (elliptical)");
+    if (s.cross_sectional_shape < 1 || s.cross_sectional_shape > 3)
+        return fail(VS_E_INVALID, "cross_sectional_shape must be 1 (circular), 2 (elliptical) or 3 (single-plane symmetric)");
     if (s.source_type < 0 || s.source_type > 2) return fail(VS_E_INVALID, "Invalid source type");
     const int n = s.num_sections;
     for (int i = 0; i < n; ++i)
@@ -127,7 +128,10 @@ int prelude(const vs_tube_spec& s, bool fp64, Prelude& p) {
     p.minor.resize(n);
     for (int i = 0; i < n; ++i) {
         const float a = s.area_cm2[i] * kCm * kCm;
-        if (s.cross_sectional_shape == 1) {
+        if (s.cross_sectional_shape == 3) {
+            width[i] = 0.0f;   // maximum chord of the contour, below
+            p.minor[i] = 0.0;
+        } else if (s.cross_sectional_shape == 1) {
             width[i] = 2.0 * sqrt(a / M_PI);
             p.minor[i] = fp64 ? 2.0 * std::sqrt((double)s.area_cm2[i] * 1e-2 * 1e-2 / M_PI) : (double)width[i];
         } else {
@@ -138,6 +142,12 @@ int prelude(const vs_tube_spec& s, bool fp64, Prelude& p) {
             if (fp64) p.minor[i] = std::sqrt((double)s.area_cm2[i] * 1e-2 * 1e-2 / ((double)q * 1.0 * M_PI)) * 2.0;
             else p.minor[i] = lx * (1.0f * 2);
         }
+    }
+    if (s.cross_sectional_shape == 3) {
+        std::string err;
+        const int rc = plane_sections(s.area_cm2, n, p.plane, err);
+        if (rc != VS_OK) return fail(rc, err);
+        for (int i = 0; i < n; ++i) width[i] = p.plane[i].diameter;
     }
     if (s.fdtd_solver_type == 0) {
         // 2D area scaling (TubeGeometry.cu:241-286); pow(x, 2) as x * x
@@ -166,7 +176,7 @@ int prelude(const vs_tube_spec& s, bool fp64, Prelude& p) {
         if (d == 0) d = 1;
         if (d % 2 == 0) {
             const float over = (float)d - width[i] / ds;
-            d = over > 0 ? d - 1 : d + 1;
+            d = (over > 0 || s.cross_sectional_shape >= 3) ? d - 1 : d + 1;
         }
         p.diam[i] = d;
     }
@@ -367,6 +377,10 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
     ALLOC(b.cum, sec); ALLOC(b.diam, sec); ALLOC(b.minor, sec); ALLOC(b.dscale, sec); ALLOC(b.zsec, sec);
     ALLOC(b.source, src); ALLOC(b.probe, src); ALLOC(b.strace, src);
     ALLOC(b.tm, h->n); ALLOC(b.pexc, h->n); ALLOC(b.status, 1);
+    bool any_plane = false;
+    for (int k = 0; k < h->n; ++k) any_plane = any_plane || specs[k].cross_sectional_shape == 3;
+    const long long npoly = any_plane ? sec * kPlanePoints : 1;
+    ALLOC(b.poly_y, npoly); ALLOC(b.poly_z, npoly); ALLOC(b.pbox, any_plane ? sec * 3 : 1);
     ALLOC(h->d_desc, h->n); ALLOC(h->d_order, h->n);
     // host staging of the per-section and per-step inputs
     std::vector<float> cum; std::vector<int> diam; std::vector<Real> minor, dsc, zs, srct;
@@ -407,6 +421,24 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
     VS_HIP(hipMemcpyAsync(b.zsec, zs.data(), zs.size() * sizeof(Real), hipMemcpyHostToDevice, st));
     VS_HIP(hipMemcpyAsync(b.source, srct.data(), srct.size() * sizeof(Real), hipMemcpyHostToDevice, st));
     VS_HIP(hipMemcpyAsync(b.tm, tms.data(), tms.size() * sizeof(TwoMassState), hipMemcpyHostToDevice, st));
+    if (any_plane) {
+        std::vector<float> py((size_t)sec * kPlanePoints, 0.0f), pz((size_t)sec * kPlanePoints, 0.0f),
+            box((size_t)sec * 3, 0.0f);
+        for (int k = 0; k < h->n; ++k) {
+            const long long so = h->desc[k].sec_off;
+            for (size_t i = 0; i < pre[k].plane.size(); ++i) {
+                const PlaneSection& P = pre[k].plane[i];
+                std::copy(P.y.begin(), P.y.end(), py.begin() + (so + i) * kPlanePoints);
+                std::copy(P.z.begin(), P.z.end(), pz.begin() + (so + i) * kPlanePoints);
+                box[3 * (so + i)] = P.minY;
+                box[3 * (so + i) + 1] = P.minZ;
+                box[3 * (so + i) + 2] = P.maxZ;
+            }
+        }
+        VS_HIP(hipMemcpy(b.poly_y, py.data(), py.size() * sizeof(float), hipMemcpyHostToDevice));
+        VS_HIP(hipMemcpy(b.poly_z, pz.data(), pz.size() * sizeof(float), hipMemcpyHostToDevice));
+        VS_HIP(hipMemcpy(b.pbox, box.data(), box.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     VS_HIP(hipMemcpyAsync(h->d_desc, h->desc.data(), h->n * sizeof(TubeDesc), hipMemcpyHostToDevice, st));
     VS_HIP(hipMemcpyAsync(h->d_order, order.data(), h->n * sizeof(int), hipMemcpyHostToDevice, st));
     // setup kernels; depth scratch is one maxHW slice per tube

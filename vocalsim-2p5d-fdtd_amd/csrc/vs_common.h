@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <string>
+#include <vector>
 
 namespace vs {
 
@@ -181,7 +183,23 @@ struct Bufs {
     TwoMassState* tm;    // per tube
     double* pexc;        // per tube: P at the excitation cell after the last step
     unsigned* status;    // device status word
+    float* poly_y;       // per section: aligned contour (kPlanePoints), single-plane shape only
+    float* poly_z;
+    float* pbox;         // per section: minY, minZ, maxZ of the aligned contour
 };
+
+// Single-plane-symmetric cross-section of one tube section (vs_plane.hip):
+// the aligned closed contour (3997 points: 1999 on the upper half and their
+// mirror images), its bounding values and its maximum chord (the section's
+// sagittal width).
+constexpr int kPlanePoints = 3997;
+struct PlaneSection {
+    std::vector<float> y, z;
+    float minY = 0, minZ = 0, maxZ = 0;
+    float diameter = 0;
+    int grow_steps = 0;   // 0.01 mm growth steps taken to reach the area
+};
+int plane_sections(const float* area_cm2, int n, std::vector<PlaneSection>& out, std::string& err);
 
 // Per-cell coefficients of the FP32 flux-form kernel (vs_fast.hip).
 struct FastBufs {

@@ -94,6 +94,30 @@ __global__ void k_classify(const TubeDesc* __restrict__ tubes, Bufs<Real> b) {
     b.bst[T.cell_off + i] = bst;
 }
 
+// Lowest and highest crossing of the vertical line y = y0 with the closed
+// contour (kPlanePoints points); crossing heights are evaluated in double and
+// rounded to float, as a float polyline intersection returns them.
+__device__ inline bool contour_chord(const float* __restrict__ py, const float* __restrict__ pz, float y0,
+                                     float& zlo, float& zhi) {
+    zlo = __builtin_inff();
+    zhi = -__builtin_inff();
+    for (int k = 0; k + 1 < kPlanePoints; ++k) {
+        const float ya = py[k], yb = py[k + 1];
+        if (!((ya <= y0 && y0 <= yb) || (yb <= y0 && y0 <= ya))) continue;
+        const float za = pz[k], zb = pz[k + 1];
+        if (ya == yb) {
+            zlo = fminf(zlo, fminf(za, zb));
+            zhi = fmaxf(zhi, fmaxf(za, zb));
+        } else {
+            const double t = ((double)y0 - (double)ya) / ((double)yb - (double)ya);
+            const float z = (float)((double)za + ((double)zb - (double)za) * t);
+            zlo = fminf(zlo, z);
+            zhi = fmaxf(zhi, z);
+        }
+    }
+    return zlo <= zhi;
+}
+
 // ---------------------------------------------------------------------------
 // Setup 3: raw depth maps (TubeDepth.cu:313-490) in closed form per cell.
 // Rows above the centre take the "upper" branch, rows below the "lower"
@@ -131,6 +155,31 @@ __global__ void k_depth_raw(const TubeDesc* __restrict__ tubes, Bufs<Real> b, Re
             const Real d = static_cast<Real>(k) * ds;
             const Real dy = upper_side ? (static_cast<Real>(k) + 0.5f) * ds : (static_cast<Real>(k) - 0.5f) * ds;
             Real sx, sy;
+            if (T.shape == 3) {
+                // single-plane-symmetric contour: half the vertical chord at the
+                // face positions counted from the lower wall (TubeDepth.cu:535-626)
+                const int cc = lo - r;
+                const long long ps = T.sec_off + sec;
+                const float* py = b.poly_y + ps * kPlanePoints;
+                const float* pz = b.poly_z + ps * kPlanePoints;
+                const float minY = b.pbox[3 * ps];
+                const float dsf = T.ds_f32;
+                const float posx = (float)((cc - 0.5) * (double)dsf);
+                const float posy = cc * dsf;
+                float zlo, zhi;
+                bool okx = contour_chord(py, pz, minY + posx, zlo, zhi);
+                const Real hx = okx ? ((Real)zhi - (Real)zlo) / (Real)2.0f : (Real)0;
+                bool oky = true;
+                Real hy = 0;
+                if (cc < air) {
+                    oky = contour_chord(py, pz, minY + posy, zlo, zhi);
+                    hy = oky ? ((Real)zhi - (Real)zlo) / (Real)2.0f : (Real)0;
+                }
+                if (!okx || !oky) atomicOr(b.status, kStatusSqrtDomain);
+                X[i] = hx * scale;
+                Y[i] = hy * scale;
+                return;
+            }
             if (T.shape == 1) {
                 const Real radius = (static_cast<Real>(air) / 2.0f) * ds;
                 const Real r2 = radius * radius;

@@ -33,6 +33,7 @@ SOURCE_TWO_MASS = 2
 
 SHAPE_CIRCULAR = 1
 SHAPE_ELLIPTICAL = 2
+SHAPE_PLANE_SYMMETRIC = 3
 
 
 @dataclasses.dataclass
