@@ -417,7 +417,7 @@ def test_fp32_oracle_close_to_fp64():
     dict(areas_cm2=np.array([2.0, -1.0]), lengths_cm=np.array([8.0, 8.0])),
     dict(source_type=5),
     dict(mouth_dirichlet=0, pml=0),
-    dict(shape=3),
+    dict(shape=4),
 ])
 def test_invalid_specs_raise(bad):
     spec = tubes.uniform_tube(num_steps=10).replace(**bad)
