@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define VS_ABI_VERSION 1
+#define VS_ABI_VERSION 2
 
 enum vs_status {
     VS_OK = 0,
@@ -57,7 +57,7 @@ enum vs_precision { VS_FP32 = 32, VS_FP64 = 64 };
  * (src/TubeGeometry.cu:83-88). */
 typedef struct vs_tube_spec {
     int fdtd_solver_type;       /* 0: 2D, 1: 2.5D                          */
-    int cross_sectional_shape;  /* 1: circular, 2: elliptical               */
+    int cross_sectional_shape;  /* 1: circular, 2: elliptical, 3: single-plane symmetric */
     int mouth_radiation_cond;   /* 1: Dirichlet mouth, 0: open              */
     int srate_mul;              /* fs = 44100 * srate_mul                   */
     int pml_flag;               /* 0 / 1                                    */
@@ -77,6 +77,9 @@ typedef struct vs_tube_spec {
     double tm_ps;               /* two-mass subglottal pressure, <= 0 -> 686.46 Pa */
     double tm_q;                /* two-mass pitch factor, <= 0 -> 1         */
     int engine_variant;         /* 0: FDTDEngineCPU rounding, 1: FDTDEngineEigen */
+    int open_space;             /* 1: open-space domain (simulation_type 0, FDTDSolver.cu:479-550):
+                                 * 43 x 341 air cells, point source at the centre,
+                                 * listener 5 cells down and right; the area function is ignored */
 } vs_tube_spec;
 
 typedef struct vs_dims {

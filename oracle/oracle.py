@@ -31,6 +31,7 @@ class VsoSpec(ctypes.Structure):
         ("gauss_amp", ctypes.c_double), ("gauss_t0", ctypes.c_double), ("gauss_width", ctypes.c_double),
         ("tm_ps", ctypes.c_double), ("tm_q", ctypes.c_double),
         ("engine_variant", ctypes.c_int),
+        ("open_space", ctypes.c_int),
     ]
 
 
@@ -107,7 +108,7 @@ class OracleTube:
             _fptr(spec.lengths_cm), _fptr(spec.areas_cm2), _fptr(spec.aspect),
             _fptr(spec.depth_scale), _fptr(spec.mu3d), spec.mu3d_default,
             spec.gauss_amp, spec.gauss_t0, spec.gauss_width, spec.tm_ps, spec.tm_q,
-            spec.engine_variant)
+            spec.engine_variant, spec.open_space)
         self.precision = precision
         self.h = lib().vso_create(ctypes.byref(s), precision)
         if not self.h:

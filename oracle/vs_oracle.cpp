@@ -426,10 +426,30 @@ bool build_geometry(const vso_spec& s, float ds, Geometry& g) {
  * walls + gap fill (TubeGeometry.cu:433-569), excitation column (:571-594),
  * Dirichlet column (:596-612), dead ring (FDTDSolver.cu:816-835), PML rings
  * (FDTDSolver.cu:371-464, 838-857). */
+/* openSpaceSimulation, FDTDSolver.cu:479-550: a 43 x 341 domain of air with
+ * the excitation cell at (frameH / 2 - 1, frameW / 2 - 1) in integer
+ * arithmetic and the listener 5 cells further down and right. */
+void build_open_space(const vso_spec& s, Geometry& g) {
+    const int pml = s.pml_flag ? 1 : 0;
+    g.nsec = 0;
+    g.domainH = 43;
+    g.domainW = 341;
+    g.frameH = g.domainH + 2 + 2 * pml * 6;
+    g.frameW = g.domainW + 2 + 2 * pml * 6;
+    g.excY = g.frameH / 2 - 1;
+    g.excX = g.frameW / 2 - 1;
+    g.listenerY = g.excY + 5;
+    g.listenerX = g.excX + 5;
+    g.L = 0; g.ts = 1; g.te = 0; g.midY = g.excY; g.mic_cells = 0;
+}
+
 void build_cells(const vso_spec& s, const Geometry& g, std::vector<int>& type, std::vector<int>& bst) {
     const int H = g.frameH, W = g.frameW;
     type.assign((size_t)H * W, cell_air);
     bst.assign((size_t)H * W, 0);
+    if (s.open_space) {
+        type[(size_t)g.excY * W + g.excX] = cell_excitation;
+    }
     for (int i = 0; i < g.L; ++i) {
         int x = g.ts + i;
         type[(size_t)g.upper[i] * W + x] = cell_wall;
@@ -443,6 +463,8 @@ void build_cells(const vso_spec& s, const Geometry& g, std::vector<int>& type, s
         return m;
     };
     /* connectSagittalWallsGap: serial, as in the reference */
+    if (s.open_space) goto frame;
+    {
     int pu = g.upper[0], pl = g.lower[0], px = g.ts;
     for (int i = 1; i <= g.L - 2; ++i) {
         int cu = g.upper[i], cl = g.lower[i], cx = g.ts + i;
@@ -468,6 +490,8 @@ void build_cells(const vso_spec& s, const Geometry& g, std::vector<int>& type, s
         int mx = g.te + 1;
         for (int y = g.upper[g.L - 1]; y <= g.lower[g.L - 1]; ++y) type[(size_t)y * W + mx] = cell_noPressure;
     }
+    }
+frame:
     /* setupDeadCells (the reference's idx <= H*W bound touches one element
      * past the end; only in-frame cells are written here) */
     for (int r = 0; r < H; ++r)
@@ -712,7 +736,12 @@ bool Tube<Real>::build(const vso_spec& s) {
     fill_tables(ph);
     num_steps = s.num_steps > 0 ? s.num_steps : float_num_steps(s.audio_dur, pf.dt);
     if (num_steps <= 0) { g_err = "number of steps must be positive"; return false; }
-    if (!build_geometry(s, pf.ds, g)) return false;
+    if (s.open_space) {
+        build_open_space(s, g);
+        dirichlet = 0;
+    } else if (!build_geometry(s, pf.ds, g)) {
+        return false;
+    }
     build_cells(s, g, type, bst);
     /* per-section minor axis and z in Real */
     for (int i = 0; i < g.nsec; ++i) {
@@ -740,7 +769,12 @@ bool Tube<Real>::build(const vso_spec& s) {
         if (bst[i] != 0) zinv[i] = zsec[bst[i] - 1];
     /* depth: FDTDSolver.cu:632-641 */
     dX.assign(type.size(), Real(1.0)); dY = dX; dP = dX;
-    if (s.fdtd_solver_type == 1 && !build_depth(s)) return false;
+    if (s.open_space) {
+        /* constant open-space depth, FDTDSolver.cu:504-514 */
+        if (s.fdtd_solver_type == 1) { dX.assign(type.size(), (Real)(1.0 * (Real)0.50f)); dY = dX; dP = dX; }
+    } else if (s.fdtd_solver_type == 1 && !build_depth(s)) {
+        return false;
+    }
     build_coefficients();
     build_source(s);
     tm_q = s.tm_q > 0 ? (Real)s.tm_q : (Real)1.0f;

@@ -18,7 +18,7 @@ extern "C" {
  * reference's SimulationParams flags plus the inputs the configs add. */
 typedef struct vso_spec {
     int fdtd_solver_type;       /* 0: 2D, 1: 2.5D */
-    int cross_sectional_shape;  /* 1: circular, 2: elliptical */
+    int cross_sectional_shape;  /* 1: circular, 2: elliptical, 3: single-plane symmetric */
     int mouth_radiation_cond;   /* 1: Dirichlet mouth, 0: open */
     int srate_mul;              /* fs = 44100 * srate_mul */
     int pml_flag;               /* 0/1 */
@@ -38,6 +38,7 @@ typedef struct vso_spec {
     double tm_ps;               /* two-mass subglottal pressure [Pa]; <= 0 -> 686.46 */
     double tm_q;                /* two-mass pitch factor; <= 0 -> 1 */
     int engine_variant;         /* 0: FDTDEngineCPU op order, 1: FDTDEngineEigen op order */
+    int open_space;             /* 1: open-space domain (FDTDSolver.cu:479-550); sections ignored */
 } vso_spec;
 
 enum vso_dim {

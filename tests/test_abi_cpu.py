@@ -27,7 +27,7 @@ def test_library_loads_and_exports_all_declared_symbols():
 
 
 def test_abi_version():
-    assert vocalsim.load().vs_abi_version() == 1
+    assert vocalsim.load().vs_abi_version() == 2
 
 
 def test_null_arguments_rejected_without_device():

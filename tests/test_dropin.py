@@ -39,9 +39,16 @@ def test_regular_tube_flag_selects_no_file(tmp_path):
     assert r.returncode == 1 and "Unable to open file" in r.stderr
 
 
-def test_open_space_rejected(tmp_path):
-    r, _ = run_demo([1, 0, 1, 1, 1, 1, 15, 0.02, 1, 1, 0, 64, 100], tmp_path)
-    assert r.returncode == 1 and "open-space" in r.stderr
+@pytest.mark.gpu
+def test_open_space_simulation(hip_device, tmp_path):
+    # simulation_type 0 (FDTDSolver.cu:479-550): no area file is read
+    r, out = run_demo([1, 0, -1, 1, -1, 1, 15, 0.02, 1, 1, 0, 64, 400], tmp_path,
+                      env_extra={"VOCALSIM_DATA_DIR": "/nonexistent/"})
+    assert r.returncode == 0, r.stderr
+    assert "frame 57 x 355, listener (32, 181)" in r.stdout
+    got = np.loadtxt(out)
+    ref = orc.OracleTube(tubes.open_space(num_steps=400), 64).run()[0]
+    np.testing.assert_allclose(got, ref, rtol=2e-6, atol=1e-6 * np.abs(ref).max())
 
 
 def _oracle_probe(vowel_file, shape, source, steps, variant=0, prec=64):

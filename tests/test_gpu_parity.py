@@ -44,6 +44,9 @@ def _cases():
         "vowel5_plane_symmetric": tubes.vowel_tube(5, num_steps=600, source=tubes.SOURCE_GAUSSIAN,
                                                    n_sections=12).replace(shape=tubes.SHAPE_PLANE_SYMMETRIC,
                                                                           aspect=None),
+        "open_space_pml": tubes.open_space(num_steps=500),
+        "open_space_2d_nopml_twomass": tubes.open_space(num_steps=500, pml=0, solver_type=0,
+                                                        source_type=tubes.SOURCE_TWO_MASS),
         "vowel_u_file_plane_symmetric": tubes.read_area_file(tubes.reference_data_dir() + "/vocaltract_u.txt").replace(
             shape=tubes.SHAPE_PLANE_SYMMETRIC, num_steps=400),
     }

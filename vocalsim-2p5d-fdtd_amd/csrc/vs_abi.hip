@@ -108,9 +108,37 @@ int float_step_count(float dur, float dt) {
     return n;
 }
 
+// Open-space domain (FDTDSolver.cu:479-550): 43 x 341 air cells inside the
+// dead ring and PML, the excitation cell at the frame centre (integer halves,
+// as `ceil(frameH / 2) - 1` evaluates on ints) and the listener 5 cells below
+// and to the right of it.  No tube, no walls, constant depth.
+int prelude_open_space(const vs_tube_spec& s, bool fp64, Prelude& p) {
+    phys_tables(s.srate_mul, fp64, p.ph, p.ds32);
+    p.num_steps = s.num_steps > 0 ? s.num_steps : float_step_count(s.audio_dur, 1.0f / (float)(44100 * s.srate_mul));
+    if (p.num_steps <= 0) return fail(VS_E_INVALID, "number of steps must be positive");
+    const int pml = s.pml_flag ? 1 : 0;
+    p.domainH = 43;
+    p.domainW = 341;
+    p.H = p.domainH + 2 + 12 * pml;
+    p.W = p.domainW + 2 + 12 * pml;
+    p.excY = p.H / 2 - 1;
+    p.excX = p.W / 2 - 1;
+    p.listenerY = p.excY + 5;
+    p.listenerX = p.excX + 5;
+    p.L = 0;
+    p.ts = 1;
+    p.te = 0;
+    p.midY = p.excY;
+    p.mic = 0;
+    return VS_OK;
+}
+
 int prelude(const vs_tube_spec& s, bool fp64, Prelude& p) {
-    if (s.num_sections <= 0 || !s.length_cm || !s.area_cm2) return fail(VS_E_INVALID, "empty area function");
     if (s.srate_mul <= 0) return fail(VS_E_INVALID, "srate_mul must be positive");
+    if (s.fdtd_solver_type != 0 && s.fdtd_solver_type != 1) return fail(VS_E_INVALID, "fdtd_solver_type must be 0 or 1");
+    if (s.source_type < 0 || s.source_type > 2) return fail(VS_E_INVALID, "Invalid source type");
+    if (s.open_space) return prelude_open_space(s, fp64, p);
+    if (s.num_sections <= 0 || !s.length_cm || !s.area_cm2) return fail(VS_E_INVALID, "empty area function");
     if (s.fdtd_solver_type != 0 && s.fdtd_solver_type != 1) return fail(VS_E_INVALID, "fdtd_solver_type must be 0 or 1");
     if (s.cross_sectional_shape < 1 || s.cross_sectional_shape > 3)
         return fail(VS_E_INVALID, "cross_sectional_shape must be 1 (circular), 2 (elliptical) or 3 (single-plane symmetric)");
@@ -342,19 +370,21 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
         const vs_tube_spec& s = specs[k];
         TubeDesc& d = h->desc[k];
         std::memset(&d, 0, sizeof(d));
-        d.H = p.H; d.W = p.W; d.L = p.L; d.nsec = s.num_sections;
+        const int nsec = s.open_space ? 0 : s.num_sections;
+        d.H = p.H; d.W = p.W; d.L = p.L; d.nsec = nsec;
+        d.open_space = s.open_space ? 1 : 0;
         d.ts = p.ts; d.te = p.te; d.midY = p.midY;
         d.listener = p.listenerY * p.W + p.listenerX;
         d.excitation = p.excY * p.W + p.excX;
         d.num_steps = p.num_steps;
-        d.source_type = s.source_type; d.dirichlet = s.mouth_radiation_cond == 1;
+        d.source_type = s.source_type; d.dirichlet = !s.open_space && s.mouth_radiation_cond == 1;
         d.variant = s.engine_variant; d.srate_mul = s.srate_mul;
         d.shape = s.cross_sectional_shape; d.solver_type = s.fdtd_solver_type;
         d.ds_f32 = p.ds32; d.pml = s.pml_flag ? 1 : 0;
         d.cell_off = cell; d.col_off = col; d.wall_off = wall; d.sec_off = sec; d.src_off = src;
         d.ph = p.ph;
         cell += ((long long)p.H * p.W + 63) / 64 * 64;   // 256-byte aligned tube blocks
-        col += p.W; wall += p.L; sec += s.num_sections; src += p.num_steps;
+        col += p.W; wall += p.L; sec += nsec; src += p.num_steps;
         h->max_steps = std::max(h->max_steps, p.num_steps);
         h->cells_per_step += (long long)(p.H - 2) * (p.W - 2);
         h->max_interior = std::max(h->max_interior, (p.H - 2) * (p.W - 2));
@@ -363,7 +393,7 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
         dm.listenerX = p.listenerX; dm.listenerY = p.listenerY;
         dm.excitationX = p.excX; dm.excitationY = p.excY; dm.num_steps = p.num_steps;
         dm.tube_start_x = p.ts; dm.tube_end_x = p.te; dm.tube_len_cells = p.L;
-        dm.midY = p.midY; dm.num_sections = s.num_sections; dm.mic_cells = p.mic;
+        dm.midY = p.midY; dm.num_sections = nsec; dm.mic_cells = p.mic;
     }
     h->total_cells = cell; h->total_cols = col; h->total_walls = wall; h->total_secs = sec; h->total_src = src;
     Bufs<Real>& b = bufs_of<Real>(h);
@@ -391,7 +421,7 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
         diam.insert(diam.end(), p.diam.begin(), p.diam.end());
         const float mu_def = specs[k].mu3d_default > 0 ? specs[k].mu3d_default
                                                        : (specs[k].fdtd_solver_type == 1 ? 0.025f : 0.005f);
-        for (int i = 0; i < specs[k].num_sections; ++i) {
+        for (int i = 0; i < h->desc[k].nsec; ++i) {
             minor.push_back((Real)p.minor[i]);
             dsc.push_back((Real)p.dscale[i]);
             zs.push_back((Real)p.zsec[i]);

@@ -44,6 +44,7 @@ struct TubeDesc {
     int shape, solver_type;
     float ds_f32;
     int pml;                      // PML rings painted (pml_flag)
+    int open_space;               // open-space domain: no tube, point source
     long long cell_off;           // start of the tube's H*W block
     long long col_off;            // start of its W-long column arrays
     long long wall_off;           // start of its L-long tube-column arrays

@@ -59,7 +59,10 @@ __global__ void k_classify(const TubeDesc* __restrict__ tubes, Bufs<Real> b) {
     const int* sg = b.seg + T.wall_off;
     const int L = T.L;
     int type = kAir, bst = 0;
-    if (c >= T.ts && c <= T.te) {
+    if (T.open_space) {
+        // FDTDSolver.cu:519-545: air everywhere, one excitation cell
+        if (i == T.excitation) type = kExcitation;
+    } else if (c >= T.ts && c <= T.te) {
         const int j = c - T.ts;
         const int uj = up[j], lj = lo[j];
         if (r == uj || r == lj) { type = kWall; bst = sg[j] + 1; }
@@ -250,6 +253,10 @@ __global__ void k_depth_final(const TubeDesc* __restrict__ tubes, Bufs<Real> b, 
     if (p < mn) p = mn;
     if (x < mn) x = mn;
     if (y < mn) y = mn;
+    if (T.open_space) {   // constant open-space depth, no smoothing (FDTDSolver.cu:504-514)
+        const Real od = (Real)(1.0 * (Real)0.50f);
+        p = od; x = od; y = od;
+    }
     if (T.solver_type == 0) { p = 1.0; x = 1.0; y = 1.0; }   // 2D: unit depth (FDTDSolver.cu:632-635)
     b.dP[T.cell_off + i] = p;
     b.dX[T.cell_off + i] = x;
@@ -457,7 +464,7 @@ hipError_t launch_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n,
     }
     const long long stride = maxHW;   // raw / smoothed depth scratch: one slice per tube
     dim3 gw((maxL + 63) / 64, n), gc((maxHW + kSetupThreads - 1) / kSetupThreads, n);
-    hipLaunchKernelGGL(k_walls<Real>, gw, dim3(64), 0, st, d_tubes, b);
+    if (maxL > 0) hipLaunchKernelGGL(k_walls<Real>, gw, dim3(64), 0, st, d_tubes, b);
     hipLaunchKernelGGL(k_classify<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, b);
     hipLaunchKernelGGL(k_depth_raw<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, b, tmpA, tmpB, stride);
     hipLaunchKernelGGL(k_depth_smooth<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, tmpA, tmpB, tmpC, tmpD, stride);

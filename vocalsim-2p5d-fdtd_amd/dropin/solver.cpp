@@ -70,7 +70,6 @@ void FDTDSolver::load_area_function() {
 // FDTDSolver.cu:289-476: constants, tables, step count, then the geometry,
 // depth, PML and coefficient setup (here: the GPU setup kernels of vs_create).
 void FDTDSolver::setupWaveSolver() {
-    if (simulation_type == 0) throw std::runtime_error("open-space simulation is not part of this build");
     SimulationParams sp(srate_mul);
     sound_speed = sp.sound_speed; rho = sp.rho; ds = sp.ds; dt = sp.dt;
     wall_pressure_coupling_coeff = sp.wall_pressure_coupling_coeff;
@@ -90,7 +89,8 @@ void FDTDSolver::setupWaveSolver() {
     solver.setTimeStep(audio_dur, sp.dt);
     time_step = solver.getTimeStep();
     num_steps = explicit_steps_ > 0 ? explicit_steps_ : (int)time_step.size();
-    load_area_function();
+    // simulation_type 0: open-space domain (FDTDSolver.cu:479-550), no area function
+    if (simulation_type != 0) load_area_function();
 
     vs_tube_spec s{};
     s.fdtd_solver_type = fdtd_solver_type;
@@ -111,6 +111,7 @@ void FDTDSolver::setupWaveSolver() {
     s.gauss_amp = g_amp_; s.gauss_t0 = g_t0_; s.gauss_width = g_w_;
     s.tm_ps = tm_ps_; s.tm_q = tm_q_;
     s.engine_variant = 0;
+    s.open_space = simulation_type == 0 ? 1 : 0;
     if (h_) { vs_destroy(h_); h_ = nullptr; }
     fptr_cache_.clear();
     check(vs_create(&s, 1, precision_, &h_), "vs_create");

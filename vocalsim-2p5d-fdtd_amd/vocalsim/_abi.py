@@ -53,6 +53,7 @@ class TubeSpecC(ctypes.Structure):
         ("gauss_amp", ctypes.c_double), ("gauss_t0", ctypes.c_double), ("gauss_width", ctypes.c_double),
         ("tm_ps", ctypes.c_double), ("tm_q", ctypes.c_double),
         ("engine_variant", ctypes.c_int),
+        ("open_space", ctypes.c_int),
     ]
 
 
@@ -133,5 +134,5 @@ def spec_to_c(spec):
                   _fptr(spec.lengths_cm), _fptr(spec.areas_cm2), _fptr(spec.aspect),
                   _fptr(spec.depth_scale), _fptr(spec.mu3d), spec.mu3d_default,
                   spec.gauss_amp, spec.gauss_t0, spec.gauss_width, spec.tm_ps, spec.tm_q,
-                  spec.engine_variant)
+                  spec.engine_variant, spec.open_space)
     return c, keep

@@ -58,6 +58,7 @@ class TubeSpec:
     tm_ps: float = 0.0            # 0: 686.46 Pa
     tm_q: float = 0.0             # 0: 1.0
     engine_variant: int = 0       # 0: FDTDEngineCPU, 1: FDTDEngineEigen
+    open_space: int = 0           # 1: open-space domain (simulation_type 0), sections ignored
 
     def __post_init__(self):
         self.lengths_cm = np.ascontiguousarray(self.lengths_cm, dtype=np.float32)
@@ -77,6 +78,16 @@ class TubeSpec:
         d = dataclasses.asdict(self)
         d.update(kw)
         return TubeSpec(**d)
+
+
+def open_space(num_steps: int = 1000, pml: int = 1, solver_type: int = 1,
+               source_type: int = SOURCE_GAUSSIAN, **kw) -> TubeSpec:
+    """Open-space domain of the reference's simulation_type 0
+    (FDTDSolver.cu:479-550): 43 x 341 cells of air, a point source at the
+    centre and the listener 5 cells down and right."""
+    return TubeSpec(lengths_cm=np.array([1.0]), areas_cm2=np.array([1.0]), open_space=1,
+                    num_steps=num_steps, pml=pml, solver_type=solver_type, source_type=source_type,
+                    **kw)
 
 
 def read_area_file(path: str) -> TubeSpec:
