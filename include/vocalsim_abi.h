@@ -202,6 +202,23 @@ int vs_probe_dtft(vs_handle* h, int tube, int kind, int hann, const double* freq
  * sum (frameH-2)(frameW-2)  (FDTDEngineCPU.cpp:316-317). */
 int vs_cells_per_step(const vs_handle* h, long long* cells);
 
+/* Audio export.  Replaces results/generate_audio.py:9-42: the probe of a
+ * tube (tube >= 0) or of every tube concatenated in tube order (tube = -1),
+ * over the steps advanced so far, normalised by its max |p|, clipped to
+ * [-1, 1] and, when fs = 44100 * srate_mul differs from 44100, low-passed by
+ * the second-order Butterworth at 5050 Hz of scipy.signal.butter(2, ...)
+ * applied as scipy.signal.lfilter does, then decimated by srate_mul
+ * (samples 0, srate_mul, 2 srate_mul, ...).  Computed on the GPU from the
+ * device-resident probe; float32 samples at 44.1 kHz.  *n_out receives the
+ * sample count; out may be NULL to query it. */
+int vs_probe_audio(vs_handle* h, int tube, float* out, size_t cap, size_t* n_out);
+/* Coefficients b[3], a[3] (a[0] = 1) of that low-pass for a cutoff and a
+ * sample rate (host only, no device needed). */
+int vs_butter2_lowpass(double cutoff_hz, double fs, double* b3, double* a3);
+/* Writes mono float32 samples as a WAV file (IEEE float, 18-byte fmt chunk
+ * and a fact chunk, the layout scipy.io.wavfile.write produces; host only). */
+int vs_write_wav(const char* path, const float* samples, size_t n, int rate);
+
 /* Reverse-time (adjoint) gradients.  No reference counterpart: the reference
  * only runs the forward time loop (FDTDEngineCPU.cpp:316-410); these entry
  * points differentiate that loop, written in flux form U = V * depth, which

@@ -1414,3 +1414,90 @@ int vs_adjoint_backward(vs_handle* h, const double* gprobe, size_t gprobe_len, d
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Audio export (results/generate_audio.py:9-42)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int vs_butter2_lowpass(double cutoff_hz, double fs, double* b3, double* a3) {
+    if (!b3 || !a3) return fail(VS_E_INVALID, "null coefficient buffer");
+    if (!(cutoff_hz > 0.0) || !(cutoff_hz < 0.5 * fs)) return fail(VS_E_INVALID, "cutoff must lie in (0, fs / 2)");
+    const AudioFilter f = butter2_lowpass(cutoff_hz, fs);
+    for (int k = 0; k < 3; ++k) { b3[k] = f.b[k]; a3[k] = f.a[k]; }
+    return VS_OK;
+}
+
+int vs_probe_audio(vs_handle* h, int tube, float* out, size_t cap, size_t* n_out) {
+    if (int rc = check_handle(h)) return rc;
+    if (tube < -1 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+    const int t0 = tube < 0 ? 0 : tube, t1 = tube < 0 ? h->n : tube + 1;
+    std::vector<AudioJob> jobs;
+    long long total = 0;
+    for (int k = t0; k < t1; ++k) {
+        const TubeDesc& d = h->desc[k];
+        AudioJob J{};
+        J.in_off = d.src_off;
+        J.n = std::min(h->steps_done, d.num_steps);
+        J.decim = d.srate_mul;   // fs = 44100 * srate_mul
+        J.flt = butter2_lowpass(5050.0, 44100.0 * d.srate_mul);
+        J.out_off = total;
+        total += J.decim == 1 ? J.n : (J.n + J.decim - 1) / J.decim;
+        jobs.push_back(J);
+    }
+    if (n_out) *n_out = (size_t)total;
+    if (!out) return VS_OK;
+    if (cap < (size_t)total) return fail(VS_E_RANGE, "audio buffer needs " + std::to_string(total) + " samples");
+    if (total == 0) return VS_OK;
+    AudioJob* dj = nullptr;
+    float* dout = nullptr;
+    VS_HIP(hipMalloc(&dj, jobs.size() * sizeof(AudioJob)));
+    hipError_t e = hipMalloc(&dout, (size_t)total * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpyAsync(dj, jobs.data(), jobs.size() * sizeof(AudioJob), hipMemcpyHostToDevice, h->stream);
+    const double* probe = h->precision == 64 ? h->bd.probe : h->bf.probe;
+    if (e == hipSuccess) e = launch_audio(probe, dj, (int)jobs.size(), dout, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)total * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(dj);
+    if (dout) (void)hipFree(dout);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("audio: ") + hipGetErrorString(e));
+    return VS_OK;
+}
+
+int vs_write_wav(const char* path, const float* samples, size_t n, int rate) {
+    if (!path || (!samples && n)) return fail(VS_E_INVALID, "vs_write_wav: bad argument");
+    if (rate <= 0) return fail(VS_E_INVALID, "sample rate must be positive");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(VS_E_INVALID, std::string("cannot open ") + path);
+    auto u32 = [&](uint32_t v) { unsigned char b[4] = {(unsigned char)v, (unsigned char)(v >> 8), (unsigned char)(v >> 16), (unsigned char)(v >> 24)}; std::fwrite(b, 1, 4, f); };
+    auto u16 = [&](uint16_t v) { unsigned char b[2] = {(unsigned char)v, (unsigned char)(v >> 8)}; std::fwrite(b, 1, 2, f); };
+    const uint32_t data_bytes = (uint32_t)(n * 4);
+    // RIFF / WAVE with an 18-byte fmt chunk (IEEE float, cbSize 0) and a fact
+    // chunk, the layout scipy.io.wavfile.write uses for float32 data
+    std::fwrite("RIFF", 1, 4, f);
+    u32(4 + (8 + 18) + (8 + 4) + (8 + data_bytes));
+    std::fwrite("WAVE", 1, 4, f);
+    std::fwrite("fmt ", 1, 4, f);
+    u32(18);
+    u16(3);                    // WAVE_FORMAT_IEEE_FLOAT
+    u16(1);                    // channels
+    u32((uint32_t)rate);
+    u32((uint32_t)rate * 4);   // bytes per second
+    u16(4);                    // block align
+    u16(32);                   // bits per sample
+    u16(0);                    // cbSize
+    std::fwrite("fact", 1, 4, f);
+    u32(4);
+    u32((uint32_t)n);
+    std::fwrite("data", 1, 4, f);
+    u32(data_bytes);
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t v;
+        std::memcpy(&v, &samples[i], 4);
+        u32(v);
+    }
+    const bool ok = std::fclose(f) == 0;
+    return ok ? VS_OK : fail(VS_E_INVALID, std::string("write failed: ") + path);
+}
+
+}  // extern "C"

@@ -229,6 +229,21 @@ struct AdjBufs {
     int S;
 };
 
+// Second-order IIR section, a[0] = 1 (vs_audio.hip).
+struct AudioFilter {
+    double b[3];
+    double a[3];
+};
+// One probe trace to turn into audio: input offset/length in the probe
+// array, output offset, decimation factor (srate_mul) and its filter.
+struct AudioJob {
+    long long in_off, out_off;
+    int n, decim;
+    AudioFilter flt;
+};
+AudioFilter butter2_lowpass(double fc, double fs);
+hipError_t launch_audio(const double* d_probe, const AudioJob* d_jobs, int njobs, float* d_out, hipStream_t st);
+
 // Kernel launchers (vs_kernels.hip, vs_fast.hip, vs_adjoint.hip).
 hipError_t launch_dtft(const double* d_x, long long stride, int nsig, int n, double fs, int hann,
                        const double* d_f, int nf, double* d_re, double* d_im, hipStream_t st);

@@ -153,6 +153,20 @@ class Solver:
         check(load().vs_get_probe(self.h, tube, out.ctypes.data_as(_abi._D), n))
         return out[: min(n, self.steps_done)]
 
+    def audio(self, tube: int = 0) -> np.ndarray:
+        """44.1 kHz float32 audio of the probe (vs_probe_audio): normalised,
+        clipped, Butterworth-filtered at 5050 Hz and decimated on the GPU, as
+        results/generate_audio.py does.  tube = -1: all tubes concatenated."""
+        n = ctypes.c_size_t()
+        check(load().vs_probe_audio(self.h, int(tube), None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=np.float32)
+        check(load().vs_probe_audio(self.h, int(tube), out.ctypes.data_as(_abi._F), out.size, ctypes.byref(n)))
+        return out
+
+    def write_wav(self, path: str, tube: int = 0):
+        """Writes :meth:`audio` as a 44.1 kHz float32 WAV file."""
+        write_wav(path, self.audio(tube))
+
     def source_trace(self, tube: int = 0) -> np.ndarray:
         n = self.dims[tube]["num_steps"]
         out = np.zeros(n, dtype=np.float64)
@@ -186,3 +200,18 @@ class Solver:
         out = np.zeros(shape, dtype=dt)
         check(load().vs_get_field(self.h, tube, fid, out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
         return out
+
+
+def butter2_lowpass(cutoff_hz: float, fs: float):
+    """(b, a) of the second-order Butterworth low-pass (vs_butter2_lowpass)."""
+    b = np.zeros(3)
+    a = np.zeros(3)
+    check(load().vs_butter2_lowpass(float(cutoff_hz), float(fs), b.ctypes.data_as(_abi._D),
+                                    a.ctypes.data_as(_abi._D)))
+    return b, a
+
+
+def write_wav(path: str, samples: np.ndarray, rate: int = 44100):
+    """Mono float32 WAV (vs_write_wav)."""
+    x = np.ascontiguousarray(samples, dtype=np.float32)
+    check(load().vs_write_wav(str(path).encode(), x.ctypes.data_as(_abi._F), x.size, int(rate)))
