@@ -300,6 +300,8 @@ struct vs_handle {
     bool fast_ready = false;   // coefficients built
     bool flux_valid = false;   // Uxf/Uyf hold the current state
     int max_interior = 0;      // largest (H-2)*(W-2) of the batch
+    int max_padded = 0;        // largest (H-2)*(W-1): interior with one pad column per row
+    int max_wp = 0;            // widest padded row W-1
     int lds_limit = 0;
     // materialised export buffers (coefficients, types, z_inv) per tube
     std::vector<void*> coeff_dev, types_dev, zinv_dev;
@@ -388,6 +390,8 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
         h->max_steps = std::max(h->max_steps, p.num_steps);
         h->cells_per_step += (long long)(p.H - 2) * (p.W - 2);
         h->max_interior = std::max(h->max_interior, (p.H - 2) * (p.W - 2));
+        h->max_padded = std::max(h->max_padded, (p.H - 2) * (p.W - 1));
+        h->max_wp = std::max(h->max_wp, p.W - 1);
         vs_dims& dm = h->dims[k];
         dm.frameH = p.H; dm.frameW = p.W; dm.domainH = p.domainH; dm.domainW = p.domainW;
         dm.listenerX = p.listenerX; dm.listenerY = p.listenerY;
@@ -498,17 +502,17 @@ int check_handle(const vs_handle* h) {
 int prepare_fast(vs_handle* h) {
     if (h->precision != VS_FP32) return fail(VS_E_INVALID, "the flux-form kernel is FP32 only");
     if (h->threads < 128) return fail(VS_E_INVALID, "the on-chip kernel needs at least 128 threads");
-    const int K = (h->max_interior + h->threads - 64 - 1) / (h->threads - 64);
+    const int K = fast_cells_per_lane(h->max_padded, h->threads);
     if (K > 24) return fail(VS_E_INVALID, "tube too large for the on-chip kernel at this work-group size");
-    if ((long long)(fast_lds_bytes(K, h->threads) + fast_lds_static_bytes()) > h->lds_limit)
-        return fail(VS_E_INVALID, "on-chip kernel needs " + std::to_string(fast_lds_bytes(K, h->threads)) +
+    const int G = fast_guard_lanes(K, h->max_wp);
+    if ((long long)(fast_lds_bytes(K, h->threads, G) + fast_lds_static_bytes()) > h->lds_limit)
+        return fail(VS_E_INVALID, "on-chip kernel needs " + std::to_string(fast_lds_bytes(K, h->threads, G)) +
                     " bytes of LDS, device allows " + std::to_string(h->lds_limit));
     if (!h->fast_ready) {
         FastBufs& f = h->fb;
         const long long n = h->total_cells;
         int rc;
         if ((rc = h->alloc(&f.kP, n)) || (rc = h->alloc(&f.gX, n)) || (rc = h->alloc(&f.gY, n)) ||
-            (rc = h->alloc(&f.wx, n)) || (rc = h->alloc(&f.wy, n)) || (rc = h->alloc(&f.ex, n)) ||
             (rc = h->alloc(&f.Uxf, n)) || (rc = h->alloc(&f.Uyf, n)) || (rc = h->alloc(&f.code, n)))
             return rc;
         const PhysTables& ph = h->desc[0].ph;
@@ -521,7 +525,7 @@ int prepare_fast(vs_handle* h) {
         h->fast_ready = true;
     }
     {
-        const int K = (h->max_interior + h->threads - 64 - 1) / (h->threads - 64);
+        const int K = fast_cells_per_lane(h->max_padded, h->threads);
         hipError_t e = configure_fast(K);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("configure: ") + hipGetErrorString(e));
     }
@@ -542,8 +546,9 @@ hipError_t launch_any<double>(vs_handle* h, Bufs<double>& b, int t0, int ns, con
 template <>
 hipError_t launch_any<float>(vs_handle* h, Bufs<float>& b, int t0, int ns, const int* d_t0) {
     if (h->kernel == VS_K_FAST) {
-        const int K = (h->max_interior + h->threads - 64 - 1) / (h->threads - 64);
-        return launch_fast_step(K, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, d_t0, h->stream);
+        const int K = fast_cells_per_lane(h->max_padded, h->threads);
+        const int G = fast_guard_lanes(K, h->max_wp);
+        return launch_fast_step(K, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, G, d_t0, h->stream);
     }
     h->flux_valid = false;
     return launch_step<float>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, d_t0, h->stream);
@@ -1069,8 +1074,9 @@ int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) 
     std::vector<int> threads;
     if (h->kernel == VS_K_FAST) {
         for (int t = 1024; t >= 128; t -= 64) {
-            const int K = (h->max_interior + t - 64 - 1) / (t - 64);
-            if (K <= 24 && (long long)(fast_lds_bytes(K, t) + fast_lds_static_bytes()) <= h->lds_limit)
+            const int K = fast_cells_per_lane(h->max_padded, t);
+            if (K <= 24 && (long long)(fast_lds_bytes(K, t, fast_guard_lanes(K, h->max_wp)) +
+                                       fast_lds_static_bytes()) <= h->lds_limit)
                 threads.push_back(t);
         }
     } else {

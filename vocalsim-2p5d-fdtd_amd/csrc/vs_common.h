@@ -205,9 +205,8 @@ int plane_sections(const float* area_cm2, int n, std::vector<PlaneSection>& out,
 // Per-cell coefficients of the FP32 flux-form kernel (vs_fast.hip).
 struct FastBufs {
     float* kP; float* gX; float* gY;     // pressure and face coefficients
-    float* wx; float* wy; float* ex;     // special faces: wall admittance, source
     float* Uxf; float* Uyf;              // flux state U = V * depth
-    uint16_t* code;                      // damping codes + special bit
+    uint16_t* code;                      // damping codes + face-form bits
     float lut[8];                        // damping factor by code
 };
 
@@ -247,12 +246,18 @@ hipError_t launch_audio(const double* d_probe, const AudioJob* d_jobs, int njobs
 // Kernel launchers (vs_kernels.hip, vs_fast.hip, vs_adjoint.hip).
 hipError_t launch_dtft(const double* d_x, long long stride, int nsig, int n, double fs, int hann,
                        const double* d_f, int nf, double* d_re, double* d_im, hipStream_t st);
-size_t fast_lds_bytes(int K, int threads);
+// On-chip FP32 kernel sizing: cells per lane K for the batch's largest padded
+// interior Hi * (Wi + 1), zero guard lanes per LDS row for its widest padded
+// row Wi + 1, and the dynamic LDS bytes of a launch.
+int fast_cells_per_lane(int max_padded_cells, int threads);
+int fast_guard_lanes(int K, int max_padded_width);
+size_t fast_lds_bytes(int K, int threads, int guard);
 size_t fast_lds_static_bytes();
 hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<float> b,
                              FastBufs f, hipStream_t st);
 hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
-                            FastBufs f, int t0, int nsteps, int threads, const int* d_t0, hipStream_t st);
+                            FastBufs f, int t0, int nsteps, int threads, int guard, const int* d_t0,
+                            hipStream_t st);
 hipError_t launch_bump(int* d_t0, int s, hipStream_t st);
 hipError_t configure_fast(int K);
 template <typename Real>

@@ -32,7 +32,12 @@ namespace vs {
 
 namespace {
 
-constexpr uint32_t kCodeSpecial = 1u << 9;
+// Face-form bits of a cell's code (above the three 3-bit damping codes):
+// kCodeSelfOff drops the cell's own pressure from its face updates (walls
+// carrying an admittance: U = -g * P_neighbour); kCodeSrcX makes the x-face
+// carry the source (excitation column: Ux = g * exeV).
+constexpr uint32_t kCodeSelfOff = 1u << 9;
+constexpr uint32_t kCodeSrcX = 1u << 10;
 
 // ---------------------------------------------------------------------------
 // Setup of the per-cell coefficients of the flux form (evaluated in double
@@ -76,15 +81,15 @@ __global__ void k_fast_setup(const TubeDesc* __restrict__ tubes, Bufs<float> b, 
     const int mcx = max(sigma_code(tc), sigma_code(tr)), mcy = max(sigma_code(tc), sigma_code(tt));
     const int codeX = (mbx == 0.0 || mcx == 8) ? 1 : mcx;
     const int codeY = (mby == 0.0 || mcy == 8) ? 1 : mcy;
-    f.gX[q] = mbx == 0.0 ? 0.0f : (float)(dX * (mbx * mbx * C) / (mbx + Mx));
-    f.gY[q] = mby == 0.0 ? 0.0f : (float)(dY * (mby * mby * C) / (mby + My));
+    float gx = mbx == 0.0 ? 0.0f : (float)(dX * (mbx * mbx * C) / (mbx + Mx));
+    float gy = mby == 0.0 ? 0.0f : (float)(dY * (mby * mby * C) / (mby + My));
     uint32_t code = (uint32_t)codeP | ((uint32_t)codeX << 3) | ((uint32_t)codeY << 6);
-    // special faces
-    float wx = 0.0f, wy = 0.0f, e = 0.0f;
+    // special faces (their damping codes are "zero": beta = 0 on both kinds)
     if (tc == kExcitation) {
         // (exeV * eWx * Mx) / (0 + Mx): the x-face carries the source velocity
-        e = (float)dX;
-        code |= kCodeSpecial;
+        gx = (float)dX;
+        gy = 0.0f;
+        code |= kCodeSelfOff | kCodeSrcX;
     } else if (segwall) {
         const int ibr = (int)br, ibt = (int)bt;
         const int lut = (ibr == kAir && ibt == kAir) ? 0 : (ibr == kAir && ibt == kWall) ? 1
@@ -94,14 +99,13 @@ __global__ void k_fast_setup(const TubeDesc* __restrict__ tubes, Bufs<float> b, 
         const int ec = 0, er = tr == kExcitation, et = tt == kExcitation;
         const double nex = (double)((1 - ec) * (1 - er)), ney = (double)((1 - ec) * (1 - et));
         // (Mx * z * nE * (-xor1 * P_r * Nin)) / (0 + Mx), xor1 = beta_r (beta_c = 0 on walls)
-        wx = (float)(dX * z * nex * br * nin);
-        wy = (float)(dY * z * ney * bt * nin);
-        code |= kCodeSpecial;
+        gx = (float)(dX * z * nex * br * nin);
+        gy = (float)(dY * z * ney * bt * nin);
+        code |= kCodeSelfOff;
     }
+    f.gX[q] = gx;
+    f.gY[q] = gy;
     f.code[q] = (uint16_t)code;
-    f.wx[q] = wx;
-    f.wy[q] = wy;
-    f.ex[q] = e;
 }
 
 // Flux <-> velocity conversions of the state (start / end of the fast path).
@@ -119,38 +123,59 @@ __device__ inline uint32_t code_of(const uint32_t* cw, int k) {
     return (k & 1) ? (cw[k >> 1] >> 16) : (cw[k >> 1] & 0xffffu);
 }
 
+// LDS footprint (floats) of the cell arrays: P and Uy rows of T + 2G lanes per
+// slot, the run-end x-fluxes with one zero guard on each side, the LUT.
+__host__ __device__ inline int fast_lds_floats(int K, int T, int G) {
+    const int n = 2 * K * (T + 2 * G) + (T + 2) + 8;
+    return (n + 1) & ~1;   // the two doubles after it stay 8-byte aligned
+}
+
 template <int K>
 __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__ tubes,
                                                     const int* __restrict__ order, Bufs<float> b,
                                                     FastBufs f, int t0_arg, int nsteps,
-                                                    const int* __restrict__ d_t0) {
+                                                    const int* __restrict__ d_t0, int G) {
+#pragma clang fp contract(fast)
     extern __shared__ float lds[];
     // threads [0, T) own cells; the last wave is the control wave (source,
     // two-mass recurrence, probe).  Keeping the double-precision recurrence
     // out of the cell waves keeps their register budget for the cell arrays.
     const int T = blockDim.x - 64, t = threadIdx.x;
-    float* Pb = lds;
-    float* Uyb = lds + K * T;
-    float* UxE = lds + 2 * K * T;
-    float* lut = UxE + T;
-    double* sexe = reinterpret_cast<double*>(lut + 8);   // [2], 8-byte aligned (T even)
+    const int TP = T + 2 * G;                    // lanes per slot row, G zero guards each side
+    float* Pb = lds + G;                         // Pb[k * TP + lane], lane in [-G, T + G)
+    float* Uyb = lds + K * TP + G;
+    float* UxE = lds + 2 * K * TP + 1;           // UxE[-1] and UxE[T] are zero guards
+    float* lut = lds + 2 * K * TP + T + 2;
+    double* sexe = reinterpret_cast<double*>(lds + fast_lds_floats(K, T, G));   // [2]
     __shared__ TwoMassState tm;
 
     const int tube = order[blockIdx.x];
     const TubeDesc& D = tubes[tube];
-    const int H = D.H, W = D.W, Hi = H - 2, Wi = W - 2, N = Hi * Wi;
+    const int H = D.H, W = D.W, Hi = H - 2, Wi = W - 2;
+    // cells are indexed over the interior with one zero pad column per row:
+    // j = r * Wp + c, c == Wi is the pad; every neighbour of a real cell is a
+    // real cell, a pad cell or a zero guard lane, so the update needs no
+    // boundary tests (pad and tail cells carry zero coefficients)
+    const int Wp = Wi + 1;
     // graph replays read the first step from device memory
     const int t0 = d_t0 ? *d_t0 : t0_arg;
     const int tend = min(t0 + nsteps, D.num_steps);
     const long long co = D.cell_off;
 
+    // zero guard lanes (LDS starts undefined)
+    for (int i = t; i < 2 * K * 2 * G; i += blockDim.x) {
+        const int row = i / (2 * G), g = i - row * (2 * G);
+        lds[row * TP + (g < G ? g : T + g)] = 0.f;
+    }
+    if (t == 0) { UxE[-1] = 0.f; UxE[T] = 0.f; }
+
     if (t >= T) {
         // ------------------------- control wave -------------------------
         const int lane = t - T;
         const bool twomass = D.source_type == 2;
-        const int jL = (D.listener / W - 1) * Wi + (D.listener % W - 1);
-        const int jE = (D.excitation / W - 1) * Wi + (D.excitation % W - 1);
-        const int aL = (jL % K) * T + jL / K, aE = (jE % K) * T + jE / K;
+        const int jL = (D.listener / W - 1) * Wp + (D.listener % W - 1);
+        const int jE = (D.excitation / W - 1) * Wp + (D.excitation % W - 1);
+        const int aL = (jL % K) * TP + jL / K, aE = (jE % K) * TP + jE / K;
         double pexc = 0.0;
         if (lane < 8) lut[lane] = f.lut[lane];
         if (lane == 0) {
@@ -190,96 +215,73 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
     float Ux[K], kP[K], gX[K], gY[K];
     uint32_t cw[(K + 1) / 2];
     const int jb = t * K;
-    const int r0 = jb / Wi, c0 = jb - r0 * Wi;
     {
-        int r = r0, c = c0;
+        int r = jb / Wp, c = jb - (jb / Wp) * Wp;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const int j = jb + k;
             float p = 0.f, uy = 0.f, ux = 0.f, kp = 0.f, gx = 0.f, gy = 0.f;
             uint32_t cd = 1u | (1u << 3) | (1u << 6);
-            if (j < N) {
+            if (r < Hi && c < Wi) {
                 const long long q = co + (long long)(r + 1) * W + (c + 1);
                 p = b.P[q]; uy = f.Uyf[q]; ux = f.Uxf[q];
                 kp = f.kP[q]; gx = f.gX[q]; gy = f.gY[q]; cd = f.code[q];
             }
-            Pb[k * T + t] = p;
-            Uyb[k * T + t] = uy;
+            Pb[k * TP + t] = p;
+            Uyb[k * TP + t] = uy;
             Ux[k] = ux; kP[k] = kp; gX[k] = gx; gY[k] = gy;
             if (k & 1) cw[k >> 1] |= cd << 16; else cw[k >> 1] = cd;
-            if (++c == Wi) { c = 0; ++r; }
+            if (++c == Wp) { c = 0; ++r; }
         }
     }
     UxE[t] = Ux[K - 1];
     __syncthreads();
 
     for (int ts = t0; ts < tend; ++ts) {
-        // Launder the lane coordinates once per step so the compiler
-        // recomputes per-cell addresses and masks (a few integer ops) instead
-        // of keeping K copies of them live across the time loop.
-        int tl = t, r0l = r0, c0l = c0;
-        asm volatile("" : "+v"(tl), "+v"(r0l), "+v"(c0l));
+        // Launder the lane index once per phase so the compiler recomputes
+        // per-cell addresses (a few integer ops) instead of keeping K copies
+        // live across the time loop.
+        int tl = t;
+        asm volatile("" : "+v"(tl));
 #pragma unroll
         for (int k = 0; k < (K + 1) / 2; ++k) asm volatile("" : "+v"(cw[k]));
         // ---------------- pressure phase ----------------
         {
-            int r = r0l, c = c0l;
-            const float uxl0 = (tl > 0) ? UxE[tl - 1] : 0.f;
+            const float uxl0 = UxE[tl - 1];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const int j = jb + k;
-                if (jb + k < N) {
-                    const float p = Pb[k * T + tl];
-                    const float uy = Uyb[k * T + tl];
-                    float uyb = 0.f;
-                    if (r < Hi - 1) {
-                        const int d = k + Wi;
-                        uyb = Uyb[(d % K) * T + tl + d / K];
-                    }
-                    const float left = (c > 0) ? (k > 0 ? Ux[k - 1] : uxl0) : 0.f;
-                    const float dv = (Ux[k] - left) + (uy - uyb);
-                    const float aP = lut[code_of<K>(cw, k) & 7u];
-                    Pb[k * T + tl] = aP * p - kP[k] * dv;
-                }
-                if (++c == Wi) { c = 0; ++r; }
+                const float p = Pb[k * TP + tl];
+                const float uy = Uyb[k * TP + tl];
+                const int d = k + Wp;
+                const float uyb = Uyb[(d % K) * TP + tl + d / K];
+                const float left = k > 0 ? Ux[k - 1] : uxl0;
+                const float dv = (Ux[k] - left) + (uy - uyb);
+                const float aP = lut[code_of<K>(cw, k) & 7u];
+                Pb[k * TP + tl] = aP * p - kP[k] * dv;
             }
         }
         __syncthreads();
         const float ev = (float)sexe[ts & 1];
         // ---------------- velocity phase ----------------
         {
-            asm volatile("" : "+v"(tl), "+v"(r0l), "+v"(c0l));
+            asm volatile("" : "+v"(tl));
 #pragma unroll
             for (int k = 0; k < (K + 1) / 2; ++k) asm volatile("" : "+v"(cw[k]));
-            int r = r0l, c = c0l;
             float pn = Pb[tl];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const float p = pn;
-                if (k < K - 1) pn = Pb[(k + 1) * T + tl];
-                else pn = (tl + 1 < T) ? Pb[tl + 1] : 0.f;
-                if (tl * K + k < N) {
-                    const float pr = (c < Wi - 1) ? pn : 0.f;
-                    float pa = 0.f;
-                    if (r > 0) {
-                        const int d = k - Wi;
-                        const int tt = d >= 0 ? d / K : -((-d + K - 1) / K);   // floor(d / K)
-                        const int kk = d - tt * K;
-                        pa = Pb[kk * T + tl + tt];
-                    }
-                    const uint32_t cd = code_of<K>(cw, k);
-                    const float ax = lut[(cd >> 3) & 7u], ay = lut[(cd >> 6) & 7u];
-                    float ux = ax * Ux[k] + gX[k] * (p - pr);
-                    float uy = ay * Uyb[k * T + tl] + gY[k] * (p - pa);
-                    if (cd & kCodeSpecial) {
-                        const long long q = co + (long long)(r + 1) * W + (c + 1);
-                        ux = f.ex[q] * ev - f.wx[q] * pr;
-                        uy = -f.wy[q] * pa;
-                    }
-                    Ux[k] = ux;
-                    Uyb[k * T + tl] = uy;
-                }
-                if (++c == Wi) { c = 0; ++r; }
+                pn = (k < K - 1) ? Pb[(k + 1) * TP + tl] : Pb[tl + 1];
+                const int d = k - Wp;
+                const int tt = d >= 0 ? d / K : -((-d + K - 1) / K);   // floor(d / K)
+                const int kk = d - tt * K;
+                const float pa = Pb[kk * TP + tl + tt];
+                const uint32_t cd = code_of<K>(cw, k);
+                const float ax = lut[(cd >> 3) & 7u], ay = lut[(cd >> 6) & 7u];
+                // walls with admittance: U = -g P_neighbour; excitation: Ux = g exeV
+                const float ps = (cd & kCodeSelfOff) ? 0.f : p;
+                const float prx = (cd & kCodeSrcX) ? -ev : pn;
+                Ux[k] = ax * Ux[k] + gX[k] * (ps - prx);
+                Uyb[k * TP + tl] = ay * Uyb[k * TP + tl] + gY[k] * (ps - pa);
             }
             UxE[tl] = Ux[K - 1];
         }
@@ -289,22 +291,21 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
     {
         // fresh lane coordinates: the addresses of the initial loads must not
         // stay live across the time loop
-        int r0w = r0, c0w = c0;
-        asm volatile("" : "+v"(r0w), "+v"(c0w));
-        int r = r0w, c = c0w;
+        int jw = jb;
+        asm volatile("" : "+v"(jw));
+        int r = jw / Wp, c = jw - (jw / Wp) * Wp;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const int j = jb + k;
-            if (j < N) {
+            if (r < Hi && c < Wi) {
                 const long long q = co + (long long)(r + 1) * W + (c + 1);
-                const float uy = Uyb[k * T + t];
-                b.P[q] = Pb[k * T + t];
+                const float uy = Uyb[k * TP + t];
+                b.P[q] = Pb[k * TP + t];
                 f.Uxf[q] = Ux[k];
                 f.Uyf[q] = uy;
                 b.Vx[q] = Ux[k] / b.dX[q];
                 b.Vy[q] = uy / b.dY[q];
             }
-            if (++c == Wi) { c = 0; ++r; }
+            if (++c == Wp) { c = 0; ++r; }
         }
     }
 }
@@ -324,20 +325,28 @@ hipError_t configure_k() {
 
 template <int K>
 hipError_t launch_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
-                    int t0, int nsteps, int threads, const int* d_t0, hipStream_t st) {
-    const size_t lds = fast_lds_bytes(K, threads);
+                    int t0, int nsteps, int threads, int guard, const int* d_t0, hipStream_t st) {
+    const size_t lds = fast_lds_bytes(K, threads, guard);
     hipError_t e = configure_k<K>();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_fast_step<K>, dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0, nsteps, d_t0);
+    hipLaunchKernelGGL(k_fast_step<K>, dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0, nsteps, d_t0,
+                       guard);
     return hipGetLastError();
 }
 
 }  // namespace
 
-size_t fast_lds_bytes(int K, int threads) {
+size_t fast_lds_bytes(int K, int threads, int guard) {
     const int T = threads - 64;   // cell threads
-    return (size_t)(2 * K * T + T + 8) * sizeof(float) + 2 * sizeof(double);
+    return (size_t)fast_lds_floats(K, T, guard) * sizeof(float) + 2 * sizeof(double);
 }
+
+int fast_cells_per_lane(int max_padded_cells, int threads) {
+    const int T = threads - 64;
+    return (max_padded_cells + T - 1) / T;
+}
+
+int fast_guard_lanes(int K, int max_padded_width) { return (max_padded_width + K - 1) / K + 1; }
 
 size_t fast_lds_static_bytes() { return sizeof(TwoMassState) + 64; }
 
@@ -363,9 +372,10 @@ hipError_t configure_fast(int K) {
 }
 
 hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
-                            FastBufs f, int t0, int nsteps, int threads, const int* d_t0, hipStream_t st) {
+                            FastBufs f, int t0, int nsteps, int threads, int guard, const int* d_t0,
+                            hipStream_t st) {
     switch (K) {
-#define VS_K(k) case k: return launch_k<k>(d_tubes, d_order, n, b, f, t0, nsteps, threads, d_t0, st);
+#define VS_K(k) case k: return launch_k<k>(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard, d_t0, st);
         VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
         VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(19) VS_K(20)
         VS_K(21) VS_K(22) VS_K(23) VS_K(24)
