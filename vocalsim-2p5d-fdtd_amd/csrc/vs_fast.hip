@@ -1,16 +1,15 @@
 // vs_fast.hip -- FP32 throughput path: one work-group per tube with the
 // tube's state held on chip for a whole block of steps.
 //
-// Layout inside a work-group of T threads.  The tube interior (Hi x Wi cells,
-// row-major, j = r*Wi + c) is cut into T runs of K consecutive cells; thread t
-// owns cells j = t*K + k, k < K.
-//   LDS   P[k][t], Uy[k][t]   pressure and y-flux of every cell, stored
-//                             slot-major so that every lane-uniform access
-//                             (own cell, cell above, cell below, next cell)
-//                             is a contiguous, bank-conflict-free row;
-//         UxEnd[t]            x-flux of each run's last cell (left neighbour
-//                             of the next run);
-//         lut[8]              damping factors 1/(1 + sigma) by code.
+// Layout inside a work-group of T cell threads (plus one control wave).
+// The tube interior is indexed with one zero pad column per row,
+// j = r * Wp + c (Wp = Wi + 1), and cut into runs of K consecutive cells;
+// thread t owns cells j = t*K + k, k < K.
+//   LDS   one record per lane, stride S = 2K + 1 floats (odd: lanes hit
+//         distinct banks):  [ P[0..K) | Uy[0..K) | Ux of the run's last cell ]
+//         with G zero guard records on each side, so every neighbour read of
+//         a real cell lands on a real cell, a pad cell or a zero guard;
+//         static tables: (ax, ay) by damping-code pair, aP by damping code.
 //   VGPRs Ux[K], kP[K], gX[K], gY[K] and the packed per-cell codes.
 //
 // Update in flux form (U = V * depth), algebraically equal to the reference
@@ -32,12 +31,17 @@ namespace vs {
 
 namespace {
 
-// Face-form bits of a cell's code (above the three 3-bit damping codes):
-// kCodeSelfOff drops the cell's own pressure from its face updates (walls
-// carrying an admittance: U = -g * P_neighbour); kCodeSrcX makes the x-face
-// carry the source (excitation column: Ux = g * exeV).
-constexpr uint32_t kCodeSelfOff = 1u << 9;
-constexpr uint32_t kCodeSrcX = 1u << 10;
+// Per-cell code (16 bits): bits [3, 9) hold code_x + 8 code_y, which times 8
+// is the byte offset of (ax, ay) in the pair table; bits [9, 12) the pressure
+// damping code; kCodeSelfOff drops the cell's own pressure from its face
+// updates (walls carrying an admittance: U = -g * P_neighbour); kCodeSrcX
+// makes the x-face carry the source (excitation column: Ux = g * exeV).
+constexpr int kShiftXY = 3;
+constexpr int kShiftP = 9;
+constexpr uint32_t kMaskXY = 0x1f8u;
+constexpr uint32_t kCodeSelfOff = 1u << 12;
+constexpr uint32_t kCodeSrcX = 1u << 13;
+constexpr uint32_t kCodeZero = ((1u + 8u * 1u) << kShiftXY) | (1u << kShiftP);   // pad cells
 
 // ---------------------------------------------------------------------------
 // Setup of the per-cell coefficients of the flux form (evaluated in double
@@ -83,7 +87,7 @@ __global__ void k_fast_setup(const TubeDesc* __restrict__ tubes, Bufs<float> b, 
     const int codeY = (mby == 0.0 || mcy == 8) ? 1 : mcy;
     float gx = mbx == 0.0 ? 0.0f : (float)(dX * (mbx * mbx * C) / (mbx + Mx));
     float gy = mby == 0.0 ? 0.0f : (float)(dY * (mby * mby * C) / (mby + My));
-    uint32_t code = (uint32_t)codeP | ((uint32_t)codeX << 3) | ((uint32_t)codeY << 6);
+    uint32_t code = ((uint32_t)(codeX + 8 * codeY) << kShiftXY) | ((uint32_t)codeP << kShiftP);
     // special faces (their damping codes are "zero": beta = 0 on both kinds)
     if (tc == kExcitation) {
         // (exeV * eWx * Mx) / (0 + Mx): the x-face carries the source velocity
@@ -123,51 +127,40 @@ __device__ inline uint32_t code_of(const uint32_t* cw, int k) {
     return (k & 1) ? (cw[k >> 1] >> 16) : (cw[k >> 1] & 0xffffu);
 }
 
-// LDS footprint (floats) of the cell arrays: P and Uy rows of T + 2G lanes per
-// slot, the run-end x-fluxes with one zero guard on each side, the LUT.
-__host__ __device__ inline int fast_lds_floats(int K, int T, int G) {
-    const int n = 2 * K * (T + 2 * G) + (T + 2) + 8;
-    return (n + 1) & ~1;   // the two doubles after it stay 8-byte aligned
-}
-
 template <int K>
 __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__ tubes,
                                                     const int* __restrict__ order, Bufs<float> b,
                                                     FastBufs f, int t0_arg, int nsteps,
                                                     const int* __restrict__ d_t0, int G) {
 #pragma clang fp contract(fast)
+    constexpr int S = 2 * K + 1;                 // record stride (floats)
     extern __shared__ float lds[];
+    __shared__ float2 vtab[64];                  // (ax, ay) by code_x + 8 code_y
+    __shared__ float plut[8];                    // aP by code_p
+    __shared__ double sexe[2];                   // source value of steps t, t + 1
+    __shared__ TwoMassState tm;
     // threads [0, T) own cells; the last wave is the control wave (source,
     // two-mass recurrence, probe).  Keeping the double-precision recurrence
     // out of the cell waves keeps their register budget for the cell arrays.
     const int T = blockDim.x - 64, t = threadIdx.x;
-    const int TP = T + 2 * G;                    // lanes per slot row, G zero guards each side
-    float* Pb = lds + G;                         // Pb[k * TP + lane], lane in [-G, T + G)
-    float* Uyb = lds + K * TP + G;
-    float* UxE = lds + 2 * K * TP + 1;           // UxE[-1] and UxE[T] are zero guards
-    float* lut = lds + 2 * K * TP + T + 2;
-    double* sexe = reinterpret_cast<double*>(lds + fast_lds_floats(K, T, G));   // [2]
-    __shared__ TwoMassState tm;
+    float* const rec0 = lds + G * S;             // record of lane 0; lane l at rec0 + l * S
 
     const int tube = order[blockIdx.x];
     const TubeDesc& D = tubes[tube];
     const int H = D.H, W = D.W, Hi = H - 2, Wi = W - 2;
-    // cells are indexed over the interior with one zero pad column per row:
-    // j = r * Wp + c, c == Wi is the pad; every neighbour of a real cell is a
-    // real cell, a pad cell or a zero guard lane, so the update needs no
-    // boundary tests (pad and tail cells carry zero coefficients)
     const int Wp = Wi + 1;
+    // j + Wp = (l + q) K + (k + m): slot k + m of lane l + q, or slot
+    // k + m - K of lane l + q + 1 when k + m >= K (Wp = q K + m); j - Wp
+    // symmetrically.  In record floats both are a uniform base plus k.
+    const int qW = Wp / K, mW = Wp - (Wp / K) * K;
+    const int nbOff = qW * S + mW;
     // graph replays read the first step from device memory
     const int t0 = d_t0 ? *d_t0 : t0_arg;
     const int tend = min(t0 + nsteps, D.num_steps);
     const long long co = D.cell_off;
 
-    // zero guard lanes (LDS starts undefined)
-    for (int i = t; i < 2 * K * 2 * G; i += blockDim.x) {
-        const int row = i / (2 * G), g = i - row * (2 * G);
-        lds[row * TP + (g < G ? g : T + g)] = 0.f;
-    }
-    if (t == 0) { UxE[-1] = 0.f; UxE[T] = 0.f; }
+    // zero guard records (LDS starts undefined)
+    for (int i = t; i < 2 * G * S; i += blockDim.x) lds[i < G * S ? i : i + T * S] = 0.f;
 
     if (t >= T) {
         // ------------------------- control wave -------------------------
@@ -175,9 +168,11 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         const bool twomass = D.source_type == 2;
         const int jL = (D.listener / W - 1) * Wp + (D.listener % W - 1);
         const int jE = (D.excitation / W - 1) * Wp + (D.excitation % W - 1);
-        const int aL = (jL % K) * TP + jL / K, aE = (jE % K) * TP + jE / K;
+        const float* pL = rec0 + (jL / K) * S + jL % K;
+        const float* pE = rec0 + (jE / K) * S + jE % K;
         double pexc = 0.0;
-        if (lane < 8) lut[lane] = f.lut[lane];
+        if (lane < 8) plut[lane] = f.lut[lane];
+        vtab[lane] = make_float2(f.lut[lane & 7], f.lut[lane >> 3]);
         if (lane == 0) {
             pexc = b.pexc[tube];
             if (twomass) tm = b.tm[tube];
@@ -192,8 +187,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         for (int ts = t0; ts < tend; ++ts) {
             __syncthreads();   // pressure phase done
             if (lane == 0) {
-                b.probe[D.src_off + ts] = (double)Pb[aL];
-                pexc = (double)Pb[aE];
+                b.probe[D.src_off + ts] = (double)*pL;
+                pexc = (double)*pE;
                 b.strace[D.src_off + ts] = sexe[ts & 1];
                 if (ts + 1 < tend) {
                     double e;
@@ -216,47 +211,51 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
     uint32_t cw[(K + 1) / 2];
     const int jb = t * K;
     {
+        float* const rec = rec0 + t * S;
         int r = jb / Wp, c = jb - (jb / Wp) * Wp;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             float p = 0.f, uy = 0.f, ux = 0.f, kp = 0.f, gx = 0.f, gy = 0.f;
-            uint32_t cd = 1u | (1u << 3) | (1u << 6);
+            uint32_t cd = kCodeZero;
             if (r < Hi && c < Wi) {
                 const long long q = co + (long long)(r + 1) * W + (c + 1);
                 p = b.P[q]; uy = f.Uyf[q]; ux = f.Uxf[q];
                 kp = f.kP[q]; gx = f.gX[q]; gy = f.gY[q]; cd = f.code[q];
             }
-            Pb[k * TP + t] = p;
-            Uyb[k * TP + t] = uy;
+            rec[k] = p;
+            rec[K + k] = uy;
             Ux[k] = ux; kP[k] = kp; gX[k] = gx; gY[k] = gy;
             if (k & 1) cw[k >> 1] |= cd << 16; else cw[k >> 1] = cd;
             if (++c == Wp) { c = 0; ++r; }
         }
+        rec[2 * K] = Ux[K - 1];
     }
-    UxE[t] = Ux[K - 1];
     __syncthreads();
 
+    const char* const vt = reinterpret_cast<const char*>(vtab);
+    const char* const pt = reinterpret_cast<const char*>(plut);
     for (int ts = t0; ts < tend; ++ts) {
         // Launder the lane index once per phase so the compiler recomputes
-        // per-cell addresses (a few integer ops) instead of keeping K copies
-        // live across the time loop.
+        // the record address instead of keeping per-cell copies live across
+        // the time loop.
         int tl = t;
         asm volatile("" : "+v"(tl));
 #pragma unroll
         for (int k = 0; k < (K + 1) / 2; ++k) asm volatile("" : "+v"(cw[k]));
         // ---------------- pressure phase ----------------
         {
-            const float uxl0 = UxE[tl - 1];
+            float* const rec = rec0 + tl * S;
+            const float* const nb = rec + nbOff + K;      // Uy of the cell below: slot k + m of lane l + q
+            const float uxl0 = rec[2 * K - S];            // run end of lane l - 1
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const float p = Pb[k * TP + tl];
-                const float uy = Uyb[k * TP + tl];
-                const int d = k + Wp;
-                const float uyb = Uyb[(d % K) * TP + tl + d / K];
+                const float p = rec[k];
+                const float uy = rec[K + k];
+                const float uyb = (k + mW >= K) ? nb[k + K + 1] : nb[k];
                 const float left = k > 0 ? Ux[k - 1] : uxl0;
                 const float dv = (Ux[k] - left) + (uy - uyb);
-                const float aP = lut[code_of<K>(cw, k) & 7u];
-                Pb[k * TP + tl] = aP * p - kP[k] * dv;
+                const float aP = *reinterpret_cast<const float*>(pt + ((code_of<K>(cw, k) >> (kShiftP - 2)) & 0x1cu));
+                rec[k] = aP * p - kP[k] * dv;
             }
         }
         __syncthreads();
@@ -266,24 +265,23 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
             asm volatile("" : "+v"(tl));
 #pragma unroll
             for (int k = 0; k < (K + 1) / 2; ++k) asm volatile("" : "+v"(cw[k]));
-            float pn = Pb[tl];
+            float* const rec = rec0 + tl * S;
+            const float* const na = rec - nbOff;          // cell above, slot k - m of lane l - q
+            float pn = rec[0];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const float p = pn;
-                pn = (k < K - 1) ? Pb[(k + 1) * TP + tl] : Pb[tl + 1];
-                const int d = k - Wp;
-                const int tt = d >= 0 ? d / K : -((-d + K - 1) / K);   // floor(d / K)
-                const int kk = d - tt * K;
-                const float pa = Pb[kk * TP + tl + tt];
+                pn = (k < K - 1) ? rec[k + 1] : rec[S];   // next lane's first cell
+                const float pa = (k < mW) ? na[k - K - 1] : na[k];
                 const uint32_t cd = code_of<K>(cw, k);
-                const float ax = lut[(cd >> 3) & 7u], ay = lut[(cd >> 6) & 7u];
+                const float2 axy = *reinterpret_cast<const float2*>(vt + (cd & kMaskXY));
                 // walls with admittance: U = -g P_neighbour; excitation: Ux = g exeV
                 const float ps = (cd & kCodeSelfOff) ? 0.f : p;
                 const float prx = (cd & kCodeSrcX) ? -ev : pn;
-                Ux[k] = ax * Ux[k] + gX[k] * (ps - prx);
-                Uyb[k * TP + tl] = ay * Uyb[k * TP + tl] + gY[k] * (ps - pa);
+                Ux[k] = axy.x * Ux[k] + gX[k] * (ps - prx);
+                rec[K + k] = axy.y * rec[K + k] + gY[k] * (ps - pa);
             }
-            UxE[tl] = Ux[K - 1];
+            rec[2 * K] = Ux[K - 1];
         }
         __syncthreads();
     }
@@ -293,13 +291,14 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         // stay live across the time loop
         int jw = jb;
         asm volatile("" : "+v"(jw));
+        const float* const rec = rec0 + (jw / K) * S;
         int r = jw / Wp, c = jw - (jw / Wp) * Wp;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (r < Hi && c < Wi) {
                 const long long q = co + (long long)(r + 1) * W + (c + 1);
-                const float uy = Uyb[k * TP + t];
-                b.P[q] = Pb[k * TP + t];
+                const float uy = rec[K + k];
+                b.P[q] = rec[k];
                 f.Uxf[q] = Ux[k];
                 f.Uyf[q] = uy;
                 b.Vx[q] = Ux[k] / b.dX[q];
@@ -338,7 +337,7 @@ hipError_t launch_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<flo
 
 size_t fast_lds_bytes(int K, int threads, int guard) {
     const int T = threads - 64;   // cell threads
-    return (size_t)fast_lds_floats(K, T, guard) * sizeof(float) + 2 * sizeof(double);
+    return (size_t)(T + 2 * guard) * (2 * K + 1) * sizeof(float);
 }
 
 int fast_cells_per_lane(int max_padded_cells, int threads) {
@@ -348,7 +347,7 @@ int fast_cells_per_lane(int max_padded_cells, int threads) {
 
 int fast_guard_lanes(int K, int max_padded_width) { return (max_padded_width + K - 1) / K + 1; }
 
-size_t fast_lds_static_bytes() { return sizeof(TwoMassState) + 64; }
+size_t fast_lds_static_bytes() { return sizeof(TwoMassState) + 64 * sizeof(float2) + 8 * sizeof(float) + 2 * sizeof(double) + 64; }
 
 hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<float> b,
                              FastBufs f, hipStream_t st) {
