@@ -299,6 +299,7 @@ struct vs_handle {
     FastBufs fb{};
     bool fast_ready = false;   // coefficients built
     bool flux_valid = false;   // Uxf/Uyf hold the current state
+    bool fast_classes = false; // class bytes and tables built
     int max_interior = 0;      // largest (H-2)*(W-2) of the batch
     int max_padded = 0;        // largest (H-2)*(W-1): interior with one pad column per row
     int max_wp = 0;            // widest padded row W-1
@@ -498,6 +499,43 @@ int check_handle(const vs_handle* h) {
     return VS_OK;
 }
 
+// Maps the 16-bit face codes of the flux-form setup to class bytes: the
+// distinct damping triples (code_p, code_x, code_y) of the batch become
+// classes 1..63 (class 0 = every damping factor zero, the pad cells), with the
+// two face-form bits on top; fills the per-class aP, ax, ay table.
+int build_fast_classes(vs_handle* h) {
+    FastBufs& f = h->fb;
+    const size_t n = (size_t)h->total_cells;
+    std::vector<uint16_t> code(n);
+    VS_HIP(hipStreamSynchronize(h->stream));
+    VS_HIP(hipMemcpy(code.data(), f.code, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    // damping part: bits [3, 12) = code_x + 8 code_y + 64 code_p
+    std::vector<int> cls_of(512, -1);
+    const int zero = 1 + 8 * 1 + 64 * 1;
+    cls_of[zero] = 0;
+    int ncls = 1;
+    std::vector<uint8_t> c8(n);
+    for (size_t i = 0; i < n; ++i) {
+        const int d = (code[i] >> 3) & 0x1ff;
+        if (cls_of[d] < 0) {
+            if (ncls == 64) return fail(VS_E_INVALID, "more than 63 damping classes in one batch");
+            cls_of[d] = ncls++;
+        }
+        c8[i] = (uint8_t)(cls_of[d] | ((code[i] & (1u << 12)) ? 0x40 : 0) | ((code[i] & (1u << 13)) ? 0x80 : 0));
+    }
+    std::vector<float> tab(3 * 64, 0.0f);
+    for (int d = 0; d < 512; ++d) {
+        const int c = cls_of[d];
+        if (c < 0) continue;
+        tab[c] = f.lut[(d >> 6) & 7];
+        tab[64 + c] = f.lut[d & 7];
+        tab[128 + c] = f.lut[(d >> 3) & 7];
+    }
+    VS_HIP(hipMemcpy(f.cls8, c8.data(), n, hipMemcpyHostToDevice));
+    VS_HIP(hipMemcpy(f.ctab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    return VS_OK;
+}
+
 // Lazily builds the per-cell coefficients of the FP32 flux-form kernel.
 int prepare_fast(vs_handle* h) {
     if (h->precision != VS_FP32) return fail(VS_E_INVALID, "the flux-form kernel is FP32 only");
@@ -513,7 +551,8 @@ int prepare_fast(vs_handle* h) {
         const long long n = h->total_cells;
         int rc;
         if ((rc = h->alloc(&f.kP, n)) || (rc = h->alloc(&f.gX, n)) || (rc = h->alloc(&f.gY, n)) ||
-            (rc = h->alloc(&f.Uxf, n)) || (rc = h->alloc(&f.Uyf, n)) || (rc = h->alloc(&f.code, n)))
+            (rc = h->alloc(&f.Uxf, n)) || (rc = h->alloc(&f.Uyf, n)) || (rc = h->alloc(&f.code, n)) ||
+            (rc = h->alloc(&f.cls8, n)) || (rc = h->alloc(&f.ctab, 3 * 64)))
             return rc;
         const PhysTables& ph = h->desc[0].ph;
         f.lut[0] = 1.0f;
@@ -532,6 +571,11 @@ int prepare_fast(vs_handle* h) {
     if (!h->flux_valid) {
         hipError_t e = launch_fast_setup(h->d_desc, h->desc.data(), h->n, h->bf, h->fb, h->stream);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("flux setup: ") + hipGetErrorString(e));
+        if (!h->fast_classes) {
+            const int rc = build_fast_classes(h);
+            if (rc != VS_OK) return rc;
+            h->fast_classes = true;
+        }
         h->flux_valid = true;
     }
     return VS_OK;

@@ -207,6 +207,8 @@ struct FastBufs {
     float* kP; float* gX; float* gY;     // pressure and face coefficients
     float* Uxf; float* Uyf;              // flux state U = V * depth
     uint16_t* code;                      // damping codes + face-form bits
+    uint8_t* cls8;                       // class byte: damping class (6 bits) + face-form bits
+    float* ctab;                         // per damping class: aP[64], ax[64], ay[64]
     float lut[8];                        // damping factor by code
 };
 

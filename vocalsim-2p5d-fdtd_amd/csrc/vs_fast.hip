@@ -41,7 +41,12 @@ constexpr int kShiftP = 9;
 constexpr uint32_t kMaskXY = 0x1f8u;
 constexpr uint32_t kCodeSelfOff = 1u << 12;
 constexpr uint32_t kCodeSrcX = 1u << 13;
-constexpr uint32_t kCodeZero = ((1u + 8u * 1u) << kShiftXY) | (1u << kShiftP);   // pad cells
+// The step kernel reads a class byte per cell instead: a batch-local damping
+// class (low 6 bits, class 0 = all damping factors zero, used for pad cells)
+// and the two face-form bits.
+constexpr uint32_t kClassMask = 0x3fu;
+constexpr uint32_t kClassSelfOff = 0x40u;
+constexpr uint32_t kClassSrcX = 0x80u;
 
 // ---------------------------------------------------------------------------
 // Setup of the per-cell coefficients of the flux form (evaluated in double
@@ -123,8 +128,8 @@ __global__ void k_vel_to_flux(const TubeDesc* __restrict__ tubes, Bufs<float> b,
 }
 
 template <int K>
-__device__ inline uint32_t code_of(const uint32_t* cw, int k) {
-    return (k & 1) ? (cw[k >> 1] >> 16) : (cw[k >> 1] & 0xffffu);
+__device__ inline uint32_t class_of(const uint32_t* cw, int k) {
+    return (cw[k >> 2] >> (8 * (k & 3))) & 0xffu;
 }
 
 template <int K>
@@ -135,8 +140,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
 #pragma clang fp contract(fast)
     constexpr int S = 2 * K + 1;                 // record stride (floats)
     extern __shared__ float lds[];
-    __shared__ float2 vtab[64];                  // (ax, ay) by code_x + 8 code_y
-    __shared__ float plut[8];                    // aP by code_p
+    __shared__ float2 vtab[64];                  // (ax, ay) by damping class
+    __shared__ float plut[64];                   // aP by damping class
     __shared__ double sexe[2];                   // source value of steps t, t + 1
     __shared__ TwoMassState tm;
     // threads [0, T) own cells; the last wave is the control wave (source,
@@ -171,8 +176,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         const float* pL = rec0 + (jL / K) * S + jL % K;
         const float* pE = rec0 + (jE / K) * S + jE % K;
         double pexc = 0.0;
-        if (lane < 8) plut[lane] = f.lut[lane];
-        vtab[lane] = make_float2(f.lut[lane & 7], f.lut[lane >> 3]);
+        plut[lane] = f.ctab[lane];
+        vtab[lane] = make_float2(f.ctab[64 + lane], f.ctab[128 + lane]);
         if (lane == 0) {
             pexc = b.pexc[tube];
             if (twomass) tm = b.tm[tube];
@@ -208,7 +213,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
 
     // --------------------------- cell waves ---------------------------
     float Ux[K], kP[K], gX[K], gY[K];
-    uint32_t cw[(K + 1) / 2];
+    uint32_t cw[(K + 3) / 4];
     const int jb = t * K;
     {
         float* const rec = rec0 + t * S;
@@ -216,24 +221,22 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             float p = 0.f, uy = 0.f, ux = 0.f, kp = 0.f, gx = 0.f, gy = 0.f;
-            uint32_t cd = kCodeZero;
+            uint32_t cd = 0u;
             if (r < Hi && c < Wi) {
                 const long long q = co + (long long)(r + 1) * W + (c + 1);
                 p = b.P[q]; uy = f.Uyf[q]; ux = f.Uxf[q];
-                kp = f.kP[q]; gx = f.gX[q]; gy = f.gY[q]; cd = f.code[q];
+                kp = f.kP[q]; gx = f.gX[q]; gy = f.gY[q]; cd = f.cls8[q];
             }
             rec[k] = p;
             rec[K + k] = uy;
             Ux[k] = ux; kP[k] = kp; gX[k] = gx; gY[k] = gy;
-            if (k & 1) cw[k >> 1] |= cd << 16; else cw[k >> 1] = cd;
+            if (k & 3) cw[k >> 2] |= cd << (8 * (k & 3)); else cw[k >> 2] = cd;
             if (++c == Wp) { c = 0; ++r; }
         }
         rec[2 * K] = Ux[K - 1];
     }
     __syncthreads();
 
-    const char* const vt = reinterpret_cast<const char*>(vtab);
-    const char* const pt = reinterpret_cast<const char*>(plut);
     for (int ts = t0; ts < tend; ++ts) {
         // Launder the lane index once per phase so the compiler recomputes
         // the record address instead of keeping per-cell copies live across
@@ -241,7 +244,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         int tl = t;
         asm volatile("" : "+v"(tl));
 #pragma unroll
-        for (int k = 0; k < (K + 1) / 2; ++k) asm volatile("" : "+v"(cw[k]));
+        for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
         // ---------------- pressure phase ----------------
         {
             float* const rec = rec0 + tl * S;
@@ -254,7 +257,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 const float uyb = (k + mW >= K) ? nb[k + K + 1] : nb[k];
                 const float left = k > 0 ? Ux[k - 1] : uxl0;
                 const float dv = (Ux[k] - left) + (uy - uyb);
-                const float aP = *reinterpret_cast<const float*>(pt + ((code_of<K>(cw, k) >> (kShiftP - 2)) & 0x1cu));
+                const float aP = plut[class_of<K>(cw, k) & kClassMask];
                 rec[k] = aP * p - kP[k] * dv;
             }
         }
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         {
             asm volatile("" : "+v"(tl));
 #pragma unroll
-            for (int k = 0; k < (K + 1) / 2; ++k) asm volatile("" : "+v"(cw[k]));
+            for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
             float* const rec = rec0 + tl * S;
             const float* const na = rec - nbOff;          // cell above, slot k - m of lane l - q
             float pn = rec[0];
@@ -273,11 +276,11 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 const float p = pn;
                 pn = (k < K - 1) ? rec[k + 1] : rec[S];   // next lane's first cell
                 const float pa = (k < mW) ? na[k - K - 1] : na[k];
-                const uint32_t cd = code_of<K>(cw, k);
-                const float2 axy = *reinterpret_cast<const float2*>(vt + (cd & kMaskXY));
+                const uint32_t cb = class_of<K>(cw, k);
+                const float2 axy = vtab[cb & kClassMask];
                 // walls with admittance: U = -g P_neighbour; excitation: Ux = g exeV
-                const float ps = (cd & kCodeSelfOff) ? 0.f : p;
-                const float prx = (cd & kCodeSrcX) ? -ev : pn;
+                const float ps = (cb & kClassSelfOff) ? 0.f : p;
+                const float prx = (cb & kClassSrcX) ? -ev : pn;
                 Ux[k] = axy.x * Ux[k] + gX[k] * (ps - prx);
                 rec[K + k] = axy.y * rec[K + k] + gY[k] * (ps - pa);
             }
@@ -347,7 +350,7 @@ int fast_cells_per_lane(int max_padded_cells, int threads) {
 
 int fast_guard_lanes(int K, int max_padded_width) { return (max_padded_width + K - 1) / K + 1; }
 
-size_t fast_lds_static_bytes() { return sizeof(TwoMassState) + 64 * sizeof(float2) + 8 * sizeof(float) + 2 * sizeof(double) + 64; }
+size_t fast_lds_static_bytes() { return sizeof(TwoMassState) + 64 * sizeof(float2) + 64 * sizeof(float) + 2 * sizeof(double) + 64; }
 
 hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<float> b,
                              FastBufs f, hipStream_t st) {
