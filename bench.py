@@ -23,7 +23,7 @@ BLOCK = 256
 # threads, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes, each scaled by
 # the same counter over a plain copy kernel of known size (FETCH x2.000,
 # WRITE x1.000); profiles/k_fast_step_batch256_fp32_pmc.md.
-PMC_TRAFFIC_BYTES = 1.515e9
+PMC_TRAFFIC_BYTES = 1.830e9
 
 
 def log(msg):
@@ -251,7 +251,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": PMC_TRAFFIC_BYTES if (kernel == _abi.K_FAST and threads == 1024
                                                       and args.tubes == 256) else None,
-                     "kernel": "k_step (batched FP32)", "mean_launch_ms": mean_launch_ms,
+                     "kernel": "k_fast_step<K> (batched FP32)", "mean_launch_ms": mean_launch_ms,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "adjoint": adjoint,
