@@ -94,6 +94,19 @@ public:
     int* getBoundarySegmentType();
     float* getZ_inv() const;
 
+    // For debugging purposes (FDTDSolver.cu:88-284): CSV dumps into debug_dir.
+    // Pointers may be device pointers handed out by the getters or host memory.
+    void printInMatrixFormat(float* data, std::string file_name, int num_rows, int num_cols);
+    void printInMatrixFormat(int* data, std::string file_name, int num_rows, int num_cols);
+    void printGridCellComponents();
+    void printNeighboringCellsData(std::string file_name, float* data_cur, float* data_right, float* data_top);
+    void printNeighboringCellsData(std::string file_name, int* data_cur, int* data_right, int* data_top);
+    void printDepthComponents(float* data, std::string file_name);
+    std::string debug_dir = "../../data/";
+    // setupWaveSolver() writes cell_type_vals, beta_vals and sigma_prime_dt_vals
+    // as the reference's calculateSolverCoefficients does (FDTDSolver.cu:689-712)
+    bool write_setup_csv = true;
+
     // Host copies of the reference's device vectors.
     std::vector<float> getDepthP();
     std::vector<float> getDepthX();
@@ -132,6 +145,7 @@ private:
     mutable std::vector<float*> fptr_cache_;
     float* coeff_ptr(int k) const;
     void load_area_function();
+    void write_setup_dumps();
 };
 
 #endif

@@ -198,6 +198,11 @@ int vs_dtft(const double* x, int n, double fs, int hann, const double* freqs, in
  * injected source velocity (kind 1) over the steps advanced so far. */
 int vs_probe_dtft(vs_handle* h, int tube, int kind, int hann, const double* freqs, int nf, double* re, double* im);
 
+/* Copies bytes from a device pointer handed out by vs_field_ptr (or from host
+ * memory) into host memory: the cudaMemcpy of the reference's debug printers
+ * (src/FDTDSolver.cu:95, 125, 201, 232-234). */
+int vs_copy_to_host(void* dst, const void* src, size_t bytes);
+
 /* Interior cell updates of one step summed over the batch:
  * sum (frameH-2)(frameW-2)  (FDTDEngineCPU.cpp:316-317). */
 int vs_cells_per_step(const vs_handle* h, long long* cells);

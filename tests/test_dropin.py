@@ -92,3 +92,29 @@ def test_dropin_float_step_count(hip_device, tmp_path):
     r, out = run_demo([1, 1, 1, 1, 1, 1, 15, 0.002, 1, 1, 0, 64, 0], tmp_path)
     assert r.returncode == 0, r.stderr
     assert np.loadtxt(out).size == orc.float_num_steps(0.002)
+
+
+@pytest.mark.gpu
+def test_setup_csv_dumps_match_cell_types(hip_device, tmp_path):
+    # FDTDSolver.cu:664-712: setupWaveSolver writes the cell types, beta and
+    # sigma'dt of every interior cell and its right / top neighbours into
+    # ../../data/ relative to the working directory
+    run_dir = tmp_path / "build" / "bin"
+    run_dir.mkdir(parents=True)
+    (tmp_path / "data").mkdir()
+    env = dict(os.environ, VOCALSIM_DATA_DIR=DATA)
+    out = tmp_path / "p.txt"
+    r = subprocess.run([DEMO] + [str(a) for a in (1, 1, 1, 1, 1, 1, 15, 0.02, 1, 1, 0, 64, 50)] + [str(out)],
+                       capture_output=True, text=True, env=env, cwd=run_dir, timeout=300)
+    assert r.returncode == 0, r.stderr
+    spec = tubes.read_area_file(DATA + "vocaltract_a.txt").replace(num_steps=50)
+    ct = orc.OracleTube(spec, 64).field("cell_type").astype(int)
+    lines = (tmp_path / "data" / "cell_type_vals.csv").read_text().splitlines()
+    assert lines[0] == "current, right, top"
+    got = np.array([[int(x) for x in l.split(",")] for l in lines[1:]])
+    want = np.stack([ct[1:-1, 1:-1].ravel(), ct[1:-1, 2:].ravel(), ct[:-2, 1:-1].ravel()], axis=1)
+    assert np.array_equal(got, want)
+    beta = (tmp_path / "data" / "beta_vals.csv").read_text().splitlines()
+    assert beta[0] == "cuurent, right, top" and len(beta) == len(lines)
+    sig = np.loadtxt(tmp_path / "data" / "sigma_prime_dt_vals.csv", delimiter=",", skiprows=1)
+    assert sig.shape == (len(lines) - 1, 3) and np.isclose(sig.max(), 1e6)

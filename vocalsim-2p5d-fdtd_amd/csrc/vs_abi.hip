@@ -1170,6 +1170,20 @@ int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) 
     return rc;
 }
 
+int vs_copy_to_host(void* dst, const void* src, size_t bytes) {
+    if (!dst || !src) return fail(VS_E_INVALID, "vs_copy_to_host: null pointer");
+    if (bytes == 0) return VS_OK;
+    hipPointerAttribute_t attr{};
+    const hipError_t e = hipPointerGetAttributes(&attr, src);
+    if (e == hipSuccess && attr.type == hipMemoryTypeDevice) {
+        VS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    } else {
+        (void)hipGetLastError();
+        std::memcpy(dst, src, bytes);
+    }
+    return VS_OK;
+}
+
 int vs_cells_per_step(const vs_handle* h, long long* cells) {
     if (check_handle(h) || !cells) return fail(VS_E_INVALID, "vs_cells_per_step: bad argument");
     *cells = h->cells_per_step;

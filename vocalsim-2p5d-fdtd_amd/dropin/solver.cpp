@@ -122,6 +122,7 @@ void FDTDSolver::setupWaveSolver() {
     excitationX = d.excitationX; excitationY = d.excitationY;
     std::cout << "Simulation frame height = " << frameH << std::endl;
     std::cout << "Simulation frame width = " << frameW << std::endl;
+    if (write_setup_csv) write_setup_dumps();
 }
 
 float* FDTDSolver::coeff_ptr(int k) const {
@@ -191,6 +192,121 @@ std::vector<FDTDSolver::GridCellComponents> FDTDSolver::getPV_N() {
     std::vector<GridCellComponents> out(n);
     for (size_t i = 0; i < n; ++i) out[i] = GridCellComponents{P[i], Vx[i], Vy[i], t[i]};
     return out;
+}
+
+// ------------------------------------------------------- debug CSV dumps
+// FDTDSolver.cu:88-284.  Values use the default stream format, as there.
+namespace {
+template <typename T>
+std::vector<T> host_copy(const T* p, size_t n) {
+    std::vector<T> v(n);
+    check(vs_copy_to_host(v.data(), p, n * sizeof(T)), "vs_copy_to_host");
+    return v;
+}
+}  // namespace
+
+void FDTDSolver::printInMatrixFormat(float* data, std::string file_name, int num_rows, int num_cols) {
+    const std::vector<float> v = host_copy(data, (size_t)num_rows * num_cols);
+    std::ofstream out(debug_dir + file_name + ".csv", std::ios::out | std::ios::trunc);
+    if (!out) { std::cerr << "Failed to open " << file_name + ".csv" << " for writing." << std::endl; return; }
+    for (int i = 0; i < num_rows; ++i) {
+        for (int j = 0; j < num_cols; ++j) {
+            out << v[(size_t)i * num_cols + j];
+            if (j < frameW - 1) out << ",";   // the reference tests against frameW, not num_cols
+        }
+        out << "\n";
+    }
+}
+
+void FDTDSolver::printInMatrixFormat(int* data, std::string file_name, int num_rows, int num_cols) {
+    const std::vector<int> v = host_copy(data, (size_t)num_rows * num_cols);
+    std::ofstream out(debug_dir + file_name + ".csv", std::ios::out | std::ios::trunc);
+    if (!out) { std::cerr << "Failed to open " << file_name + ".csv" << " for writing." << std::endl; return; }
+    for (int i = 0; i < num_rows; ++i) {
+        for (int j = 0; j < num_cols; ++j) {
+            out << v[(size_t)i * num_cols + j];
+            if (j < frameW - 1) out << ",";
+        }
+        out << "\n";
+    }
+}
+
+void FDTDSolver::printGridCellComponents() {
+    const std::vector<GridCellComponents> pv = getPV_N();
+    auto write = [&](auto get, const std::string& name) {
+        std::ofstream out(debug_dir + name, std::ios::out | std::ios::trunc);
+        if (!out) { std::cerr << "Failed to open " << debug_dir + name << "\n"; return; }
+        for (int i = 0; i < frameH; ++i) {
+            for (int j = 0; j < frameW; ++j) {
+                out << get(pv[(size_t)i * frameW + j]);
+                if (j + 1 < frameW) out << ',';
+            }
+            out << '\n';
+        }
+    };
+    write([](const GridCellComponents& c) { return c.Pr; }, "Pressure.csv");
+    write([](const GridCellComponents& c) { return c.Vx; }, "Vx.csv");
+    write([](const GridCellComponents& c) { return c.Vy; }, "Vy.csv");
+    write([](const GridCellComponents& c) { return c.cell_type; }, "cell_types.csv");
+}
+
+void FDTDSolver::printDepthComponents(float* data, std::string file_name) {
+    const std::vector<float> v = host_copy(data, (size_t)frameH * frameW);
+    std::ofstream out(debug_dir + file_name + ".csv", std::ios::out | std::ios::trunc);
+    if (!out) { std::cerr << "Failed to open " << file_name + ".csv" << " for writing." << std::endl; return; }
+    for (int i = 0; i < frameH; ++i) {
+        for (int j = 0; j < frameW; ++j) {
+            out << v[(size_t)i * frameW + j];
+            if (j + 1 < frameW) out << ",";
+        }
+        out << "\n";
+    }
+}
+
+namespace {
+template <typename T>
+void neighbor_csv(const std::string& path, const char* header, const std::vector<T>& a, const std::vector<T>& b,
+                  const std::vector<T>& c) {
+    std::ofstream out(path, std::ios::out | std::ios::trunc);
+    if (!out) { std::cerr << "Failed to open " << path << " for writing." << std::endl; return; }
+    out << header;
+    for (size_t i = 0; i < a.size(); ++i) out << a[i] << "," << b[i] << "," << c[i] << "\n";
+}
+}  // namespace
+
+void FDTDSolver::printNeighboringCellsData(std::string file_name, float* cur, float* right, float* top) {
+    const size_t n = (size_t)(frameH - 2) * (frameW - 2);
+    // the float overload's header carries the reference's spelling
+    neighbor_csv(debug_dir + file_name + ".csv", "cuurent, right, top\n", host_copy(cur, n), host_copy(right, n),
+                 host_copy(top, n));
+}
+
+void FDTDSolver::printNeighboringCellsData(std::string file_name, int* cur, int* right, int* top) {
+    const size_t n = (size_t)(frameH - 2) * (frameW - 2);
+    neighbor_csv(debug_dir + file_name + ".csv", "current, right, top\n", host_copy(cur, n), host_copy(right, n),
+                 host_copy(top, n));
+}
+
+// The three dumps of calculateSolverCoefficients (FDTDSolver.cu:664-712):
+// cell types, beta and sigma'dt of each interior cell and its right and top
+// neighbours, interior cells in row-major order.
+void FDTDSolver::write_setup_dumps() {
+    const size_t HW = (size_t)frameH * frameW, n = (size_t)(frameH - 2) * (frameW - 2);
+    std::vector<int> t(HW);
+    check(vs_get_field(h_, 0, VS_F_CELL_TYPE, t.data(), HW * sizeof(int)), "vs_get_field");
+    std::vector<int> tc(n), tr(n), tt(n);
+    std::vector<float> bc(n), br(n), bt(n), sc(n), sr(n), st(n);
+    for (size_t i = 0; i < n; ++i) {
+        const size_t r = 1 + i / (frameW - 2), c = 1 + i % (frameW - 2);
+        tc[i] = t[r * frameW + c];
+        tr[i] = t[r * frameW + c + 1];
+        tt[i] = t[(r - 1) * frameW + c];
+        bc[i] = beta_[tc[i]]; br[i] = beta_[tr[i]]; bt[i] = beta_[tt[i]];
+        sc[i] = sigma_[tc[i]]; sr[i] = sigma_[tr[i]]; st[i] = sigma_[tt[i]];
+    }
+    neighbor_csv(debug_dir + "cell_type_vals.csv", "current, right, top\n", tc, tr, tt);
+    neighbor_csv(debug_dir + "beta_vals.csv", "cuurent, right, top\n", bc, br, bt);
+    neighbor_csv(debug_dir + "sigma_prime_dt_vals.csv", "cuurent, right, top\n", sc, sr, st);
 }
 
 // ---------------------------------------------------------------- TubeGeometry
