@@ -357,6 +357,47 @@ Bufs<float>& bufs_of<float>(vs_handle* h) { return h->bf; }
 template <>
 Bufs<double>& bufs_of<double>(vs_handle* h) { return h->bd; }
 
+// Per tube, the distinct (cur, right, top, segment-wall) classes of its
+// interior cells, a class byte per cell, and the class coefficients computed
+// on the device in the step's own arithmetic.
+template <typename Real>
+int build_exact_classes(vs_handle* h, Bufs<Real>& b) {
+    std::vector<uint8_t> cls((size_t)h->total_cells), xc((size_t)h->total_cells, 0);
+    VS_HIP(hipMemcpy(cls.data(), b.cls, cls.size(), hipMemcpyDeviceToHost));
+    std::vector<int> keys((size_t)h->n * kExactClasses, 0), nk(h->n, 0);
+    std::vector<int> id(1 << 13, -1);
+    for (int k = 0; k < h->n; ++k) {
+        const TubeDesc& d = h->desc[k];
+        const uint8_t* c = cls.data() + d.cell_off;
+        uint8_t* x = xc.data() + d.cell_off;
+        std::vector<int> used;
+        for (int r = 1; r < d.H - 1; ++r)
+            for (int col = 1; col < d.W - 1; ++col) {
+                const int q = r * d.W + col;
+                const int key = (c[q] & kTypeMask) | ((c[q] & kSegWallBit) ? 16 : 0) | ((c[q + 1] & kTypeMask) << 5) |
+                                ((c[q - d.W] & kTypeMask) << 9);
+                if (id[key] < 0) {
+                    if (nk[k] == kExactClasses)
+                        return fail(VS_E_GEOMETRY, "tube " + std::to_string(k) + ": more than " +
+                                                       std::to_string(kExactClasses) + " cell classes");
+                    id[key] = nk[k];
+                    keys[(size_t)k * kExactClasses + nk[k]] = key;
+                    ++nk[k];
+                    used.push_back(key);
+                }
+                x[q] = (uint8_t)id[key];
+            }
+        for (int key : used) id[key] = -1;
+    }
+    VS_HIP(hipMemcpy(b.xcls, xc.data(), xc.size(), hipMemcpyHostToDevice));
+    VS_HIP(hipMemcpy(b.xkey, keys.data(), keys.size() * sizeof(int), hipMemcpyHostToDevice));
+    VS_HIP(hipMemcpy(b.xn, nk.data(), nk.size() * sizeof(int), hipMemcpyHostToDevice));
+    hipError_t e = launch_exact_classes<Real>(h->d_desc, h->n, b, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("class coefficients: ") + hipGetErrorString(e));
+    return VS_OK;
+}
+
 template <typename Real>
 int build(vs_handle* h, const vs_tube_spec* specs) {
     const bool fp64 = sizeof(Real) == 8;
@@ -416,6 +457,8 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
     for (int k = 0; k < h->n; ++k) any_plane = any_plane || specs[k].cross_sectional_shape == 3;
     const long long npoly = any_plane ? sec * kPlanePoints : 1;
     ALLOC(b.poly_y, npoly); ALLOC(b.poly_z, npoly); ALLOC(b.pbox, any_plane ? sec * 3 : 1);
+    ALLOC(b.xcls, cell); ALLOC(b.xkey, (long long)h->n * kExactClasses); ALLOC(b.xn, h->n);
+    ALLOC(b.xtab, (long long)h->n * kExactClasses * kExactSlots);
     ALLOC(h->d_desc, h->n); ALLOC(h->d_order, h->n);
     // host staging of the per-section and per-step inputs
     std::vector<float> cum; std::vector<int> diam; std::vector<Real> minor, dsc, zs, srct;
@@ -491,7 +534,7 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
     unsigned status = 0;
     VS_HIP(hipMemcpy(&status, b.status, sizeof(unsigned), hipMemcpyDeviceToHost));
     if (status & kStatusSqrtDomain) return fail(VS_E_GEOMETRY, "depth sqrt domain error");
-    return VS_OK;
+    return build_exact_classes<Real>(h, b);
 }
 
 int check_handle(const vs_handle* h) {

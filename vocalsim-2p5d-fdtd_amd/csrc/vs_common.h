@@ -187,7 +187,18 @@ struct Bufs {
     float* poly_y;       // per section: aligned contour (kPlanePoints), single-plane shape only
     float* poly_z;
     float* pbox;         // per section: minY, minZ, maxZ of the aligned contour
+    uint8_t* xcls;       // per cell: index of its (cur, right, top, segment-wall) class in its tube
+    int* xkey;           // per tube: kExactClasses class keys (cur | wall << 4 | right << 5 | top << 9)
+    int* xn;             // per tube: number of classes
+    Real* xtab;          // per tube: kExactClasses x kExactSlots coefficients of each class
 };
+
+// Exact step: the 17 coefficients of the reference (FDTDSolver.cu:860-1056)
+// depend only on the (cur, right, top) cell types, so each tube holds them
+// per class: slots 0..16 the coefficients, 17 = Dirichlet cell, 18 = wall
+// cell carrying a boundary segment (z_inv applies).
+constexpr int kExactClasses = 64;
+constexpr int kExactSlots = 20;
 
 // Single-plane-symmetric cross-section of one tube section (vs_plane.hip):
 // the aligned closed contour (3997 points: 1999 on the upper half and their
@@ -271,6 +282,8 @@ hipError_t launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<
 template <typename Real>
 hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tube, Bufs<Real> b,
                          Real* out17, hipStream_t st);
+template <typename Real>
+hipError_t launch_exact_classes(const TubeDesc* d_tubes, int n, Bufs<Real> b, hipStream_t st);
 template <typename Real>
 hipError_t launch_depth_raw(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<Real> b, Real* X,
                             Real* Y, long long stride, hipStream_t st);

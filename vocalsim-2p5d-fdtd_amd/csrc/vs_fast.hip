@@ -38,7 +38,6 @@ namespace {
 // makes the x-face carry the source (excitation column: Ux = g * exeV).
 constexpr int kShiftXY = 3;
 constexpr int kShiftP = 9;
-constexpr uint32_t kMaskXY = 0x1f8u;
 constexpr uint32_t kCodeSelfOff = 1u << 12;
 constexpr uint32_t kCodeSrcX = 1u << 13;
 // The step kernel reads a class byte per cell instead: a batch-local damping

@@ -334,6 +334,25 @@ __global__ void k_coeffs(const TubeDesc* __restrict__ tubes, int tube, Bufs<Real
     for (int k = 0; k < 17; ++k) out[(size_t)k * n + i] = o[k];
 }
 
+// Coefficients of every class of every tube, in the arithmetic of coeffs_of.
+template <typename Real>
+__global__ void k_exact_classes(const TubeDesc* __restrict__ tubes, Bufs<Real> b) {
+    const int tube = blockIdx.x, c = threadIdx.x;
+    if (c >= b.xn[tube]) return;
+    const TubeDesc& T = tubes[tube];
+    Tables<Real> t;
+    load_tables(T, t);
+    const int key = b.xkey[tube * kExactClasses + c];
+    const int tc = key & 0xf, wall = (key >> 4) & 1, tr = (key >> 5) & 0xf, tt = (key >> 9) & 0xf;
+    Real o[17];
+    coeffs_of(t, tc, tr, tt, o);
+    Real* dst = b.xtab + ((long long)tube * kExactClasses + c) * kExactSlots;
+    for (int k = 0; k < 17; ++k) dst[k] = o[k];
+    dst[17] = (Real)(tc == kNoPressure ? 1 : 0);
+    dst[18] = (Real)wall;
+    dst[19] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // Time step, reference operation order.  One work-group owns one tube for a
 // block of steps; P and V are updated in place with a work-group barrier
@@ -351,10 +370,14 @@ __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict_
     // graph replays read the first step from device memory
     const int t0 = d_t0 ? *d_t0 : t0_arg;
     const TubeDesc& T = tubes[tube];
-    __shared__ Tables<Real> tab;
+    __shared__ Real ctab[kExactClasses * kExactSlots];   // this tube's class coefficients
     __shared__ Real s_exeV;
     const int tid = threadIdx.x, B = blockDim.x;
-    if (tid == 0) load_tables(T, tab);
+    {
+        const Real* src = b.xtab + (long long)tube * kExactClasses * kExactSlots;
+        const int nv = b.xn[tube] * kExactSlots;
+        for (int i = tid; i < nv; i += B) ctab[i] = src[i];
+    }
     const int H = T.H, W = T.W, Wi = W - 2, NI = (H - 2) * Wi;
     const int tend = min(t0 + nsteps, T.num_steps);
     Real* __restrict__ P = b.P + T.cell_off;
@@ -363,8 +386,7 @@ __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict_
     const Real* __restrict__ dP = b.dP + T.cell_off;
     const Real* __restrict__ dX = b.dX + T.cell_off;
     const Real* __restrict__ dY = b.dY + T.cell_off;
-    const uint8_t* __restrict__ cls = b.cls + T.cell_off;
-    const Real* __restrict__ zc = b.zcol + T.col_off;
+        const Real* __restrict__ zc = b.zcol + T.col_off;
     const Real* __restrict__ src = b.source + T.src_off;
     double* probe = b.probe + T.src_off;
     double* strace = b.strace + T.src_off;
@@ -378,7 +400,8 @@ __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict_
         if (twomass) tm = b.tm[tube];
     }
     __syncthreads();
-    const Real K1 = tab.K1;
+    const Real K1 = (Real)T.ph.K1;
+    const uint8_t* __restrict__ xc = b.xcls + T.cell_off;
     // per-thread walk over interior cells i = tid + k * B in row-major order
     const int dr = B / Wi, dc = B - dr * Wi;
     const int r0 = tid / Wi, c0 = tid - r0 * Wi;
@@ -398,11 +421,11 @@ __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict_
                 const Real cxvx = Vx[q] * dX[q] - Vx[q - 1] * dX[q - 1];
                 const Real cyvy = Vy[q] * dY[q] - Vy[q + W] * dY[q + W];
                 const Real num = (P[q] * dP[q]) - (K1 * (cxvx + cyvy));
-                const int tc = cls[q] & kTypeMask;
-                const Real sp = tab.sigma[tc];
+                const Real* o = ctab + xc[q] * kExactSlots;
+                const Real sp = o[8];   // sigma'dt of the cell's own type
                 const Real den = eigen ? (sp * dP[q] + dP[q]) : ((1 + sp) * dP[q]);
                 Real v = num / den;
-                if (dirichlet && tc == kNoPressure) v = 0;
+                if (dirichlet && o[17] != 0) v = 0;
                 P[q] = v;
                 r += dr; c += dc;
                 if (c > Wi) { c -= Wi; ++r; }
@@ -419,10 +442,8 @@ __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict_
             int r = r0 + 1, c = c0 + 1;
             for (int i = tid; i < NI; i += B) {
                 const int q = r * W + c;
-                const uint8_t cc = cls[q];
-                Real o[17];
-                coeffs_of(tab, cc & kTypeMask, cls[q + 1] & kTypeMask, cls[q - W] & kTypeMask, o);
-                const Real z = (cc & kSegWallBit) ? zc[c] : (Real)0;
+                const Real* o = ctab + xc[q] * kExactSlots;
+                const Real z = (o[18] != 0) ? zc[c] : (Real)0;
                 const Real pc = P[q], pr = P[q + 1], pt = P[q - W];
                 const Real cxp = pr - pc;
                 const Real cyp = pt - pc;
@@ -469,6 +490,12 @@ hipError_t launch_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n,
     hipLaunchKernelGGL(k_depth_raw<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, b, tmpA, tmpB, stride);
     hipLaunchKernelGGL(k_depth_smooth<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, tmpA, tmpB, tmpC, tmpD, stride);
     hipLaunchKernelGGL(k_depth_final<Real>, gc, dim3(kSetupThreads), 0, st, d_tubes, b, tmpC, tmpD, stride);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_exact_classes(const TubeDesc* d_tubes, int n, Bufs<Real> b, hipStream_t st) {
+    hipLaunchKernelGGL(k_exact_classes<Real>, dim3(n), dim3(kExactClasses), 0, st, d_tubes, b);
     return hipGetLastError();
 }
 
@@ -519,7 +546,8 @@ hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tu
     template hipError_t launch_coeffs<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,       \
                                             Real*, hipStream_t);                                    \
     template hipError_t launch_depth_raw<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,    \
-                                               Real*, Real*, long long, hipStream_t);
+                                               Real*, Real*, long long, hipStream_t);                 \
+    template hipError_t launch_exact_classes<Real>(const TubeDesc*, int, Bufs<Real>, hipStream_t);
 
 VS_INSTANTIATE(float)
 VS_INSTANTIATE(double)
