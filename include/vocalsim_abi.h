@@ -121,7 +121,13 @@ enum vs_option {
     VS_OPT_USE_GRAPH = 2,        /* 1: replay step blocks from a captured hipGraph */
     VS_OPT_KERNEL = 3,           /* step kernel family, see vs_kernel */
     VS_OPT_TIMING = 4,           /* 1: time every launch with HIP events */
-    VS_OPT_ENGINE_VARIANT = 5    /* 0: FDTDEngineCPU rounding, 1: FDTDEngineEigen (all tubes) */
+    VS_OPT_ENGINE_VARIANT = 5,   /* 0: FDTDEngineCPU rounding, 1: FDTDEngineEigen (all tubes) */
+    VS_OPT_ADJOINT_KERNEL = 6,   /* FP32 reverse-time pass: 0 (default) on-chip kernels when every
+                                    tube fits one work-group, 1: per-step global-memory kernels.
+                                    FP64 always uses the per-step kernels. */
+    VS_OPT_ADJOINT_BLOCK = 7     /* on-chip reverse pass: most steps recorded per block (0, the
+                                    default: as many of the checkpoint spacing as fit in 16 GiB);
+                                    shorter blocks trade one extra forward pass for memory */
 };
 
 enum vs_kernel {

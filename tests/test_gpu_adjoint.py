@@ -11,12 +11,18 @@ Bars:
 * The differentiable forward's probe matches the ordinary step: FP64 within
   1e-12 relative L2, FP32 within 1e-5.
 * Checkpoint spacing and batching do not change the result (bit-identical).
+* The FP32 reverse pass runs on chip by default (scaled adjoints Kp = kP M,
+  Lx = gX lambda_x, Ly = gY lambda_y in the layout of the FP32 step kernel):
+  its forward probe is bit-identical to the FP32 step kernel's; its gradient
+  matches the FP32 per-step kernels (relative L2 <= 1e-4 per kind) and the
+  FP64 adjoint (<= 2e-3); batching is bit-identical; checkpoint spacing and
+  shorter record blocks change only the grouping of double sums (<= 1e-12).
 """
 import numpy as np
 import pytest
 
 import oracle as orc
-from vocalsim import Solver, VocalSimError, tubes
+from vocalsim import Solver, VocalSimError, _abi, tubes
 
 pytestmark = pytest.mark.gpu
 
@@ -63,8 +69,10 @@ def perturbed(spec, kind, sec, sign):
     return spec.replace(**{field: arr}), float(arr[sec])
 
 
-def adjoint_grads(specs, gs, precision=64, checkpoint_every=512):
+def adjoint_grads(specs, gs, precision=64, checkpoint_every=512, adjoint_kernel=0, block=0):
     with Solver(specs, precision=precision) as s:
+        s.set_option(_abi.OPT_ADJOINT_KERNEL, adjoint_kernel)
+        s.set_option(_abi.OPT_ADJOINT_BLOCK, block)
         probes = s.adjoint_forward(checkpoint_every)
         ga, gsc, gm = s.adjoint_backward(gs)
     return probes, ga, gsc, gm
@@ -244,3 +252,51 @@ def test_log_spectrum_loss_gradient_four_tubes(hip_device):
         g = float(grad[tube, sec])
         print(f"[log-spectrum loss] {kind}[{tube},{sec}] adjoint={g:.10e} fd={fd:.10e}", flush=True)
         assert abs(g - fd) <= FD_REL * abs(fd), (kind, g, fd)
+
+
+def test_onchip_forward_probe_is_the_fp32_step_kernel(hip_device):
+    a = base_tube(seed=3, num_steps=700)
+    b = base_tube(seed=6, num_steps=650)
+    with Solver([a, b], precision=32) as s:
+        p_adj = s.adjoint_forward(200)
+    with Solver([a, b], precision=32, threads=1024, kernel=_abi.K_FAST) as s:
+        s.run()
+        p_run = [s.probe(0), s.probe(1)]
+    print(f"[on-chip forward] max |diff| = {max(np.abs(p_adj[i] - p_run[i]).max() for i in range(2)):.3e}", flush=True)
+    assert np.array_equal(p_adj[0], p_run[0]) and np.array_equal(p_adj[1], p_run[1])
+
+
+def test_onchip_reverse_matches_per_step_kernels(hip_device):
+    spec = base_tube(seed=7, num_steps=1100)
+    g = loss_weights(spec.num_steps, seed=2)
+    p1, a1, s1, m1 = adjoint_grads([spec], [g], 32, 256, adjoint_kernel=0)
+    p2, a2, s2, m2 = adjoint_grads([spec], [g], 32, 256, adjoint_kernel=1)
+    print(f"[on-chip vs per-step fp32] probe rel_l2={rel_l2(p1[0], p2[0]):.3e}", flush=True)
+    assert rel_l2(p1[0], p2[0]) <= 1e-5
+    for kind, x, y in (("area", a1, a2), ("scale", s1, s2), ("mu", m1, m2)):
+        r = rel_l2(x[0], y[0])
+        print(f"[on-chip vs per-step fp32] {kind} rel_l2={r:.3e}", flush=True)
+        assert np.all(np.isfinite(x[0])) and r <= 1e-4, kind
+
+
+def test_onchip_reverse_blocks_spacing_and_batching(hip_device):
+    a = base_tube(seed=3, num_steps=800)
+    b = base_tube(seed=4, num_steps=650).replace(lengths_cm=np.full(44, 15.0 / 44, dtype=np.float32))
+    ga_ = loss_weights(a.num_steps, 0)
+    gb_ = loss_weights(b.num_steps, 1)
+    ref = adjoint_grads([a], [ga_], 32, 512)
+    for label, kw in (("spacing 97", dict(checkpoint_every=97)), ("block 60", dict(checkpoint_every=512, block=60)),
+                      ("spacing 300 block 128", dict(checkpoint_every=300, block=128))):
+        got = adjoint_grads([a], [ga_], 32, **kw)
+        assert np.array_equal(got[0][0], ref[0][0]), label
+        for x, y in zip(got[1:], ref[1:]):
+            r = rel_l2(x[0], y[0])
+            print(f"[on-chip {label}] rel_l2={r:.3e}", flush=True)
+            assert r <= 1e-12, label
+    pb, B1, SB1, MB1 = adjoint_grads([b], [gb_], 32, 200)
+    pa, A1, S1, M1 = adjoint_grads([a], [ga_], 32, 200)
+    pab, AB, SAB, MAB = adjoint_grads([a, b], [ga_, gb_], 32, 200)
+    assert np.array_equal(pab[0], pa[0]) and np.array_equal(pab[1], pb[0])
+    for x, y in ((AB[0], A1[0]), (SAB[0], S1[0]), (MAB[0], M1[0]), (AB[1], B1[0]), (SAB[1], SB1[0]),
+                 (MAB[1], MB1[0])):
+        assert np.array_equal(x, y)

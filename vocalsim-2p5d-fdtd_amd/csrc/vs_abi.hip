@@ -321,11 +321,24 @@ struct vs_handle {
     int adj_S = 0;             // checkpoint spacing of the slabs held
     bool adj_forward = false;  // checkpoints hold a completed forward pass
     std::vector<void*> adj_slabs;
+    int adj_kernel = 0;        // VS_OPT_ADJOINT_KERNEL
+    int adj_block = 0;         // VS_OPT_ADJOINT_BLOCK
+    bool adj_onchip = false;   // the held forward pass ran on chip
+    // on-chip FP32 reverse pass: lanes shape, slabs and scratch
+    int adj_K = 0, adj_G = 0, adj_B = 0, adj_fast_S = 0;
+    FastAdj fa{};
+    float* adj_ck = nullptr;    // checkpoints: P, Ux, Uy (total_cells each) every S steps
+    float* adj_sub = nullptr;   // block starts inside one segment when the record block B < S
+    double* adj_strace = nullptr;
+    double* adj_pexc = nullptr;
+    TwoMassState* adj_tm = nullptr;
+    std::vector<void*> adj_fast_slabs;
 
     ~vs_handle() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (void* p : allocs) (void)hipFree(p);
         for (void* p : adj_slabs) (void)hipFree(p);
+        for (void* p : adj_fast_slabs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -579,15 +592,30 @@ int build_fast_classes(vs_handle* h) {
     return VS_OK;
 }
 
+// Whether the batch fits the on-chip kernels at `threads` threads per tube
+// (cells per lane K <= 24, LDS within the device limit, one srate_mul).
+bool fast_fits(const vs_handle* h, int threads, int* K_out, int* G_out) {
+    if (h->precision != VS_FP32 || threads < 128) return false;
+    const int K = fast_cells_per_lane(h->max_padded, threads);
+    if (K > 24) return false;
+    const int G = fast_guard_lanes(K, h->max_wp);
+    if ((long long)(fast_lds_bytes(K, threads, G) + fast_lds_static_bytes()) > h->lds_limit) return false;
+    for (const TubeDesc& d : h->desc)
+        if (d.srate_mul != h->desc[0].srate_mul) return false;
+    if (K_out) *K_out = K;
+    if (G_out) *G_out = G;
+    return true;
+}
+
 // Lazily builds the per-cell coefficients of the FP32 flux-form kernel.
-int prepare_fast(vs_handle* h) {
+int prepare_fast(vs_handle* h, int threads) {
     if (h->precision != VS_FP32) return fail(VS_E_INVALID, "the flux-form kernel is FP32 only");
-    if (h->threads < 128) return fail(VS_E_INVALID, "the on-chip kernel needs at least 128 threads");
-    const int K = fast_cells_per_lane(h->max_padded, h->threads);
+    if (threads < 128) return fail(VS_E_INVALID, "the on-chip kernel needs at least 128 threads");
+    const int K = fast_cells_per_lane(h->max_padded, threads);
     if (K > 24) return fail(VS_E_INVALID, "tube too large for the on-chip kernel at this work-group size");
     const int G = fast_guard_lanes(K, h->max_wp);
-    if ((long long)(fast_lds_bytes(K, h->threads, G) + fast_lds_static_bytes()) > h->lds_limit)
-        return fail(VS_E_INVALID, "on-chip kernel needs " + std::to_string(fast_lds_bytes(K, h->threads, G)) +
+    if ((long long)(fast_lds_bytes(K, threads, G) + fast_lds_static_bytes()) > h->lds_limit)
+        return fail(VS_E_INVALID, "on-chip kernel needs " + std::to_string(fast_lds_bytes(K, threads, G)) +
                     " bytes of LDS, device allows " + std::to_string(h->lds_limit));
     if (!h->fast_ready) {
         FastBufs& f = h->fb;
@@ -607,7 +635,6 @@ int prepare_fast(vs_handle* h) {
         h->fast_ready = true;
     }
     {
-        const int K = fast_cells_per_lane(h->max_padded, h->threads);
         hipError_t e = configure_fast(K);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("configure: ") + hipGetErrorString(e));
     }
@@ -673,7 +700,7 @@ template <typename Real>
 int run_steps(vs_handle* h, int n_steps) {
     Bufs<Real>& b = bufs_of<Real>(h);
     if (h->kernel == VS_K_FAST) {
-        const int rc = prepare_fast(h);
+        const int rc = prepare_fast(h, h->threads);
         if (rc != VS_OK) return rc;
     }
     bool graph_ran = false;
@@ -1047,6 +1074,17 @@ int vs_set_option(vs_handle* h, int option, int value) {
         if (value != h->kernel && value == VS_K_FAST) h->flux_valid = false;
         h->kernel = value; return VS_OK;
     case VS_OPT_TIMING: h->timing = value ? 1 : 0; return VS_OK;
+    case VS_OPT_ADJOINT_KERNEL:
+        if (value != 0 && value != 1) return fail(VS_E_INVALID, "adjoint kernel must be 0 (on chip) or 1 (per step)");
+        if (value != h->adj_kernel) h->adj_forward = false;
+        h->adj_kernel = value; return VS_OK;
+    case VS_OPT_ADJOINT_BLOCK:
+        if (value < 0) return fail(VS_E_INVALID, "adjoint block must be >= 0");
+        if (value != h->adj_block) {   // slabs are re-sized by the next forward pass
+            h->adj_fast_S = 0;
+            if (h->adj_onchip) h->adj_forward = false;
+        }
+        h->adj_block = value; return VS_OK;
     case VS_OPT_ENGINE_VARIANT: {
         if (value != 0 && value != 1) return fail(VS_E_INVALID, "engine variant must be 0 or 1");
         for (TubeDesc& d : h->desc) d.variant = value;
@@ -1067,6 +1105,8 @@ int vs_get_option(const vs_handle* h, int option, int* value) {
     case VS_OPT_KERNEL: *value = h->kernel; return VS_OK;
     case VS_OPT_TIMING: *value = h->timing; return VS_OK;
     case VS_OPT_ENGINE_VARIANT: *value = h->desc.empty() ? 0 : h->desc[0].variant; return VS_OK;
+    case VS_OPT_ADJOINT_KERNEL: *value = h->adj_kernel; return VS_OK;
+    case VS_OPT_ADJOINT_BLOCK: *value = h->adj_block; return VS_OK;
     default: return fail(VS_E_INVALID, "unknown option");
     }
 }
@@ -1262,10 +1302,9 @@ double dz_dmu(double mu, double rho_c) {
     return dz_ds * ds_da * da_dm * kap;
 }
 
-// Allocates the per-cell buffers (once) and the checkpoint / segment slabs
-// for checkpoint spacing S (again when S changes).
+// Allocates the per-cell buffers of the reverse-time pass (once).
 template <typename Real>
-int adj_prepare(vs_handle* h, int S) {
+int adj_cells(vs_handle* h) {
     AdjBufs<Real>& a = adj_of<Real>(h);
     Bufs<Real>& b = bufs_of<Real>(h);
     int rc;
@@ -1284,6 +1323,16 @@ int adj_prepare(vs_handle* h, int S) {
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint coefficients: ") + hipGetErrorString(e));
         h->adj_ready = true;
     }
+    return VS_OK;
+}
+
+// Per-cell buffers plus the checkpoint / segment slabs of the per-step
+// kernels for checkpoint spacing S (again when S changes).
+template <typename Real>
+int adj_prepare(vs_handle* h, int S) {
+    AdjBufs<Real>& a = adj_of<Real>(h);
+    int rc = adj_cells<Real>(h);
+    if (rc != VS_OK) return rc;
     if (h->adj_S != S) {
         VS_HIP(hipStreamSynchronize(h->stream));
         for (void* p : h->adj_slabs) (void)hipFree(p);
@@ -1324,6 +1373,107 @@ int adj_prepare(vs_handle* h, int S) {
     return VS_OK;
 }
 
+// ----- on-chip FP32 reverse pass -----
+// Cell lanes per tube of the on-chip reverse kernels (one work-group of
+// kAdjLanes + 64 threads per tube in the forward, kAdjLanes in the reverse).
+constexpr int kAdjThreads = 1024;
+constexpr int kAdjLanes = kAdjThreads - 64;
+// Device memory allowed for the two per-step slabs of one record block.
+constexpr long long kAdjSlabBudget = 16LL << 30;
+
+bool adj_uses_onchip(const vs_handle* h) {
+    return h->precision == VS_FP32 && h->adj_kernel == 0 && fast_fits(h, kAdjThreads, nullptr, nullptr);
+}
+
+// Slabs of the on-chip pass for checkpoint spacing S: checkpoints of the
+// whole state, the record block of B <= S steps (rec, adj), the adjoint
+// state between launches and, when B < S, the block starts of one segment.
+int adj_fast_prepare(vs_handle* h, int S) {
+    int K = 0, G = 0;
+    if (!fast_fits(h, kAdjThreads, &K, &G)) return fail(VS_E_INVALID, "batch does not fit the on-chip kernels");
+    int rc = prepare_fast(h, kAdjThreads);
+    if (rc != VS_OK) return rc;
+    if (h->adj_fast_S == S && h->adj_K == K) return VS_OK;
+    VS_HIP(hipStreamSynchronize(h->stream));
+    for (void* p : h->adj_fast_slabs) (void)hipFree(p);
+    h->adj_fast_slabs.clear();
+    h->adj_fast_S = 0;
+    h->adj_forward = false;
+    const long long KT = (long long)K * kAdjLanes;
+    const long long per_step = 2LL * h->n * 3 * KT * (long long)sizeof(float);
+    int B = (int)std::max(1LL, std::min<long long>(S, kAdjSlabBudget / per_step));
+    if (h->adj_block > 0) B = std::min(B, h->adj_block);
+    const long long C = h->total_cells;
+    const long long nck = (h->max_steps + S - 1) / S;
+    const long long nsub = B < S ? (S + B - 1) / B : 0;
+    auto get = [&](void** p, long long bytes, const char* what) -> int {
+        hipError_t e = hipMalloc(p, (size_t)std::max(bytes, 1LL));
+        if (e != hipSuccess)
+            return fail(VS_E_DEVICE, std::string(what) + " of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+        h->adj_fast_slabs.push_back(*p);
+        return VS_OK;
+    };
+    void *ck, *sub, *rec, *adj, *st, *tr, *pe, *tm;
+    if ((rc = get(&ck, nck * 3 * C * 4, "checkpoints")) || (rc = get(&sub, nsub * 3 * C * 4, "block starts")) ||
+        (rc = get(&rec, (long long)h->n * B * 3 * KT * 4, "forward record slab")) ||
+        (rc = get(&adj, (long long)h->n * B * 3 * KT * 4, "adjoint record slab")) ||
+        (rc = get(&st, (long long)h->n * 3 * KT * 4, "adjoint state")) ||
+        (rc = get(&tr, h->total_src * 8, "source trace")) || (rc = get(&pe, h->n * 8LL, "excitation pressure")) ||
+        (rc = get(&tm, h->n * (long long)sizeof(TwoMassState), "two-mass state")))
+        return rc;
+    h->adj_ck = static_cast<float*>(ck);
+    h->adj_sub = static_cast<float*>(sub);
+    h->adj_strace = static_cast<double*>(tr);
+    h->adj_pexc = static_cast<double*>(pe);
+    h->adj_tm = static_cast<TwoMassState*>(tm);
+    FastAdj& fa = h->fa;
+    fa.rec = static_cast<float*>(rec);
+    fa.adj = static_cast<float*>(adj);
+    fa.state = static_cast<float*>(st);
+    fa.slab_tube = (long long)B * 3 * KT;
+    fa.lanes = kAdjLanes;
+    h->adj_K = K;
+    h->adj_G = G;
+    h->adj_B = B;
+    h->adj_fast_S = S;
+    return VS_OK;
+}
+
+// The forward state of the on-chip pass lives in the adjoint buffers, so the
+// solver's own state, probe and source trace are left untouched.
+Bufs<float> adj_fast_bufs(vs_handle* h) {
+    Bufs<float> b = h->bf;
+    AdjBufs<float>& a = h->af;
+    b.P = a.P;
+    b.Vx = a.LX;   // velocity write-back of a launch: scratch
+    b.Vy = a.LY;
+    b.probe = a.probe;
+    b.strace = h->adj_strace;
+    b.pexc = h->adj_pexc;
+    b.tm = h->adj_tm;
+    return b;
+}
+FastBufs adj_fast_flux(vs_handle* h) {
+    FastBufs f = h->fb;
+    f.Uxf = h->af.Ux;
+    f.Uyf = h->af.Uy;
+    return f;
+}
+
+// Copies the forward state (P, Ux, Uy) to / from a slab of 3 * total_cells.
+int adj_state_copy(vs_handle* h, float* slab, bool to_slab) {
+    AdjBufs<float>& a = h->af;
+    const size_t cb = sizeof(float) * (size_t)h->total_cells;
+    const long long C = h->total_cells;
+    float* st[3] = {a.P, a.Ux, a.Uy};
+    for (int i = 0; i < 3; ++i) {
+        float* dst = to_slab ? slab + i * C : st[i];
+        const float* src = to_slab ? st[i] : slab + i * C;
+        VS_HIP(hipMemcpyAsync(dst, src, cb, hipMemcpyDeviceToDevice, h->stream));
+    }
+    return VS_OK;
+}
+
 int adj_check(vs_handle* h) {
     for (int k = 0; k < h->n; ++k)
         if (h->desc[k].source_type == 2)
@@ -1334,12 +1484,80 @@ int adj_check(vs_handle* h) {
     return VS_OK;
 }
 
+// Forward pass on chip: the FP32 flux-form step kernel in S-step launches,
+// the whole state copied to a checkpoint before each.
+int adj_forward_fast(vs_handle* h, int S, double* probe) {
+    int rc;
+    if ((rc = adj_cells<float>(h)) != VS_OK || (rc = adj_fast_prepare(h, S)) != VS_OK) return rc;
+    AdjBufs<float>& a = h->af;
+    const size_t cb = sizeof(float) * (size_t)h->total_cells;
+    VS_HIP(hipMemsetAsync(a.P, 0, cb, h->stream));
+    VS_HIP(hipMemsetAsync(a.Ux, 0, cb, h->stream));
+    VS_HIP(hipMemsetAsync(a.Uy, 0, cb, h->stream));
+    const Bufs<float> b = adj_fast_bufs(h);
+    const FastBufs f = adj_fast_flux(h);
+    for (int t0 = 0; t0 < h->max_steps; t0 += S) {
+        if ((rc = adj_state_copy(h, h->adj_ck + (long long)(t0 / S) * 3 * h->total_cells, true)) != VS_OK) return rc;
+        hipError_t e = launch_fast_step(h->adj_K, h->d_desc, h->d_order, h->n, b, f, t0, S, kAdjThreads, h->adj_G,
+                                        nullptr, h->stream);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint forward: ") + hipGetErrorString(e));
+    }
+    VS_HIP(hipStreamSynchronize(h->stream));
+    h->adj_forward = true;
+    h->adj_onchip = true;
+    if (probe) VS_HIP(hipMemcpy(probe, a.probe, sizeof(double) * (size_t)h->total_src, hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+// Reverse pass on chip.  Per segment (last first): the segment is recomputed
+// from its checkpoint in record blocks of B steps (block starts kept when
+// B < S); per block, last first: the recording forward launch, the reverse
+// launch, and the per-cell sums of the block.  Leaves d J / d log(gX, gY, kP)
+// in a.Gx, a.Gy, a.GkP.
+int adj_backward_fast(vs_handle* h) {
+    AdjBufs<float>& a = h->af;
+    const int S = h->adj_fast_S, B = h->adj_B, K = h->adj_K, G = h->adj_G;
+    const long long C = h->total_cells;
+    const Bufs<float> b = adj_fast_bufs(h);
+    const FastBufs f = adj_fast_flux(h);
+    FastAdj fa = h->fa;
+    fa.gprobe = a.gprobe;
+    fa.Lx = a.Gx;
+    fa.Ly = a.Gy;
+    fa.Lp = a.GkP;
+    VS_HIP(hipMemsetAsync(fa.state, 0, sizeof(float) * (size_t)h->n * 3 * K * kAdjLanes, h->stream));
+    const int nseg = (h->max_steps + S - 1) / S;
+    int rc;
+    for (int k = nseg - 1; k >= 0; --k) {
+        const int t0 = k * S, len = std::min(S, h->max_steps - t0);
+        const int nb = (len + B - 1) / B;
+        if ((rc = adj_state_copy(h, h->adj_ck + (long long)k * 3 * C, false)) != VS_OK) return rc;
+        for (int j = 0; j + 1 < nb; ++j) {
+            if ((rc = adj_state_copy(h, h->adj_sub + (long long)j * 3 * C, true)) != VS_OK) return rc;
+            hipError_t e = launch_fast_step(K, h->d_desc, h->d_order, h->n, b, f, t0 + j * B, B, kAdjThreads, G,
+                                            nullptr, h->stream);
+            if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint block starts: ") + hipGetErrorString(e));
+        }
+        for (int j = nb - 1; j >= 0; --j) {
+            if (j + 1 < nb && (rc = adj_state_copy(h, h->adj_sub + (long long)j * 3 * C, false)) != VS_OK) return rc;
+            const int s0 = t0 + j * B, ns = std::min(B, len - j * B);
+            hipError_t e = launch_fast_record(K, h->d_desc, h->d_order, h->n, b, f, fa, s0, ns, kAdjThreads, G,
+                                              h->stream);
+            if (e == hipSuccess) e = launch_fast_reverse(K, h->d_desc, h->d_order, h->n, f, fa, s0, ns, G, h->stream);
+            if (e == hipSuccess) e = launch_fast_adj_sums(K, h->d_desc, h->d_order, h->n, fa, s0, ns, h->stream);
+            if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint reverse: ") + hipGetErrorString(e));
+        }
+    }
+    return VS_OK;
+}
+
 template <typename Real>
 int adj_forward(vs_handle* h, int S, double* probe, size_t len) {
     int rc = adj_check(h);
     if (rc != VS_OK) return rc;
     if (probe && len < (size_t)h->total_src)
         return fail(VS_E_RANGE, "probe buffer needs " + std::to_string(h->total_src) + " values");
+    if (adj_uses_onchip(h)) return adj_forward_fast(h, S, probe);
     if ((rc = adj_prepare<Real>(h, S)) != VS_OK) return rc;
     AdjBufs<Real>& a = adj_of<Real>(h);
     Bufs<Real>& b = bufs_of<Real>(h);
@@ -1353,6 +1571,7 @@ int adj_forward(vs_handle* h, int S, double* probe, size_t len) {
     }
     VS_HIP(hipStreamSynchronize(h->stream));
     h->adj_forward = true;
+    h->adj_onchip = false;
     if (probe)
         VS_HIP(hipMemcpy(probe, a.probe, sizeof(double) * (size_t)h->total_src, hipMemcpyDeviceToHost));
     return VS_OK;
@@ -1385,11 +1604,16 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
     VS_HIP(hipMemsetAsync(a.GkP, 0, cd, h->stream));
     VS_HIP(hipMemsetAsync(a.Gx, 0, cd, h->stream));
     VS_HIP(hipMemsetAsync(a.Gy, 0, cd, h->stream));
-    const int nseg = (h->max_steps + S - 1) / S;
-    for (int k = nseg - 1; k >= 0; --k) {
-        hipError_t e = launch_adj_fwd<Real>(h->d_desc, h->d_order, h->n, b, a, k * S, S, 2, k, h->threads, h->stream);
-        if (e == hipSuccess) e = launch_adj_rev<Real>(h->d_desc, h->d_order, h->n, b, a, k * S, S, h->threads, h->stream);
-        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint reverse: ") + hipGetErrorString(e));
+    int rc = VS_OK;
+    if (h->adj_onchip) {
+        if ((rc = adj_backward_fast(h)) != VS_OK) return rc;
+    } else {
+        const int nseg = (h->max_steps + S - 1) / S;
+        for (int k = nseg - 1; k >= 0; --k) {
+            hipError_t e = launch_adj_fwd<Real>(h->d_desc, h->d_order, h->n, b, a, k * S, S, 2, k, h->threads, h->stream);
+            if (e == hipSuccess) e = launch_adj_rev<Real>(h->d_desc, h->d_order, h->n, b, a, k * S, S, h->threads, h->stream);
+            if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint reverse: ") + hipGetErrorString(e));
+        }
     }
     // raw depth maps for the chain through smoothing and clamping
     long long maxHW = 0;
@@ -1399,20 +1623,29 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
     hipError_t e = launch_depth_raw<Real>(h->d_desc, h->desc.data(), h->n, b, raw, raw + maxHW * h->n, maxHW, h->stream);
     hipError_t e2 = hipStreamSynchronize(h->stream);
     std::vector<Real> rawh;
-    int rc = VS_OK;
     if (e == hipSuccess && e2 == hipSuccess) rc = fetch(rawh, raw, maxHW * h->n * 2);
     (void)hipFree(raw);
     if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("raw depth: ") + hipGetErrorString(e));
     if (e2 != hipSuccess) return fail(VS_E_DEVICE, std::string("reverse pass: ") + hipGetErrorString(e2));
     if (rc != VS_OK) return rc;
-    std::vector<double> GkP, Gx, Gy;
-    std::vector<Real> kP, cx, cy, dP, dX, dY, zcol, minor, dscale;
+    // Lp, Lx, Ly: d J / d log of each cell's kP, x-face and y-face coefficient
+    std::vector<double> Lp, Lx, Ly;
+    std::vector<Real> dP, dX, dY, zcol, minor, dscale;
     std::vector<uint8_t> cls;
     std::vector<int> seg, upper, lower;
     const long long C = h->total_cells;
-    if ((rc = fetch(GkP, a.GkP, C)) || (rc = fetch(Gx, a.Gx, C)) || (rc = fetch(Gy, a.Gy, C)) ||
-        (rc = fetch(kP, a.kP, C)) || (rc = fetch(cx, a.cx, C)) || (rc = fetch(cy, a.cy, C)) ||
-        (rc = fetch(dP, b.dP, C)) || (rc = fetch(dX, b.dX, C)) || (rc = fetch(dY, b.dY, C)) ||
+    if ((rc = fetch(Lp, a.GkP, C)) || (rc = fetch(Lx, a.Gx, C)) || (rc = fetch(Ly, a.Gy, C))) return rc;
+    if (!h->adj_onchip) {
+        // the per-step kernels accumulate d J / d (kP, cx, cy)
+        std::vector<Real> kP, cx, cy;
+        if ((rc = fetch(kP, a.kP, C)) || (rc = fetch(cx, a.cx, C)) || (rc = fetch(cy, a.cy, C))) return rc;
+        for (long long q = 0; q < C; ++q) {
+            Lp[q] *= (double)kP[q];
+            Lx[q] *= (double)cx[q];
+            Ly[q] *= (double)cy[q];
+        }
+    }
+    if ((rc = fetch(dP, b.dP, C)) || (rc = fetch(dX, b.dX, C)) || (rc = fetch(dY, b.dY, C)) ||
         (rc = fetch(cls, b.cls, C)) || (rc = fetch(zcol, b.zcol, h->total_cols)) ||
         (rc = fetch(seg, b.seg, h->total_walls)) || (rc = fetch(upper, b.upper, h->total_walls)) ||
         (rc = fetch(lower, b.lower, h->total_walls)) || (rc = fetch(minor, b.minor, h->total_secs)) ||
@@ -1447,9 +1680,10 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
                 const int i = r * W + c;
                 const long long q = base + i;
                 if (depth3d) {
-                    const double gdp = -GkP[q] * (double)kP[q] / (double)dP[q];
-                    const double gdx = Gx[q] * (double)cx[q] / (double)dX[q];
-                    const double gdy = Gy[q] * (double)cy[q] / (double)dY[q];
+                    // kP ~ 1 / dP, cx ~ dX, cy ~ dY
+                    const double gdp = -Lp[q] / (double)dP[q];
+                    const double gdx = Lx[q] / (double)dX[q];
+                    const double gdy = Ly[q] / (double)dY[q];
                     if (!(Xs[i] < mnR)) gXs[i] += gdx;
                     if (!(Ys[i] < mnR)) gYs[i] += gdy;
                     const Real lX = Xs[i - 1], dYv = Ys[i + W];
@@ -1462,7 +1696,8 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
                 const int tc = cls[q] & kTypeMask;
                 if ((cls[q] & kSegWallBit) && tc != kExcitation) {
                     const double z = (double)zcol[d.col_off + c];
-                    if (z != 0.0) gzc[c] += (Gx[q] * (double)cx[q] + Gy[q] * (double)cy[q]) / z;
+                    // admittance faces: cx, cy ~ z
+                    if (z != 0.0) gzc[c] += (Lx[q] + Ly[q]) / z;
                 }
             }
         // half-cell smoothing, transposed (open-space cells read the mouth-end cell)

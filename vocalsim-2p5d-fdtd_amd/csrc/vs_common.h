@@ -241,6 +241,27 @@ struct AdjBufs {
     int S;
 };
 
+// On-chip FP32 reverse pass (vs_fast.hip).  With the scaled adjoint
+// variables  Kp = kP * M,  Lx = gX * lambda_x,  Ly = gY * lambda_y  the
+// reverse step has the stencil of the forward step (pressure-like phase
+// reading Lx on the left and Ly below, face phase reading Kp on the right
+// and above), so it runs in the same on-chip layout.  Slabs hold, per tube
+// (launch order) and per step of a segment, three fields of K * T floats in
+// lane-slot order s = k * T + t (slot k of cell lane t):
+//   rec  forward factors of step n: dUx'/dgX, dUy'/dgY, flux divergence
+//   adj  Lx and Ly entering reverse step n, and Kp of step n
+// The sums over steps of adj * rec are the log-derivatives of J with respect
+// to each cell's gX, gY and kP.
+struct FastAdj {
+    float* rec;
+    float* adj;
+    float* state;          // per tube 3 * K * T: Kp, Ly, Lx between launches
+    long long slab_tube;   // floats per tube of rec / adj (S * 3 * K * T)
+    int lanes;             // T: cell lanes per work-group
+    const double* gprobe;  // per step: d J / d probe
+    double *Lx, *Ly, *Lp;  // d J / d log gX, d log gY, d log kP (cell layout)
+};
+
 // Second-order IIR section, a[0] = 1 (vs_audio.hip).
 struct AudioFilter {
     double b[3];
@@ -271,6 +292,16 @@ hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, i
 hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
                             FastBufs f, int t0, int nsteps, int threads, int guard, const int* d_t0,
                             hipStream_t st);
+// The on-chip step that also writes the forward factors of every step into
+// a.rec (slot n - t0 of the segment starting at t0).
+hipError_t launch_fast_record(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
+                              FastBufs f, FastAdj a, int t0, int nsteps, int threads, int guard, hipStream_t st);
+// Reverse steps s0 + ns - 1 ... s0 on chip (a.lanes threads per tube), then
+// the per-cell sums of the segment into a.Lx, a.Ly, a.Lp.
+hipError_t launch_fast_reverse(int K, const TubeDesc* d_tubes, const int* d_order, int n, FastBufs f,
+                               FastAdj a, int s0, int ns, int guard, hipStream_t st);
+hipError_t launch_fast_adj_sums(int K, const TubeDesc* d_tubes, const int* d_order, int n, FastAdj a, int s0,
+                                int ns, hipStream_t st);
 hipError_t launch_bump(int* d_t0, int s, hipStream_t st);
 hipError_t configure_fast(int K);
 template <typename Real>

@@ -131,11 +131,14 @@ __device__ inline uint32_t class_of(const uint32_t* cw, int k) {
     return (cw[k >> 2] >> (8 * (k & 3))) & 0xffu;
 }
 
-template <int K>
+// REC: also write the forward factors of every step (the values that
+// multiply gX, gY and kP) into rec, for the reverse pass.
+template <int K, bool REC>
 __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__ tubes,
                                                     const int* __restrict__ order, Bufs<float> b,
                                                     FastBufs f, int t0_arg, int nsteps,
-                                                    const int* __restrict__ d_t0, int G) {
+                                                    const int* __restrict__ d_t0, int G,
+                                                    float* __restrict__ frec, long long slab_tube) {
 #pragma clang fp contract(fast)
     constexpr int S = 2 * K + 1;                 // record stride (floats)
     extern __shared__ float lds[];
@@ -211,6 +214,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
     }
 
     // --------------------------- cell waves ---------------------------
+    const int KT = K * T;
     float Ux[K], kP[K], gX[K], gY[K];
     uint32_t cw[(K + 3) / 4];
     const int jb = t * K;
@@ -244,6 +248,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         asm volatile("" : "+v"(tl));
 #pragma unroll
         for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
+        // forward factors of this step: field f of slot k at dr[f * KT + k * T]
+        float* const dr = REC ? frec + blockIdx.x * slab_tube + (long long)(ts - t0) * 3 * KT + tl : nullptr;
         // ---------------- pressure phase ----------------
         {
             float* const rec = rec0 + tl * S;
@@ -258,6 +264,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 const float dv = (Ux[k] - left) + (uy - uyb);
                 const float aP = plut[class_of<K>(cw, k) & kClassMask];
                 rec[k] = aP * p - kP[k] * dv;
+                if (REC) dr[2 * KT + k * T] = dv;
             }
         }
         __syncthreads();
@@ -280,8 +287,13 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 // walls with admittance: U = -g P_neighbour; excitation: Ux = g exeV
                 const float ps = (cb & kClassSelfOff) ? 0.f : p;
                 const float prx = (cb & kClassSrcX) ? -ev : pn;
-                Ux[k] = axy.x * Ux[k] + gX[k] * (ps - prx);
-                rec[K + k] = axy.y * rec[K + k] + gY[k] * (ps - pa);
+                const float fx = ps - prx, fy = ps - pa;
+                Ux[k] = axy.x * Ux[k] + gX[k] * fx;
+                rec[K + k] = axy.y * rec[K + k] + gY[k] * fy;
+                if (REC) {
+                    dr[k * T] = fx;
+                    dr[KT + k * T] = fy;
+                }
             }
             rec[2 * K] = Ux[K - 1];
         }
@@ -311,13 +323,186 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// Reverse steps of the flux-form step above, one work-group of T = blockDim
+// cell lanes per tube, in the same on-chip layout.  Differentiating
+//   P'  = aP P - kP (Ux - Ux_left + Uy - Uy_below)
+//   Ux' = ax Ux + gX (s P' - r P'_right) + gX (1 - r) ev     s = !SelfOff, r = !SrcX
+//   Uy' = ay Uy + gY (s P' - P'_top)
+//   probe_n = P'_listener
+// and scaling the adjoints by their coefficients (Kp = kP M, Lx = gX lambda_x,
+// Ly = gY lambda_y, M the adjoint of P') gives, for reverse step n:
+//   Kp' = aP Kp + kP (g_n [listener] + s (Lx + Ly) - r_left Lx_left - Ly_below)
+//   Lx' = ax Lx + gX (Kp'_right - Kp')
+//   Ly' = ay Ly + gY (Kp'_top - Kp')
+// i.e. the forward stencil with Kp in the place of P.  LDS records hold
+// [ Kp[0..K) | Ly[0..K) | r Lx of the run's last cell ]; Lx lives in VGPRs.
+// Every step stores Lx, Ly (as they enter the step) and Kp' into a.adj.
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restrict__ tubes,
+                                                       const int* __restrict__ order, FastBufs f, FastAdj a,
+                                                       int s0, int ns, int G) {
+#pragma clang fp contract(fast)
+    constexpr int S = 2 * K + 1;
+    extern __shared__ float lds[];
+    __shared__ float2 vtab[64];
+    __shared__ float plut[64];
+    const int T = blockDim.x, t = threadIdx.x;
+    const int KT = K * T;
+    float* const rec0 = lds + G * S;
+
+    const int tube = order[blockIdx.x];
+    const TubeDesc& D = tubes[tube];
+    const int H = D.H, W = D.W, Hi = H - 2, Wi = W - 2;
+    const int Wp = Wi + 1;
+    const int qW = Wp / K, mW = Wp - (Wp / K) * K;
+    const int nbOff = qW * S + mW;
+    const int tend = min(s0 + ns, D.num_steps);
+    const long long co = D.cell_off;
+    // the listener's lane and slot
+    const int jL = (D.listener / W - 1) * Wp + (D.listener % W - 1);
+    const int lL = jL / K, kL = jL - (jL / K) * K;
+    const double* const gp = a.gprobe + D.src_off;
+
+    for (int i = t; i < 2 * G * S; i += T) lds[i < G * S ? i : i + T * S] = 0.f;
+    if (t < 64) {
+        plut[t] = f.ctab[t];
+        vtab[t] = make_float2(f.ctab[64 + t], f.ctab[128 + t]);
+    }
+    float LX[K], kP[K], gX[K], gY[K];
+    uint32_t cw[(K + 3) / 4];
+    float* const st = a.state + (long long)blockIdx.x * 3 * KT + t;
+    {
+        float* const rec = rec0 + t * S;
+        const int jb = t * K;
+        int r = jb / Wp, c = jb - (jb / Wp) * Wp;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            float kp = 0.f, gx = 0.f, gy = 0.f;
+            uint32_t cd = 0u;
+            if (r < Hi && c < Wi) {
+                const long long q = co + (long long)(r + 1) * W + (c + 1);
+                kp = f.kP[q]; gx = f.gX[q]; gy = f.gY[q]; cd = f.cls8[q];
+            }
+            kP[k] = kp; gX[k] = gx; gY[k] = gy;
+            if (k & 3) cw[k >> 2] |= cd << (8 * (k & 3)); else cw[k >> 2] = cd;
+            rec[k] = st[k * T];
+            rec[K + k] = st[KT + k * T];
+            LX[k] = st[2 * KT + k * T];
+            if (++c == Wp) { c = 0; ++r; }
+        }
+        rec[2 * K] = (class_of<K>(cw, K - 1) & kClassSrcX) ? 0.f : LX[K - 1];
+    }
+    __syncthreads();
+
+    for (int n = tend - 1; n >= s0; --n) {
+        int tl = t;
+        asm volatile("" : "+v"(tl));
+#pragma unroll
+        for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
+        float* const ad = a.adj + blockIdx.x * a.slab_tube + (long long)(n - s0) * 3 * KT + tl;
+        const float gl = tl == lL ? (float)gp[n] : 0.f;
+        // ---------------- pressure-like phase ----------------
+        {
+            float* const rec = rec0 + tl * S;
+            const float* const nb = rec + nbOff + K;      // Ly of the cell below
+            const float lx0 = rec[2 * K - S];             // r Lx of the previous lane's last cell
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t cb = class_of<K>(cw, k);
+                const float kold = rec[k], ly = rec[K + k], lx = LX[k];
+                ad[k * T] = lx;
+                ad[KT + k * T] = ly;
+                const float lyb = (k + mW >= K) ? nb[k + K + 1] : nb[k];
+                const float left = k > 0 ? ((class_of<K>(cw, k - 1) & kClassSrcX) ? 0.f : LX[k - 1]) : lx0;
+                float br = ((cb & kClassSelfOff) ? 0.f : lx + ly) - left - lyb;
+                if (k == kL) br += gl;
+                const float kn = plut[cb & kClassMask] * kold + kP[k] * br;
+                rec[k] = kn;
+                ad[2 * KT + k * T] = kn;
+            }
+        }
+        __syncthreads();
+        // ---------------- face phase ----------------
+        {
+            asm volatile("" : "+v"(tl));
+#pragma unroll
+            for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
+            float* const rec = rec0 + tl * S;
+            const float* const na = rec - nbOff;          // cell above
+            float kn = rec[0];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const float kc = kn;
+                kn = (k < K - 1) ? rec[k + 1] : rec[S];
+                const float ka = (k < mW) ? na[k - K - 1] : na[k];
+                const float2 axy = vtab[class_of<K>(cw, k) & kClassMask];
+                LX[k] = axy.x * LX[k] + gX[k] * (kn - kc);
+                rec[K + k] = axy.y * rec[K + k] + gY[k] * (ka - kc);
+            }
+            rec[2 * K] = (class_of<K>(cw, K - 1) & kClassSrcX) ? 0.f : LX[K - 1];
+        }
+        __syncthreads();
+    }
+    {
+        int tw = t;
+        asm volatile("" : "+v"(tw));
+        const float* const rec = rec0 + tw * S;
+        float* const sw = a.state + (long long)blockIdx.x * 3 * KT + tw;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            sw[k * T] = rec[k];
+            sw[KT + k * T] = rec[K + k];
+            sw[2 * KT + k * T] = LX[k];
+        }
+    }
+}
+
+// Per-cell sums over the steps of one segment of adj * rec (double), added to
+// the cell-layout log-derivatives: one thread per slot s = k * T + t.
+__global__ __launch_bounds__(256) void k_fast_adj_sums(const TubeDesc* __restrict__ tubes,
+                                                       const int* __restrict__ order, int K, FastAdj a, int s0,
+                                                       int ns) {
+    const int T = a.lanes, KT = K * T;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= KT) return;
+    const TubeDesc& D = tubes[order[blockIdx.y]];
+    const int k = s / T, t = s - (s / T) * T;
+    const int j = t * K + k;
+    const int Wp = D.W - 1, Wi = D.W - 2, Hi = D.H - 2;
+    const int r = j / Wp, c = j - (j / Wp) * Wp;
+    if (r >= Hi || c >= Wi) return;
+    const int tend = min(s0 + ns, D.num_steps);
+    const float* R = a.rec + blockIdx.y * a.slab_tube + s;
+    const float* A = a.adj + blockIdx.y * a.slab_tube + s;
+    double sx = 0.0, sy = 0.0, sp = 0.0;
+#pragma unroll 4
+    for (int n = s0; n < tend; ++n) {
+        const long long o = (long long)(n - s0) * 3 * KT;
+        sx += (double)A[o] * (double)R[o];
+        sy += (double)A[o + KT] * (double)R[o + KT];
+        sp += (double)A[o + 2 * KT] * (double)R[o + 2 * KT];
+    }
+    const long long q = D.cell_off + (long long)(r + 1) * D.W + (c + 1);
+    a.Lx[q] += sx;
+    a.Ly[q] += sy;
+    a.Lp[q] -= sp;
+}
+
 template <int K>
 hipError_t configure_k() {
     static bool configured = false;
     if (!configured) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_fast_step<K>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024 - (int)fast_lds_static_bytes());
+        const int dyn = 160 * 1024 - (int)fast_lds_static_bytes();
+        hipError_t e = hipFuncSetAttribute((const void*)k_fast_step<K, false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_fast_step<K, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    dyn);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_fast_reverse<K>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    dyn);
         if (e != hipSuccess) return e;
         configured = true;
     }
@@ -330,8 +515,29 @@ hipError_t launch_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<flo
     const size_t lds = fast_lds_bytes(K, threads, guard);
     hipError_t e = configure_k<K>();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_fast_step<K>, dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0, nsteps, d_t0,
-                       guard);
+    hipLaunchKernelGGL((k_fast_step<K, false>), dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0,
+                       nsteps, d_t0, guard, (float*)nullptr, 0LL);
+    return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_rec_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
+                        FastAdj a, int t0, int nsteps, int threads, int guard, hipStream_t st) {
+    const size_t lds = fast_lds_bytes(K, threads, guard);
+    hipError_t e = configure_k<K>();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_fast_step<K, true>), dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0,
+                       nsteps, (const int*)nullptr, guard, a.rec, a.slab_tube);
+    return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_rev_k(const TubeDesc* d_tubes, const int* d_order, int n, FastBufs f, FastAdj a, int s0,
+                        int ns, int guard, hipStream_t st) {
+    const size_t lds = fast_lds_bytes(K, a.lanes + 64, guard);
+    hipError_t e = configure_k<K>();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fast_reverse<K>, dim3(n), dim3(a.lanes), lds, st, d_tubes, d_order, f, a, s0, ns, guard);
     return hipGetLastError();
 }
 
@@ -383,6 +589,39 @@ hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, 
 #undef VS_K
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_fast_record(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
+                              FastBufs f, FastAdj a, int t0, int nsteps, int threads, int guard, hipStream_t st) {
+    if (threads - 64 != a.lanes) return hipErrorInvalidValue;
+    switch (K) {
+#define VS_K(k) case k: return launch_rec_k<k>(d_tubes, d_order, n, b, f, a, t0, nsteps, threads, guard, st);
+        VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
+        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(19) VS_K(20)
+        VS_K(21) VS_K(22) VS_K(23) VS_K(24)
+#undef VS_K
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fast_reverse(int K, const TubeDesc* d_tubes, const int* d_order, int n, FastBufs f,
+                               FastAdj a, int s0, int ns, int guard, hipStream_t st) {
+    switch (K) {
+#define VS_K(k) case k: return launch_rev_k<k>(d_tubes, d_order, n, f, a, s0, ns, guard, st);
+        VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
+        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(19) VS_K(20)
+        VS_K(21) VS_K(22) VS_K(23) VS_K(24)
+#undef VS_K
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fast_adj_sums(int K, const TubeDesc* d_tubes, const int* d_order, int n, FastAdj a, int s0,
+                                int ns, hipStream_t st) {
+    const int KT = K * a.lanes;
+    hipLaunchKernelGGL(k_fast_adj_sums, dim3((KT + 255) / 256, n), dim3(256), 0, st, d_tubes, d_order, K, a, s0,
+                       ns);
+    return hipGetLastError();
 }
 
 }  // namespace vs
