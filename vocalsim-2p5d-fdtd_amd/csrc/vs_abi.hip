@@ -327,6 +327,10 @@ struct vs_handle {
     // on-chip FP32 reverse pass: lanes shape, slabs and scratch
     int adj_K = 0, adj_G = 0, adj_B = 0, adj_fast_S = 0;
     FastAdj fa{};
+    float* adj_rec[2] = {nullptr, nullptr};   // double-buffered record slabs
+    float* adj_adj[2] = {nullptr, nullptr};
+    hipStream_t adj_rev_stream = nullptr, adj_sum_stream = nullptr;
+    hipEvent_t adj_ev[7] = {};               // init, rec x2, rev x2, sum x2
     float* adj_ck = nullptr;    // checkpoints: P, Ux, Uy (total_cells each) every S steps
     float* adj_sub = nullptr;   // block starts inside one segment when the record block B < S
     double* adj_strace = nullptr;
@@ -339,6 +343,10 @@ struct vs_handle {
         for (void* p : allocs) (void)hipFree(p);
         for (void* p : adj_slabs) (void)hipFree(p);
         for (void* p : adj_fast_slabs) (void)hipFree(p);
+        for (hipEvent_t e : adj_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (adj_rev_stream) (void)hipStreamDestroy(adj_rev_stream);
+        if (adj_sum_stream) (void)hipStreamDestroy(adj_sum_stream);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1378,8 +1386,9 @@ int adj_prepare(vs_handle* h, int S) {
 // kAdjLanes + 64 threads per tube in the forward, kAdjLanes in the reverse).
 constexpr int kAdjThreads = 1024;
 constexpr int kAdjLanes = kAdjThreads - 64;
-// Device memory allowed for the two per-step slabs of one record block.
-constexpr long long kAdjSlabBudget = 16LL << 30;
+// Device memory allowed for the record slabs (two buffers of two slabs):
+// half of the free memory, at most 64 GiB.
+constexpr long long kAdjSlabCap = 64LL << 30;
 
 bool adj_uses_onchip(const vs_handle* h) {
     return h->precision == VS_FP32 && h->adj_kernel == 0 && fast_fits(h, kAdjThreads, nullptr, nullptr);
@@ -1400,8 +1409,11 @@ int adj_fast_prepare(vs_handle* h, int S) {
     h->adj_fast_S = 0;
     h->adj_forward = false;
     const long long KT = (long long)K * kAdjLanes;
-    const long long per_step = 2LL * h->n * 3 * KT * (long long)sizeof(float);
-    int B = (int)std::max(1LL, std::min<long long>(S, kAdjSlabBudget / per_step));
+    const long long per_step = 4LL * h->n * 3 * KT * (long long)sizeof(float);
+    size_t mem_free = 0, mem_total = 0;
+    VS_HIP(hipMemGetInfo(&mem_free, &mem_total));
+    const long long budget = std::min<long long>(kAdjSlabCap, (long long)(mem_free / 2));
+    int B = (int)std::max(1LL, std::min<long long>(S, budget / per_step));
     if (h->adj_block > 0) B = std::min(B, h->adj_block);
     const long long C = h->total_cells;
     const long long nck = (h->max_steps + S - 1) / S;
@@ -1413,11 +1425,12 @@ int adj_fast_prepare(vs_handle* h, int S) {
         h->adj_fast_slabs.push_back(*p);
         return VS_OK;
     };
-    void *ck, *sub, *rec, *adj, *st, *tr, *pe, *tm;
+    void *ck, *sub, *rec0, *rec1, *adj0, *adj1, *st, *sk, *tr, *pe, *tm;
+    const long long slab = (long long)h->n * B * 3 * KT * 4;
     if ((rc = get(&ck, nck * 3 * C * 4, "checkpoints")) || (rc = get(&sub, nsub * 3 * C * 4, "block starts")) ||
-        (rc = get(&rec, (long long)h->n * B * 3 * KT * 4, "forward record slab")) ||
-        (rc = get(&adj, (long long)h->n * B * 3 * KT * 4, "adjoint record slab")) ||
-        (rc = get(&st, (long long)h->n * 3 * KT * 4, "adjoint state")) ||
+        (rc = get(&rec0, slab, "forward record slab")) || (rc = get(&rec1, slab, "forward record slab")) ||
+        (rc = get(&adj0, slab, "adjoint record slab")) || (rc = get(&adj1, slab, "adjoint record slab")) ||
+        (rc = get(&st, (long long)h->n * 3 * KT * 4, "adjoint state")) || (rc = get(&sk, 3 * KT * 4, "store sink")) ||
         (rc = get(&tr, h->total_src * 8, "source trace")) || (rc = get(&pe, h->n * 8LL, "excitation pressure")) ||
         (rc = get(&tm, h->n * (long long)sizeof(TwoMassState), "two-mass state")))
         return rc;
@@ -1426,10 +1439,19 @@ int adj_fast_prepare(vs_handle* h, int S) {
     h->adj_strace = static_cast<double*>(tr);
     h->adj_pexc = static_cast<double*>(pe);
     h->adj_tm = static_cast<TwoMassState*>(tm);
+    h->adj_rec[0] = static_cast<float*>(rec0);
+    h->adj_rec[1] = static_cast<float*>(rec1);
+    h->adj_adj[0] = static_cast<float*>(adj0);
+    h->adj_adj[1] = static_cast<float*>(adj1);
+    if (!h->adj_rev_stream) {
+        VS_HIP(hipStreamCreateWithFlags(&h->adj_rev_stream, hipStreamNonBlocking));
+        VS_HIP(hipStreamCreateWithFlags(&h->adj_sum_stream, hipStreamNonBlocking));
+        for (hipEvent_t& e : h->adj_ev) VS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     FastAdj& fa = h->fa;
-    fa.rec = static_cast<float*>(rec);
-    fa.adj = static_cast<float*>(adj);
+    fa.rec = fa.adj = nullptr;   // per block: one of the two buffers
     fa.state = static_cast<float*>(st);
+    fa.sink = static_cast<float*>(sk);
     fa.slab_tube = (long long)B * 3 * KT;
     fa.lanes = kAdjLanes;
     h->adj_K = K;
@@ -1511,9 +1533,12 @@ int adj_forward_fast(vs_handle* h, int S, double* probe) {
 
 // Reverse pass on chip.  Per segment (last first): the segment is recomputed
 // from its checkpoint in record blocks of B steps (block starts kept when
-// B < S); per block, last first: the recording forward launch, the reverse
-// launch, and the per-cell sums of the block.  Leaves d J / d log(gX, gY, kP)
-// in a.Gx, a.Gy, a.GkP.
+// B < S); per block, last first: the recording forward launch (this stream),
+// the reverse launch (its own stream: it reads only the adjoint state, so it
+// overlaps the recording) and the per-cell sums of the block (a third stream,
+// after both).  Record slabs alternate between two buffers; a buffer is
+// written again only after the sums of the block before last have read it.
+// Leaves d J / d log(gX, gY, kP) in a.Gx, a.Gy, a.GkP.
 int adj_backward_fast(vs_handle* h) {
     AdjBufs<float>& a = h->af;
     const int S = h->adj_fast_S, B = h->adj_B, K = h->adj_K, G = h->adj_G;
@@ -1525,9 +1550,14 @@ int adj_backward_fast(vs_handle* h) {
     fa.Lx = a.Gx;
     fa.Ly = a.Gy;
     fa.Lp = a.GkP;
-    VS_HIP(hipMemsetAsync(fa.state, 0, sizeof(float) * (size_t)h->n * 3 * K * kAdjLanes, h->stream));
+    hipStream_t sm = h->stream, sr = h->adj_rev_stream, ss = h->adj_sum_stream;
+    hipEvent_t* ev = h->adj_ev;   // 0 init, 1 + i rec, 3 + i rev, 5 + i sums (buffer i)
+    VS_HIP(hipMemsetAsync(fa.state, 0, sizeof(float) * (size_t)h->n * 3 * K * kAdjLanes, sm));
+    VS_HIP(hipEventRecord(ev[0], sm));
+    VS_HIP(hipStreamWaitEvent(sr, ev[0], 0));
+    VS_HIP(hipStreamWaitEvent(ss, ev[0], 0));
     const int nseg = (h->max_steps + S - 1) / S;
-    int rc;
+    int rc, blocks = 0;
     for (int k = nseg - 1; k >= 0; --k) {
         const int t0 = k * S, len = std::min(S, h->max_steps - t0);
         const int nb = (len + B - 1) / B;
@@ -1535,19 +1565,31 @@ int adj_backward_fast(vs_handle* h) {
         for (int j = 0; j + 1 < nb; ++j) {
             if ((rc = adj_state_copy(h, h->adj_sub + (long long)j * 3 * C, true)) != VS_OK) return rc;
             hipError_t e = launch_fast_step(K, h->d_desc, h->d_order, h->n, b, f, t0 + j * B, B, kAdjThreads, G,
-                                            nullptr, h->stream);
+                                            nullptr, sm);
             if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint block starts: ") + hipGetErrorString(e));
         }
-        for (int j = nb - 1; j >= 0; --j) {
+        for (int j = nb - 1; j >= 0; --j, ++blocks) {
             if (j + 1 < nb && (rc = adj_state_copy(h, h->adj_sub + (long long)j * 3 * C, false)) != VS_OK) return rc;
+            const int i = blocks & 1;
             const int s0 = t0 + j * B, ns = std::min(B, len - j * B);
-            hipError_t e = launch_fast_record(K, h->d_desc, h->d_order, h->n, b, f, fa, s0, ns, kAdjThreads, G,
-                                              h->stream);
-            if (e == hipSuccess) e = launch_fast_reverse(K, h->d_desc, h->d_order, h->n, f, fa, s0, ns, G, h->stream);
-            if (e == hipSuccess) e = launch_fast_adj_sums(K, h->d_desc, h->d_order, h->n, fa, s0, ns, h->stream);
+            fa.rec = h->adj_rec[i];
+            fa.adj = h->adj_adj[i];
+            if (blocks >= 2) VS_HIP(hipStreamWaitEvent(sm, ev[5 + i], 0));
+            hipError_t e = launch_fast_record(K, h->d_desc, h->d_order, h->n, b, f, fa, s0, ns, kAdjThreads, G, sm);
+            if (e == hipSuccess) e = hipEventRecord(ev[1 + i], sm);
+            if (e == hipSuccess && blocks >= 2) e = hipStreamWaitEvent(sr, ev[5 + i], 0);
+            if (e == hipSuccess) e = launch_fast_reverse(K, h->d_desc, h->d_order, h->n, f, fa, s0, ns, G, sr);
+            if (e == hipSuccess) e = hipEventRecord(ev[3 + i], sr);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ss, ev[1 + i], 0);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ss, ev[3 + i], 0);
+            if (e == hipSuccess) e = launch_fast_adj_sums(K, h->d_desc, h->d_order, h->n, fa, s0, ns, ss);
+            if (e == hipSuccess) e = hipEventRecord(ev[5 + i], ss);
             if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint reverse: ") + hipGetErrorString(e));
         }
     }
+    // the sums stream has waited for every reverse launch
+    VS_HIP(hipEventRecord(ev[0], ss));
+    VS_HIP(hipStreamWaitEvent(sm, ev[0], 0));
     return VS_OK;
 }
 

@@ -256,6 +256,7 @@ struct FastAdj {
     float* rec;
     float* adj;
     float* state;          // per tube 3 * K * T: Kp, Ly, Lx between launches
+    float* sink;           // 3 * K * T: stores of waves past every cell of their tube
     long long slab_tube;   // floats per tube of rec / adj (S * 3 * K * T)
     int lanes;             // T: cell lanes per work-group
     const double* gprobe;  // per step: d J / d probe

@@ -131,6 +131,13 @@ __device__ inline uint32_t class_of(const uint32_t* cw, int k) {
     return (cw[k >> 2] >> (8 * (k & 3))) & 0xffu;
 }
 
+// Slab stores: a work-group-uniform base plus a per-lane 32-bit byte offset
+// (one VGPR, advanced by T floats per slot), so each store is a single
+// global store with a scalar base instead of a 64-bit address per slot.
+__device__ inline void slab_store(char* base, uint32_t off, float v) {
+    *reinterpret_cast<float*>(base + off) = v;
+}
+
 // REC: also write the forward factors of every step (the values that
 // multiply gX, gY and kP) into rec, for the reverse pass.
 template <int K, bool REC>
@@ -138,7 +145,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                                                     const int* __restrict__ order, Bufs<float> b,
                                                     FastBufs f, int t0_arg, int nsteps,
                                                     const int* __restrict__ d_t0, int G,
-                                                    float* __restrict__ frec, long long slab_tube) {
+                                                    float* __restrict__ frec, long long slab_tube,
+                                                    float* __restrict__ fsink) {
 #pragma clang fp contract(fast)
     constexpr int S = 2 * K + 1;                 // record stride (floats)
     extern __shared__ float lds[];
@@ -248,8 +256,15 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         asm volatile("" : "+v"(tl));
 #pragma unroll
         for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
-        // forward factors of this step: field f of slot k at dr[f * KT + k * T]
-        float* const dr = REC ? frec + blockIdx.x * slab_tube + (long long)(ts - t0) * 3 * KT + tl : nullptr;
+        // forward factors of this step: field f of slot k at float f * KT + k * T + lane;
+        // waves wholly past the tube's cells write a small shared sink instead
+        // (it stays in L2: the sums never read those slots)
+        char* const dr = !REC ? nullptr
+                         : __builtin_amdgcn_readfirstlane(tl) * K < Hi * Wp
+                             ? reinterpret_cast<char*>(frec + blockIdx.x * slab_tube + (long long)(ts - t0) * 3 * KT)
+                             : reinterpret_cast<char*>(fsink);
+        const uint32_t T4 = 4u * (uint32_t)T, KT4 = 4u * (uint32_t)KT;
+        uint32_t vo = 4u * (uint32_t)tl;
         // ---------------- pressure phase ----------------
         {
             float* const rec = rec0 + tl * S;
@@ -264,7 +279,11 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 const float dv = (Ux[k] - left) + (uy - uyb);
                 const float aP = plut[class_of<K>(cw, k) & kClassMask];
                 rec[k] = aP * p - kP[k] * dv;
-                if (REC) dr[2 * KT + k * T] = dv;
+                if (REC) {
+                    slab_store(dr + 2 * KT4, vo, dv);
+                    vo += T4;
+                    asm volatile("" : "+v"(vo));
+                }
             }
         }
         __syncthreads();
@@ -277,6 +296,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
             float* const rec = rec0 + tl * S;
             const float* const na = rec - nbOff;          // cell above, slot k - m of lane l - q
             float pn = rec[0];
+            vo = 4u * (uint32_t)tl;
+            if (REC) asm volatile("" : "+v"(vo));
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const float p = pn;
@@ -291,8 +312,10 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 Ux[k] = axy.x * Ux[k] + gX[k] * fx;
                 rec[K + k] = axy.y * rec[K + k] + gY[k] * fy;
                 if (REC) {
-                    dr[k * T] = fx;
-                    dr[KT + k * T] = fy;
+                    slab_store(dr, vo, fx);
+                    slab_store(dr + KT4, vo, fy);
+                    vo += T4;
+                    asm volatile("" : "+v"(vo));
                 }
             }
             rec[2 * K] = Ux[K - 1];
@@ -401,7 +424,12 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
         asm volatile("" : "+v"(tl));
 #pragma unroll
         for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
-        float* const ad = a.adj + blockIdx.x * a.slab_tube + (long long)(n - s0) * 3 * KT + tl;
+        // waves wholly past the tube's cells write the shared sink
+        char* const ad = __builtin_amdgcn_readfirstlane(tl) * K < Hi * Wp
+                             ? reinterpret_cast<char*>(a.adj + blockIdx.x * a.slab_tube + (long long)(n - s0) * 3 * KT)
+                             : reinterpret_cast<char*>(a.sink);
+        const uint32_t T4 = 4u * (uint32_t)T, KT4 = 4u * (uint32_t)KT;
+        uint32_t vo = 4u * (uint32_t)tl;
         const float gl = tl == lL ? (float)gp[n] : 0.f;
         // ---------------- pressure-like phase ----------------
         {
@@ -412,15 +440,17 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
             for (int k = 0; k < K; ++k) {
                 const uint32_t cb = class_of<K>(cw, k);
                 const float kold = rec[k], ly = rec[K + k], lx = LX[k];
-                ad[k * T] = lx;
-                ad[KT + k * T] = ly;
+                slab_store(ad, vo, lx);
+                slab_store(ad + KT4, vo, ly);
                 const float lyb = (k + mW >= K) ? nb[k + K + 1] : nb[k];
                 const float left = k > 0 ? ((class_of<K>(cw, k - 1) & kClassSrcX) ? 0.f : LX[k - 1]) : lx0;
                 float br = ((cb & kClassSelfOff) ? 0.f : lx + ly) - left - lyb;
                 if (k == kL) br += gl;
                 const float kn = plut[cb & kClassMask] * kold + kP[k] * br;
                 rec[k] = kn;
-                ad[2 * KT + k * T] = kn;
+                slab_store(ad + 2 * KT4, vo, kn);
+                vo += T4;
+                asm volatile("" : "+v"(vo));
             }
         }
         __syncthreads();
@@ -516,7 +546,7 @@ hipError_t launch_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<flo
     hipError_t e = configure_k<K>();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_fast_step<K, false>), dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0,
-                       nsteps, d_t0, guard, (float*)nullptr, 0LL);
+                       nsteps, d_t0, guard, (float*)nullptr, 0LL, (float*)nullptr);
     return hipGetLastError();
 }
 
@@ -527,7 +557,7 @@ hipError_t launch_rec_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs
     hipError_t e = configure_k<K>();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_fast_step<K, true>), dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0,
-                       nsteps, (const int*)nullptr, guard, a.rec, a.slab_tube);
+                       nsteps, (const int*)nullptr, guard, a.rec, a.slab_tube, a.sink);
     return hipGetLastError();
 }
 
