@@ -87,21 +87,28 @@ def adjoint_leg(args, np, torch, vocalsim, tubes, _abi):
         target = log_magnitude(torch.tensor(np.stack([s.probe(i) for i in range(B)]), device="cuda"), basis)
         cells = s.cells_per_step
     dev = "cuda"
-    A = torch.tensor(np.stack([sp.areas_cm2 for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
-    D = torch.tensor(np.stack([sp.depth_scale for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
-    M = torch.tensor(np.stack([sp.mu3d for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    p = tube_probe(A, D, M, specs, 32, 512)
-    loss = ((log_magnitude(p, basis) - target) ** 2).mean()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    loss.backward()
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
+
+    def one_pass():
+        A = torch.tensor(np.stack([sp.areas_cm2 for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
+        D = torch.tensor(np.stack([sp.depth_scale for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
+        M = torch.tensor(np.stack([sp.mu3d for sp in specs]), dtype=torch.float64, device=dev).requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p = tube_probe(A, D, M, specs, 32, 512)
+        loss = ((log_magnitude(p, basis) - target) ** 2).mean()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return A, loss, t0, t1, t2
+
+    one_pass()   # untimed: first-call costs (kernel loading, library GEMM selection)
+    A, loss, t0, t1, t2 = one_pass()
     out = {"workload": f"{B} seeded 44-section vowel tubes, Gaussian pulse, {n} steps; loss = mean squared "
                        f"ln|Y(f)| difference over 1024 bins 0-5 kHz against {B} other seeds",
            "precision": "f32 state, f64 gradient accumulation", "checkpoint_every": 512,
+           "timing": "second of two identical passes",
            "cells_per_step": cells, "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
            "forward_cell_updates_per_s": cells * n / (t1 - t0),
            "gradient_cell_updates_per_s": cells * n / (t2 - t0),
@@ -118,7 +125,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(tubes would shard by batch; see DESIGN.md).", file=sys.stderr)
+              "(tubes would shard by batch).", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

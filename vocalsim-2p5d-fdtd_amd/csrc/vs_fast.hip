@@ -138,6 +138,73 @@ __device__ inline void slab_store(char* base, uint32_t off, float v) {
     *reinterpret_cast<float*>(base + off) = v;
 }
 
+// Advances a slab offset through an opaque (non-volatile) asm, so the
+// compiler keeps one VGPR instead of K precomputed scalar offsets, without a
+// scheduling barrier.
+__device__ inline uint32_t next_off(uint32_t vo, uint32_t step) {
+    vo += step;
+    asm("" : "+v"(vo));
+    return vo;
+}
+
+// The two phases of one step over a lane's run of K cells.  The pointers are
+// __restrict__: in the pressure-like phase only the lane's own P-region is
+// written (Uy of this lane and of the lane below are only read), in the face
+// phase only the Uy-region (P of this lane, the next and the one above are
+// only read), so the compiler may issue the LDS reads of later cells ahead of
+// the writes of earlier ones.
+template <int K, bool REC>
+__device__ __forceinline__ void fast_pressure(float* __restrict__ P, const float* __restrict__ UY,
+                                              const float* __restrict__ UYB, const float* __restrict__ plut,
+                                              const float (&Ux)[K], const float (&kP)[K], const uint32_t (&cw)[(K + 3) / 4],
+                                              float uxl0, int mW, char* dr, uint32_t vo, uint32_t T4, uint32_t KT4) {
+#pragma clang fp contract(fast)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const float p = P[k];
+        const float uy = UY[k];
+        const float uyb = (k + mW >= K) ? UYB[k + K + 1] : UYB[k];
+        const float left = k > 0 ? Ux[k - 1] : uxl0;
+        const float dv = (Ux[k] - left) + (uy - uyb);
+        const float aP = plut[class_of<K>(cw, k) & kClassMask];
+        P[k] = aP * p - kP[k] * dv;
+        if (REC) {
+            slab_store(dr + 2 * KT4, vo, dv);
+            vo = next_off(vo, T4);
+        }
+    }
+}
+
+template <int K, bool REC>
+__device__ __forceinline__ void fast_faces(const float* __restrict__ P, const float* __restrict__ PA,
+                                           float* __restrict__ UY, const float2* __restrict__ vtab, float (&Ux)[K],
+                                           const float (&gX)[K], const float (&gY)[K],
+                                           const uint32_t (&cw)[(K + 3) / 4], float ev, int mW, char* dr,
+                                           uint32_t vo, uint32_t T4, uint32_t KT4) {
+#pragma clang fp contract(fast)
+    float pn = P[0];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const float p = pn;
+        pn = (k < K - 1) ? P[k + 1] : P[2 * K + 1];   // next lane's first cell
+        const float pa = (k < mW) ? PA[k - K - 1] : PA[k];
+        const uint32_t cb = class_of<K>(cw, k);
+        const float2 axy = vtab[cb & kClassMask];
+        // walls with admittance: U = -g P_neighbour; excitation: Ux = g exeV
+        const float ps = (cb & kClassSelfOff) ? 0.f : p;
+        const float prx = (cb & kClassSrcX) ? -ev : pn;
+        const float fx = ps - prx, fy = ps - pa;
+        Ux[k] = axy.x * Ux[k] + gX[k] * fx;
+        UY[k] = axy.y * UY[k] + gY[k] * fy;
+        if (REC) {
+            slab_store(dr, vo, fx);
+            slab_store(dr + KT4, vo, fy);
+            vo = next_off(vo, T4);
+        }
+    }
+    UY[K] = Ux[K - 1];
+}
+
 // REC: also write the forward factors of every step (the values that
 // multiply gX, gY and kP) into rec, for the reverse pass.
 template <int K, bool REC>
@@ -268,23 +335,9 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         // ---------------- pressure phase ----------------
         {
             float* const rec = rec0 + tl * S;
-            const float* const nb = rec + nbOff + K;      // Uy of the cell below: slot k + m of lane l + q
-            const float uxl0 = rec[2 * K - S];            // run end of lane l - 1
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const float p = rec[k];
-                const float uy = rec[K + k];
-                const float uyb = (k + mW >= K) ? nb[k + K + 1] : nb[k];
-                const float left = k > 0 ? Ux[k - 1] : uxl0;
-                const float dv = (Ux[k] - left) + (uy - uyb);
-                const float aP = plut[class_of<K>(cw, k) & kClassMask];
-                rec[k] = aP * p - kP[k] * dv;
-                if (REC) {
-                    slab_store(dr + 2 * KT4, vo, dv);
-                    vo += T4;
-                    asm volatile("" : "+v"(vo));
-                }
-            }
+            // Uy of the cell below: slot k + m of lane l + q; run end of lane l - 1
+            fast_pressure<K, REC>(rec, rec + K, rec + nbOff + K, plut, Ux, kP, cw, rec[2 * K - S], mW, dr, vo, T4,
+                                  KT4);
         }
         __syncthreads();
         const float ev = (float)sexe[ts & 1];
@@ -294,31 +347,10 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
 #pragma unroll
             for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
             float* const rec = rec0 + tl * S;
-            const float* const na = rec - nbOff;          // cell above, slot k - m of lane l - q
-            float pn = rec[0];
             vo = 4u * (uint32_t)tl;
-            if (REC) asm volatile("" : "+v"(vo));
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const float p = pn;
-                pn = (k < K - 1) ? rec[k + 1] : rec[S];   // next lane's first cell
-                const float pa = (k < mW) ? na[k - K - 1] : na[k];
-                const uint32_t cb = class_of<K>(cw, k);
-                const float2 axy = vtab[cb & kClassMask];
-                // walls with admittance: U = -g P_neighbour; excitation: Ux = g exeV
-                const float ps = (cb & kClassSelfOff) ? 0.f : p;
-                const float prx = (cb & kClassSrcX) ? -ev : pn;
-                const float fx = ps - prx, fy = ps - pa;
-                Ux[k] = axy.x * Ux[k] + gX[k] * fx;
-                rec[K + k] = axy.y * rec[K + k] + gY[k] * fy;
-                if (REC) {
-                    slab_store(dr, vo, fx);
-                    slab_store(dr + KT4, vo, fy);
-                    vo += T4;
-                    asm volatile("" : "+v"(vo));
-                }
-            }
-            rec[2 * K] = Ux[K - 1];
+            if (REC) asm("" : "+v"(vo));
+            // cell above: slot k - m of lane l - q
+            fast_faces<K, REC>(rec, rec - nbOff, rec + K, vtab, Ux, gX, gY, cw, ev, mW, dr, vo, T4, KT4);
         }
         __syncthreads();
     }
@@ -344,6 +376,52 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
             if (++c == Wp) { c = 0; ++r; }
         }
     }
+}
+
+// The two phases of one reverse step over a lane's run (same aliasing
+// structure as fast_pressure / fast_faces: Kp-region written in the first,
+// Ly-region in the second).
+template <int K>
+__device__ __forceinline__ void rev_pressure(float* __restrict__ KP, const float* __restrict__ LY,
+                                             const float* __restrict__ LYB, const float* __restrict__ plut,
+                                             const float (&LX)[K], const float (&kP)[K],
+                                             const uint32_t (&cw)[(K + 3) / 4], float lx0, int mW, int kL, float gl,
+                                             char* ad, uint32_t vo, uint32_t T4, uint32_t KT4) {
+#pragma clang fp contract(fast)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t cb = class_of<K>(cw, k);
+        const float kold = KP[k], ly = LY[k], lx = LX[k];
+        slab_store(ad, vo, lx);
+        slab_store(ad + KT4, vo, ly);
+        const float lyb = (k + mW >= K) ? LYB[k + K + 1] : LYB[k];
+        const float left = k > 0 ? ((class_of<K>(cw, k - 1) & kClassSrcX) ? 0.f : LX[k - 1]) : lx0;
+        float br = ((cb & kClassSelfOff) ? 0.f : lx + ly) - left - lyb;
+        if (k == kL) br += gl;
+        const float kn = plut[cb & kClassMask] * kold + kP[k] * br;
+        KP[k] = kn;
+        slab_store(ad + 2 * KT4, vo, kn);
+        vo = next_off(vo, T4);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void rev_faces(const float* __restrict__ KP, const float* __restrict__ KA,
+                                          float* __restrict__ LY, const float2* __restrict__ vtab, float (&LX)[K],
+                                          const float (&gX)[K], const float (&gY)[K],
+                                          const uint32_t (&cw)[(K + 3) / 4], int mW) {
+#pragma clang fp contract(fast)
+    float kn = KP[0];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const float kc = kn;
+        kn = (k < K - 1) ? KP[k + 1] : KP[2 * K + 1];
+        const float ka = (k < mW) ? KA[k - K - 1] : KA[k];
+        const float2 axy = vtab[class_of<K>(cw, k) & kClassMask];
+        LX[k] = axy.x * LX[k] + gX[k] * (kn - kc);
+        LY[k] = axy.y * LY[k] + gY[k] * (ka - kc);
+    }
+    LY[K] = (class_of<K>(cw, K - 1) & kClassSrcX) ? 0.f : LX[K - 1];
 }
 
 // ---------------------------------------------------------------------------
@@ -434,24 +512,9 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
         // ---------------- pressure-like phase ----------------
         {
             float* const rec = rec0 + tl * S;
-            const float* const nb = rec + nbOff + K;      // Ly of the cell below
-            const float lx0 = rec[2 * K - S];             // r Lx of the previous lane's last cell
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const uint32_t cb = class_of<K>(cw, k);
-                const float kold = rec[k], ly = rec[K + k], lx = LX[k];
-                slab_store(ad, vo, lx);
-                slab_store(ad + KT4, vo, ly);
-                const float lyb = (k + mW >= K) ? nb[k + K + 1] : nb[k];
-                const float left = k > 0 ? ((class_of<K>(cw, k - 1) & kClassSrcX) ? 0.f : LX[k - 1]) : lx0;
-                float br = ((cb & kClassSelfOff) ? 0.f : lx + ly) - left - lyb;
-                if (k == kL) br += gl;
-                const float kn = plut[cb & kClassMask] * kold + kP[k] * br;
-                rec[k] = kn;
-                slab_store(ad + 2 * KT4, vo, kn);
-                vo += T4;
-                asm volatile("" : "+v"(vo));
-            }
+            // Ly of the cell below; r Lx of the previous lane's last cell
+            rev_pressure<K>(rec, rec + K, rec + nbOff + K, plut, LX, kP, cw, rec[2 * K - S], mW, kL, gl, ad, vo, T4,
+                            KT4);
         }
         __syncthreads();
         // ---------------- face phase ----------------
@@ -460,18 +523,7 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
 #pragma unroll
             for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
             float* const rec = rec0 + tl * S;
-            const float* const na = rec - nbOff;          // cell above
-            float kn = rec[0];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const float kc = kn;
-                kn = (k < K - 1) ? rec[k + 1] : rec[S];
-                const float ka = (k < mW) ? na[k - K - 1] : na[k];
-                const float2 axy = vtab[class_of<K>(cw, k) & kClassMask];
-                LX[k] = axy.x * LX[k] + gX[k] * (kn - kc);
-                rec[K + k] = axy.y * rec[K + k] + gY[k] * (ka - kc);
-            }
-            rec[2 * K] = (class_of<K>(cw, K - 1) & kClassSrcX) ? 0.f : LX[K - 1];
+            rev_faces<K>(rec, rec - nbOff, rec + K, vtab, LX, gX, gY, cw, mW);   // cell above: rec - nbOff
         }
         __syncthreads();
     }
