@@ -325,11 +325,12 @@ struct vs_handle {
     int adj_block = 0;         // VS_OPT_ADJOINT_BLOCK
     bool adj_onchip = false;   // the held forward pass ran on chip
     // on-chip FP32 reverse pass: lanes shape, slabs and scratch
-    int adj_K = 0, adj_G = 0, adj_B = 0, adj_fast_S = 0;
+    int adj_K = 0, adj_G = 0, adj_B = 0, adj_fast_S = 0, adj_sum_groups = 0;
     FastAdj fa{};
     float* adj_rec[2] = {nullptr, nullptr};   // double-buffered record slabs
     float* adj_adj[2] = {nullptr, nullptr};
-    hipStream_t adj_rev_stream = nullptr, adj_sum_stream = nullptr;
+    float* adj_start[2] = {nullptr, nullptr};
+    hipStream_t adj_rec_stream = nullptr, adj_rev_stream = nullptr, adj_sum_stream = nullptr;
     hipEvent_t adj_ev[7] = {};               // init, rec x2, rev x2, sum x2
     float* adj_ck = nullptr;    // checkpoints: P, Ux, Uy (total_cells each) every S steps
     float* adj_sub = nullptr;   // block starts inside one segment when the record block B < S
@@ -345,6 +346,7 @@ struct vs_handle {
         for (void* p : adj_fast_slabs) (void)hipFree(p);
         for (hipEvent_t e : adj_ev)
             if (e) (void)hipEventDestroy(e);
+        if (adj_rec_stream) (void)hipStreamDestroy(adj_rec_stream);
         if (adj_rev_stream) (void)hipStreamDestroy(adj_rev_stream);
         if (adj_sum_stream) (void)hipStreamDestroy(adj_sum_stream);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -1409,7 +1411,7 @@ int adj_fast_prepare(vs_handle* h, int S) {
     h->adj_fast_S = 0;
     h->adj_forward = false;
     const long long KT = (long long)K * kAdjLanes;
-    const long long per_step = 4LL * h->n * 3 * KT * (long long)sizeof(float);
+    const long long per_step = 2LL * h->n * 3 * KT * (long long)sizeof(float);   // two buffers of rec (2) + adj (1)
     size_t mem_free = 0, mem_total = 0;
     VS_HIP(hipMemGetInfo(&mem_free, &mem_total));
     const long long budget = std::min<long long>(kAdjSlabCap, (long long)(mem_free / 2));
@@ -1425,11 +1427,13 @@ int adj_fast_prepare(vs_handle* h, int S) {
         h->adj_fast_slabs.push_back(*p);
         return VS_OK;
     };
-    void *ck, *sub, *rec0, *rec1, *adj0, *adj1, *st, *sk, *tr, *pe, *tm;
-    const long long slab = (long long)h->n * B * 3 * KT * 4;
+    void *ck, *sub, *rec0, *rec1, *adj0, *adj1, *sa0, *sa1, *st, *sk, *tr, *pe, *tm;
+    const long long slab = (long long)h->n * B * KT * 4;   // one field
     if ((rc = get(&ck, nck * 3 * C * 4, "checkpoints")) || (rc = get(&sub, nsub * 3 * C * 4, "block starts")) ||
-        (rc = get(&rec0, slab, "forward record slab")) || (rc = get(&rec1, slab, "forward record slab")) ||
+        (rc = get(&rec0, 2 * slab, "forward record slab")) || (rc = get(&rec1, 2 * slab, "forward record slab")) ||
         (rc = get(&adj0, slab, "adjoint record slab")) || (rc = get(&adj1, slab, "adjoint record slab")) ||
+        (rc = get(&sa0, (long long)h->n * 2 * KT * 4, "block-start adjoints")) ||
+        (rc = get(&sa1, (long long)h->n * 2 * KT * 4, "block-start adjoints")) ||
         (rc = get(&st, (long long)h->n * 3 * KT * 4, "adjoint state")) || (rc = get(&sk, 3 * KT * 4, "store sink")) ||
         (rc = get(&tr, h->total_src * 8, "source trace")) || (rc = get(&pe, h->n * 8LL, "excitation pressure")) ||
         (rc = get(&tm, h->n * (long long)sizeof(TwoMassState), "two-mass state")))
@@ -1443,16 +1447,30 @@ int adj_fast_prepare(vs_handle* h, int S) {
     h->adj_rec[1] = static_cast<float*>(rec1);
     h->adj_adj[0] = static_cast<float*>(adj0);
     h->adj_adj[1] = static_cast<float*>(adj1);
+    h->adj_start[0] = static_cast<float*>(sa0);
+    h->adj_start[1] = static_cast<float*>(sa1);
     if (!h->adj_rev_stream) {
+        VS_HIP(hipStreamCreateWithFlags(&h->adj_rec_stream, hipStreamNonBlocking));
         VS_HIP(hipStreamCreateWithFlags(&h->adj_rev_stream, hipStreamNonBlocking));
         VS_HIP(hipStreamCreateWithFlags(&h->adj_sum_stream, hipStreamNonBlocking));
         for (hipEvent_t& e : h->adj_ev) VS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
+    // Recording and reverse launches hold one CU per tube for the whole
+    // launch.  The sums run as a fixed grid of 1024-thread work-groups (two
+    // fill a CU), two per CU the other two leave free, so that they cannot
+    // take the CUs the next recording launch is waiting for.
+    {
+        int dev = 0, ncu = 0;
+        VS_HIP(hipGetDevice(&dev));
+        VS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        h->adj_sum_groups = 2 * std::max(16, ncu - 2 * h->n);
+    }
     FastAdj& fa = h->fa;
-    fa.rec = fa.adj = nullptr;   // per block: one of the two buffers
+    fa.rec = fa.adj = fa.start = nullptr;   // per block: one of the two buffers
     fa.state = static_cast<float*>(st);
     fa.sink = static_cast<float*>(sk);
-    fa.slab_tube = (long long)B * 3 * KT;
+    fa.rec_tube = (long long)B * 2 * KT;
+    fa.adj_tube = (long long)B * KT;
     fa.lanes = kAdjLanes;
     h->adj_K = K;
     h->adj_G = G;
@@ -1483,15 +1501,15 @@ FastBufs adj_fast_flux(vs_handle* h) {
 }
 
 // Copies the forward state (P, Ux, Uy) to / from a slab of 3 * total_cells.
-int adj_state_copy(vs_handle* h, float* slab, bool to_slab) {
+int adj_state_copy(vs_handle* h, float* slab, bool to_slab, hipStream_t st) {
     AdjBufs<float>& a = h->af;
     const size_t cb = sizeof(float) * (size_t)h->total_cells;
     const long long C = h->total_cells;
-    float* st[3] = {a.P, a.Ux, a.Uy};
+    float* state[3] = {a.P, a.Ux, a.Uy};
     for (int i = 0; i < 3; ++i) {
-        float* dst = to_slab ? slab + i * C : st[i];
-        const float* src = to_slab ? st[i] : slab + i * C;
-        VS_HIP(hipMemcpyAsync(dst, src, cb, hipMemcpyDeviceToDevice, h->stream));
+        float* dst = to_slab ? slab + i * C : state[i];
+        const float* src = to_slab ? state[i] : slab + i * C;
+        VS_HIP(hipMemcpyAsync(dst, src, cb, hipMemcpyDeviceToDevice, st));
     }
     return VS_OK;
 }
@@ -1519,7 +1537,8 @@ int adj_forward_fast(vs_handle* h, int S, double* probe) {
     const Bufs<float> b = adj_fast_bufs(h);
     const FastBufs f = adj_fast_flux(h);
     for (int t0 = 0; t0 < h->max_steps; t0 += S) {
-        if ((rc = adj_state_copy(h, h->adj_ck + (long long)(t0 / S) * 3 * h->total_cells, true)) != VS_OK) return rc;
+        if ((rc = adj_state_copy(h, h->adj_ck + (long long)(t0 / S) * 3 * h->total_cells, true, h->stream)) != VS_OK)
+            return rc;
         hipError_t e = launch_fast_step(h->adj_K, h->d_desc, h->d_order, h->n, b, f, t0, S, kAdjThreads, h->adj_G,
                                         nullptr, h->stream);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint forward: ") + hipGetErrorString(e));
@@ -1550,10 +1569,11 @@ int adj_backward_fast(vs_handle* h) {
     fa.Lx = a.Gx;
     fa.Ly = a.Gy;
     fa.Lp = a.GkP;
-    hipStream_t sm = h->stream, sr = h->adj_rev_stream, ss = h->adj_sum_stream;
+    hipStream_t s_main = h->stream, sm = h->adj_rec_stream, sr = h->adj_rev_stream, ss = h->adj_sum_stream;
     hipEvent_t* ev = h->adj_ev;   // 0 init, 1 + i rec, 3 + i rev, 5 + i sums (buffer i)
-    VS_HIP(hipMemsetAsync(fa.state, 0, sizeof(float) * (size_t)h->n * 3 * K * kAdjLanes, sm));
-    VS_HIP(hipEventRecord(ev[0], sm));
+    VS_HIP(hipMemsetAsync(fa.state, 0, sizeof(float) * (size_t)h->n * 3 * K * kAdjLanes, s_main));
+    VS_HIP(hipEventRecord(ev[0], s_main));
+    VS_HIP(hipStreamWaitEvent(sm, ev[0], 0));
     VS_HIP(hipStreamWaitEvent(sr, ev[0], 0));
     VS_HIP(hipStreamWaitEvent(ss, ev[0], 0));
     const int nseg = (h->max_steps + S - 1) / S;
@@ -1561,19 +1581,21 @@ int adj_backward_fast(vs_handle* h) {
     for (int k = nseg - 1; k >= 0; --k) {
         const int t0 = k * S, len = std::min(S, h->max_steps - t0);
         const int nb = (len + B - 1) / B;
-        if ((rc = adj_state_copy(h, h->adj_ck + (long long)k * 3 * C, false)) != VS_OK) return rc;
+        if ((rc = adj_state_copy(h, h->adj_ck + (long long)k * 3 * C, false, sm)) != VS_OK) return rc;
         for (int j = 0; j + 1 < nb; ++j) {
-            if ((rc = adj_state_copy(h, h->adj_sub + (long long)j * 3 * C, true)) != VS_OK) return rc;
+            if ((rc = adj_state_copy(h, h->adj_sub + (long long)j * 3 * C, true, sm)) != VS_OK) return rc;
             hipError_t e = launch_fast_step(K, h->d_desc, h->d_order, h->n, b, f, t0 + j * B, B, kAdjThreads, G,
                                             nullptr, sm);
             if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint block starts: ") + hipGetErrorString(e));
         }
         for (int j = nb - 1; j >= 0; --j, ++blocks) {
-            if (j + 1 < nb && (rc = adj_state_copy(h, h->adj_sub + (long long)j * 3 * C, false)) != VS_OK) return rc;
+            if (j + 1 < nb && (rc = adj_state_copy(h, h->adj_sub + (long long)j * 3 * C, false, sm)) != VS_OK)
+                return rc;
             const int i = blocks & 1;
             const int s0 = t0 + j * B, ns = std::min(B, len - j * B);
             fa.rec = h->adj_rec[i];
             fa.adj = h->adj_adj[i];
+            fa.start = h->adj_start[i];
             if (blocks >= 2) VS_HIP(hipStreamWaitEvent(sm, ev[5 + i], 0));
             hipError_t e = launch_fast_record(K, h->d_desc, h->d_order, h->n, b, f, fa, s0, ns, kAdjThreads, G, sm);
             if (e == hipSuccess) e = hipEventRecord(ev[1 + i], sm);
@@ -1582,14 +1604,16 @@ int adj_backward_fast(vs_handle* h) {
             if (e == hipSuccess) e = hipEventRecord(ev[3 + i], sr);
             if (e == hipSuccess) e = hipStreamWaitEvent(ss, ev[1 + i], 0);
             if (e == hipSuccess) e = hipStreamWaitEvent(ss, ev[3 + i], 0);
-            if (e == hipSuccess) e = launch_fast_adj_sums(K, h->d_desc, h->d_order, h->n, fa, s0, ns, ss);
+            if (e == hipSuccess)
+                e = launch_fast_adj_sums(K, h->d_desc, h->d_order, h->n, f, fa, h->bf.source, s0, ns,
+                                         h->adj_sum_groups, ss);
             if (e == hipSuccess) e = hipEventRecord(ev[5 + i], ss);
             if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint reverse: ") + hipGetErrorString(e));
         }
     }
-    // the sums stream has waited for every reverse launch
+    // the sums stream has waited for every recording and reverse launch
     VS_HIP(hipEventRecord(ev[0], ss));
-    VS_HIP(hipStreamWaitEvent(sm, ev[0], 0));
+    VS_HIP(hipStreamWaitEvent(s_main, ev[0], 0));
     return VS_OK;
 }
 

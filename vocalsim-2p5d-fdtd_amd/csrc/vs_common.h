@@ -246,18 +246,22 @@ struct AdjBufs {
 // reverse step has the stencil of the forward step (pressure-like phase
 // reading Lx on the left and Ly below, face phase reading Kp on the right
 // and above), so it runs in the same on-chip layout.  Slabs hold, per tube
-// (launch order) and per step of a segment, three fields of K * T floats in
+// (launch order) and per step of a block, fields of K * T floats in
 // lane-slot order s = k * T + t (slot k of cell lane t):
-//   rec  forward factors of step n: dUx'/dgX, dUy'/dgY, flux divergence
-//   adj  Lx and Ly entering reverse step n, and Kp of step n
-// The sums over steps of adj * rec are the log-derivatives of J with respect
-// to each cell's gX, gY and kP.
+//   rec  the forward step's new pressure P' and flux divergence (2 fields)
+//   adj  Kp of the reverse step (1 field)
+//   start  Lx, Ly as the block's reverse begins (2 fields, per tube)
+// The sums kernel replays the Lx, Ly recurrences from start and adj and
+// forms the forward factors from rec; the sums over steps of their products
+// are the log-derivatives of J with respect to each cell's gX, gY and kP.
 struct FastAdj {
     float* rec;
     float* adj;
+    float* start;
     float* state;          // per tube 3 * K * T: Kp, Ly, Lx between launches
-    float* sink;           // 3 * K * T: stores of waves past every cell of their tube
-    long long slab_tube;   // floats per tube of rec / adj (S * 3 * K * T)
+    float* sink;           // 2 * K * T: stores of waves past every cell of their tube
+    long long rec_tube;    // floats per tube of rec (B * 2 * K * T)
+    long long adj_tube;    // floats per tube of adj (B * K * T)
     int lanes;             // T: cell lanes per work-group
     const double* gprobe;  // per step: d J / d probe
     double *Lx, *Ly, *Lp;  // d J / d log gX, d log gY, d log kP (cell layout)
@@ -301,8 +305,8 @@ hipError_t launch_fast_record(int K, const TubeDesc* d_tubes, const int* d_order
 // the per-cell sums of the segment into a.Lx, a.Ly, a.Lp.
 hipError_t launch_fast_reverse(int K, const TubeDesc* d_tubes, const int* d_order, int n, FastBufs f,
                                FastAdj a, int s0, int ns, int guard, hipStream_t st);
-hipError_t launch_fast_adj_sums(int K, const TubeDesc* d_tubes, const int* d_order, int n, FastAdj a, int s0,
-                                int ns, hipStream_t st);
+hipError_t launch_fast_adj_sums(int K, const TubeDesc* d_tubes, const int* d_order, int n, FastBufs f, FastAdj a,
+                                const float* source, int s0, int ns, int groups, hipStream_t st);
 hipError_t launch_bump(int* d_t0, int s, hipStream_t st);
 hipError_t configure_fast(int K);
 template <typename Real>

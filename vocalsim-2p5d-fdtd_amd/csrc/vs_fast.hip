@@ -167,20 +167,21 @@ __device__ __forceinline__ void fast_pressure(float* __restrict__ P, const float
         const float left = k > 0 ? Ux[k - 1] : uxl0;
         const float dv = (Ux[k] - left) + (uy - uyb);
         const float aP = plut[class_of<K>(cw, k) & kClassMask];
-        P[k] = aP * p - kP[k] * dv;
+        const float pn = aP * p - kP[k] * dv;
+        P[k] = pn;
         if (REC) {
-            slab_store(dr + 2 * KT4, vo, dv);
+            slab_store(dr, vo, pn);
+            slab_store(dr + KT4, vo, dv);
             vo = next_off(vo, T4);
         }
     }
 }
 
-template <int K, bool REC>
+template <int K>
 __device__ __forceinline__ void fast_faces(const float* __restrict__ P, const float* __restrict__ PA,
                                            float* __restrict__ UY, const float2* __restrict__ vtab, float (&Ux)[K],
                                            const float (&gX)[K], const float (&gY)[K],
-                                           const uint32_t (&cw)[(K + 3) / 4], float ev, int mW, char* dr,
-                                           uint32_t vo, uint32_t T4, uint32_t KT4) {
+                                           const uint32_t (&cw)[(K + 3) / 4], float ev, int mW) {
 #pragma clang fp contract(fast)
     float pn = P[0];
 #pragma unroll
@@ -196,11 +197,6 @@ __device__ __forceinline__ void fast_faces(const float* __restrict__ P, const fl
         const float fx = ps - prx, fy = ps - pa;
         Ux[k] = axy.x * Ux[k] + gX[k] * fx;
         UY[k] = axy.y * UY[k] + gY[k] * fy;
-        if (REC) {
-            slab_store(dr, vo, fx);
-            slab_store(dr + KT4, vo, fy);
-            vo = next_off(vo, T4);
-        }
     }
     UY[K] = Ux[K - 1];
 }
@@ -323,12 +319,12 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         asm volatile("" : "+v"(tl));
 #pragma unroll
         for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
-        // forward factors of this step: field f of slot k at float f * KT + k * T + lane;
+        // P' and divergence of this step: field f of slot k at float f * KT + k * T + lane;
         // waves wholly past the tube's cells write a small shared sink instead
         // (it stays in L2: the sums never read those slots)
         char* const dr = !REC ? nullptr
                          : __builtin_amdgcn_readfirstlane(tl) * K < Hi * Wp
-                             ? reinterpret_cast<char*>(frec + blockIdx.x * slab_tube + (long long)(ts - t0) * 3 * KT)
+                             ? reinterpret_cast<char*>(frec + blockIdx.x * slab_tube + (long long)(ts - t0) * 2 * KT)
                              : reinterpret_cast<char*>(fsink);
         const uint32_t T4 = 4u * (uint32_t)T, KT4 = 4u * (uint32_t)KT;
         uint32_t vo = 4u * (uint32_t)tl;
@@ -347,10 +343,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
 #pragma unroll
             for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
             float* const rec = rec0 + tl * S;
-            vo = 4u * (uint32_t)tl;
-            if (REC) asm("" : "+v"(vo));
             // cell above: slot k - m of lane l - q
-            fast_faces<K, REC>(rec, rec - nbOff, rec + K, vtab, Ux, gX, gY, cw, ev, mW, dr, vo, T4, KT4);
+            fast_faces<K>(rec, rec - nbOff, rec + K, vtab, Ux, gX, gY, cw, ev, mW);
         }
         __syncthreads();
     }
@@ -378,6 +372,13 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
     }
 }
 
+// Face update of the scaled adjoint, a L + g (K_neighbour - K_cell), with its
+// rounding fixed (a product, a difference, one fused multiply-add) so the
+// sums kernel replays it bit for bit.
+__device__ __forceinline__ float adj_face(float a, float L, float g, float kn, float kc) {
+    return __builtin_fmaf(g, kn - kc, a * L);
+}
+
 // The two phases of one reverse step over a lane's run (same aliasing
 // structure as fast_pressure / fast_faces: Kp-region written in the first,
 // Ly-region in the second).
@@ -386,21 +387,19 @@ __device__ __forceinline__ void rev_pressure(float* __restrict__ KP, const float
                                              const float* __restrict__ LYB, const float* __restrict__ plut,
                                              const float (&LX)[K], const float (&kP)[K],
                                              const uint32_t (&cw)[(K + 3) / 4], float lx0, int mW, int kL, float gl,
-                                             char* ad, uint32_t vo, uint32_t T4, uint32_t KT4) {
+                                             char* ad, uint32_t vo, uint32_t T4) {
 #pragma clang fp contract(fast)
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const uint32_t cb = class_of<K>(cw, k);
         const float kold = KP[k], ly = LY[k], lx = LX[k];
-        slab_store(ad, vo, lx);
-        slab_store(ad + KT4, vo, ly);
         const float lyb = (k + mW >= K) ? LYB[k + K + 1] : LYB[k];
         const float left = k > 0 ? ((class_of<K>(cw, k - 1) & kClassSrcX) ? 0.f : LX[k - 1]) : lx0;
         float br = ((cb & kClassSelfOff) ? 0.f : lx + ly) - left - lyb;
         if (k == kL) br += gl;
         const float kn = plut[cb & kClassMask] * kold + kP[k] * br;
         KP[k] = kn;
-        slab_store(ad + 2 * KT4, vo, kn);
+        slab_store(ad, vo, kn);
         vo = next_off(vo, T4);
     }
 }
@@ -418,8 +417,8 @@ __device__ __forceinline__ void rev_faces(const float* __restrict__ KP, const fl
         kn = (k < K - 1) ? KP[k + 1] : KP[2 * K + 1];
         const float ka = (k < mW) ? KA[k - K - 1] : KA[k];
         const float2 axy = vtab[class_of<K>(cw, k) & kClassMask];
-        LX[k] = axy.x * LX[k] + gX[k] * (kn - kc);
-        LY[k] = axy.y * LY[k] + gY[k] * (ka - kc);
+        LX[k] = adj_face(axy.x, LX[k], gX[k], kn, kc);
+        LY[k] = adj_face(axy.y, LY[k], gY[k], ka, kc);
     }
     LY[K] = (class_of<K>(cw, K - 1) & kClassSrcX) ? 0.f : LX[K - 1];
 }
@@ -491,6 +490,9 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
             rec[k] = st[k * T];
             rec[K + k] = st[KT + k * T];
             LX[k] = st[2 * KT + k * T];
+            // Lx, Ly as this block's reverse begins, for the sums
+            a.start[(long long)blockIdx.x * 2 * KT + k * T + t] = LX[k];
+            a.start[(long long)blockIdx.x * 2 * KT + KT + k * T + t] = rec[K + k];
             if (++c == Wp) { c = 0; ++r; }
         }
         rec[2 * K] = (class_of<K>(cw, K - 1) & kClassSrcX) ? 0.f : LX[K - 1];
@@ -504,17 +506,16 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
         for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
         // waves wholly past the tube's cells write the shared sink
         char* const ad = __builtin_amdgcn_readfirstlane(tl) * K < Hi * Wp
-                             ? reinterpret_cast<char*>(a.adj + blockIdx.x * a.slab_tube + (long long)(n - s0) * 3 * KT)
+                             ? reinterpret_cast<char*>(a.adj + blockIdx.x * a.adj_tube + (long long)(n - s0) * KT)
                              : reinterpret_cast<char*>(a.sink);
-        const uint32_t T4 = 4u * (uint32_t)T, KT4 = 4u * (uint32_t)KT;
+        const uint32_t T4 = 4u * (uint32_t)T;
         uint32_t vo = 4u * (uint32_t)tl;
         const float gl = tl == lL ? (float)gp[n] : 0.f;
         // ---------------- pressure-like phase ----------------
         {
             float* const rec = rec0 + tl * S;
             // Ly of the cell below; r Lx of the previous lane's last cell
-            rev_pressure<K>(rec, rec + K, rec + nbOff + K, plut, LX, kP, cw, rec[2 * K - S], mW, kL, gl, ad, vo, T4,
-                            KT4);
+            rev_pressure<K>(rec, rec + K, rec + nbOff + K, plut, LX, kP, cw, rec[2 * K - S], mW, kL, gl, ad, vo, T4);
         }
         __syncthreads();
         // ---------------- face phase ----------------
@@ -541,35 +542,65 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
     }
 }
 
-// Per-cell sums over the steps of one segment of adj * rec (double), added to
-// the cell-layout log-derivatives: one thread per slot s = k * T + t.
-__global__ __launch_bounds__(256) void k_fast_adj_sums(const TubeDesc* __restrict__ tubes,
-                                                       const int* __restrict__ order, int K, FastAdj a, int s0,
-                                                       int ns) {
+// Per-cell sums over the steps of one block.  Grid (gx, tubes): the gx
+// work-groups of a tube stride over its slots s = k * T + t (real cells
+// only), so the slab bases are work-group uniform.  From the recorded P' and
+// divergence and the reverse pass's Kp, each thread rebuilds, step by step
+// backwards:
+//   the forward factors  fx = s P' - (r P'_right or -ev),  fy = s P' - P'_top
+//   (exactly as the face phase of the forward formed them), and
+//   the face adjoints Lx, Ly entering each step, replaying adj_face from the
+//   values at the block's start,
+// and adds Lx fx, Ly fy, -Kp div (double) to the cell-layout log-derivatives.
+// Neighbour values come from the same slab step (L2-resident).
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_fast_adj_sums(const TubeDesc* __restrict__ tubes, const int* __restrict__ order, int K, FastBufs f,
+                     FastAdj a, const float* __restrict__ source, int s0, int ns) {
+#pragma clang fp contract(fast)
     const int T = a.lanes, KT = K * T;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= KT) return;
-    const TubeDesc& D = tubes[order[blockIdx.y]];
-    const int k = s / T, t = s - (s / T) * T;
-    const int j = t * K + k;
-    const int Wp = D.W - 1, Wi = D.W - 2, Hi = D.H - 2;
-    const int r = j / Wp, c = j - (j / Wp) * Wp;
-    if (r >= Hi || c >= Wi) return;
+    const int b = blockIdx.y;
+    const TubeDesc& D = tubes[order[b]];
+    const int W = D.W, Wp = W - 1, Wi = W - 2, Hi = D.H - 2;
     const int tend = min(s0 + ns, D.num_steps);
-    const float* R = a.rec + blockIdx.y * a.slab_tube + s;
-    const float* A = a.adj + blockIdx.y * a.slab_tube + s;
-    double sx = 0.0, sy = 0.0, sp = 0.0;
-#pragma unroll 4
-    for (int n = s0; n < tend; ++n) {
-        const long long o = (long long)(n - s0) * 3 * KT;
-        sx += (double)A[o] * (double)R[o];
-        sy += (double)A[o + KT] * (double)R[o + KT];
-        sp += (double)A[o + 2 * KT] * (double)R[o + 2 * KT];
+    const float* const R = a.rec + b * a.rec_tube;
+    const float* const A = a.adj + b * a.adj_tube;
+    const float* const st0 = a.start + (long long)b * 2 * KT;
+    const float* const src = source + D.src_off;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < KT; s += gridDim.x * blockDim.x) {
+        const int k = s / T, t = s - (s / T) * T;
+        const int j = t * K + k;
+        const int r = j / Wp, c = j - (j / Wp) * Wp;
+        if (r >= Hi || c >= Wi) continue;
+        const long long q = D.cell_off + (long long)(r + 1) * W + (c + 1);
+        // neighbour slots: right j + 1 (a real or pad cell of this row), top j - Wp
+        const int jr = j + 1, jt = j - Wp;
+        const int sr = (jr % K) * T + jr / K;
+        const bool has_top = jt >= 0;
+        const int stp = has_top ? (jt % K) * T + jt / K : s;
+        const uint32_t cb = f.cls8[q];
+        const float ax = f.ctab[64 + (cb & kClassMask)], ay = f.ctab[128 + (cb & kClassMask)];
+        const float gx = f.gX[q], gy = f.gY[q];
+        const bool selfoff = (cb & kClassSelfOff) != 0, srcx = (cb & kClassSrcX) != 0;
+        float lx = st0[s], ly = st0[KT + s];
+        double sx = 0.0, sy = 0.0, sp = 0.0;
+#pragma unroll 2
+        for (int n = tend - 1; n >= s0; --n) {
+            const int o = (n - s0) * 2 * KT, oa = (n - s0) * KT;
+            const float p = R[o + s], pr = R[o + sr], pt = has_top ? R[o + stp] : 0.f, dv = R[o + KT + s];
+            const float kc = A[oa + s], kr = A[oa + sr], kt = has_top ? A[oa + stp] : 0.f;
+            const float ps = selfoff ? 0.f : p;
+            const float prx = srcx ? -src[n] : pr;
+            const float fx = ps - prx, fy = ps - pt;
+            sx += (double)lx * (double)fx;
+            sy += (double)ly * (double)fy;
+            sp += (double)kc * (double)dv;
+            lx = adj_face(ax, lx, gx, kr, kc);
+            ly = adj_face(ay, ly, gy, kt, kc);
+        }
+        a.Lx[q] += sx;
+        a.Ly[q] += sy;
+        a.Lp[q] -= sp;
     }
-    const long long q = D.cell_off + (long long)(r + 1) * D.W + (c + 1);
-    a.Lx[q] += sx;
-    a.Ly[q] += sy;
-    a.Lp[q] -= sp;
 }
 
 template <int K>
@@ -609,7 +640,7 @@ hipError_t launch_rec_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs
     hipError_t e = configure_k<K>();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_fast_step<K, true>), dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0,
-                       nsteps, (const int*)nullptr, guard, a.rec, a.slab_tube, a.sink);
+                       nsteps, (const int*)nullptr, guard, a.rec, a.rec_tube, a.sink);
     return hipGetLastError();
 }
 
@@ -698,11 +729,10 @@ hipError_t launch_fast_reverse(int K, const TubeDesc* d_tubes, const int* d_orde
     }
 }
 
-hipError_t launch_fast_adj_sums(int K, const TubeDesc* d_tubes, const int* d_order, int n, FastAdj a, int s0,
-                                int ns, hipStream_t st) {
-    const int KT = K * a.lanes;
-    hipLaunchKernelGGL(k_fast_adj_sums, dim3((KT + 255) / 256, n), dim3(256), 0, st, d_tubes, d_order, K, a, s0,
-                       ns);
+hipError_t launch_fast_adj_sums(int K, const TubeDesc* d_tubes, const int* d_order, int n, FastBufs f, FastAdj a,
+                                const float* source, int s0, int ns, int groups, hipStream_t st) {
+    hipLaunchKernelGGL(k_fast_adj_sums, dim3(std::max(1, groups / n), n), dim3(1024), 0, st, d_tubes, d_order, K, f,
+                       a, source, s0, ns);
     return hipGetLastError();
 }
 
