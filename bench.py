@@ -19,11 +19,12 @@ BYTES_FP32 = 37   # P, Vx, Vy read+write, dP, dX, dY read (4 B each), 1 B class
 BYTES_FP64 = 73
 HBM_PEAK_GBS = 8000.0
 BLOCK = 256
-# HBM bytes per 256-step launch of k_fast_step<19> on the 256-tube batch at 1024
-# threads, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes, each scaled by
-# the same counter over a plain copy kernel of known size (FETCH x2.000,
-# WRITE x1.000); profiles/k_fast_step_batch256_fp32_pmc.md.
-PMC_TRAFFIC_BYTES = 1.830e9
+# HBM bytes per 256-step launch of k_fast_step<20> on the 256-tube batch at 960
+# threads (the autotuned shape), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# passes, each scaled by the same counter over a plain copy kernel of known size
+# (FETCH x2.000, WRITE x1.000); profiles/k_fast_step_batch256_fp32_pmc.md.
+PMC_TRAFFIC_BYTES = 1.851e9
+PMC_THREADS = 960
 
 
 def log(msg):
@@ -141,10 +142,18 @@ def main():
     specs = tubes.batch_tubes(args.tubes, seed=0, num_steps=nsteps)
     kernel = args.kernel if args.kernel >= 0 else _abi.K_FAST
     threads = args.threads or 1024
-    log(f"[bench] building {args.tubes} tubes (FP32), kernel={kernel} threads={threads}")
+    log(f"[bench] building {args.tubes} tubes (FP32), kernel={kernel}")
     s = vocalsim.Solver(specs, precision=32, threads=threads, steps_per_launch=BLOCK, kernel=kernel)
+    if not args.threads:
+        # work-group shape from the library's autotuner (timed on this batch);
+        # the launch length stays at the 256-step block
+        rep = s.autotune(256)
+        threads = s.get_option(_abi.OPT_THREADS)
+        s.set_option(_abi.OPT_STEPS_PER_LAUNCH, BLOCK)
+        log("[bench] autotune: " + ", ".join(f"{c['threads']}/{c['steps_per_launch']}: {c['us_per_step']:.2f} us"
+                                            for c in rep.get("candidates", [])) + f" -> threads {threads}")
     cells = s.cells_per_step
-    log(f"[bench] interior cells per step: {cells}")
+    log(f"[bench] interior cells per step: {cells}, threads {threads}")
     s.run(args.warmup * BLOCK)
     torch.cuda.synchronize()
     s.set_option(_abi.OPT_TIMING, 0)
@@ -256,7 +265,7 @@ def main():
         "precision_fp32_vs_fp64": precision,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": PMC_TRAFFIC_BYTES if (kernel == _abi.K_FAST and threads == 1024
+                     "traffic": PMC_TRAFFIC_BYTES if (kernel == _abi.K_FAST and threads == PMC_THREADS
                                                       and args.tubes == 256) else None,
                      "kernel": "k_fast_step<K> (batched FP32)", "mean_launch_ms": mean_launch_ms,
                      "algorithmic_bytes_per_launch": alg_bytes},

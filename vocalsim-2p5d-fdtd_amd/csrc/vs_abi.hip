@@ -566,36 +566,45 @@ int check_handle(const vs_handle* h) {
 }
 
 // Maps the 16-bit face codes of the flux-form setup to class bytes: the
-// distinct damping triples (code_p, code_x, code_y) of the batch become
-// classes 1..63 (class 0 = every damping factor zero, the pad cells), with the
-// two face-form bits on top; fills the per-class aP, ax, ay table.
+// distinct (damping triple, face-form bits) of the batch become classes
+// 1..63 (class 0 = every damping factor zero and no face-form bit, the pad
+// cells).  Per class the table holds aP, ax, ay and two 0/1 factors:
+// s = 1 unless the cell's own pressure is left out of its face updates
+// (walls carrying an admittance, excitation) and r = 1 unless the x-face
+// carries the source instead of the right neighbour's pressure (excitation).
+// The step kernels read s and r from the sign bits of their ax, ay entries
+// (ax, ay >= 0; the sign bit is set where s = 0, resp. r = 0).
 int build_fast_classes(vs_handle* h) {
     FastBufs& f = h->fb;
     const size_t n = (size_t)h->total_cells;
     std::vector<uint16_t> code(n);
     VS_HIP(hipStreamSynchronize(h->stream));
     VS_HIP(hipMemcpy(code.data(), f.code, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
-    // damping part: bits [3, 12) = code_x + 8 code_y + 64 code_p
-    std::vector<int> cls_of(512, -1);
+    // bits [3, 12) = code_x + 8 code_y + 64 code_p; bit 12 self-off, bit 13 source face
+    std::vector<int> cls_of(2048, -1);
     const int zero = 1 + 8 * 1 + 64 * 1;
     cls_of[zero] = 0;
     int ncls = 1;
     std::vector<uint8_t> c8(n);
     for (size_t i = 0; i < n; ++i) {
-        const int d = (code[i] >> 3) & 0x1ff;
+        const int d = (code[i] >> 3) & 0x7ff;
         if (cls_of[d] < 0) {
-            if (ncls == 64) return fail(VS_E_INVALID, "more than 63 damping classes in one batch");
+            if (ncls == kFastClasses)
+                return fail(VS_E_INVALID, "more than " + std::to_string(kFastClasses - 1) + " cell classes in one batch");
             cls_of[d] = ncls++;
         }
-        c8[i] = (uint8_t)(cls_of[d] | ((code[i] & (1u << 12)) ? 0x40 : 0) | ((code[i] & (1u << 13)) ? 0x80 : 0));
+        c8[i] = (uint8_t)cls_of[d];
     }
-    std::vector<float> tab(3 * 64, 0.0f);
-    for (int d = 0; d < 512; ++d) {
+    std::vector<float> tab(5 * kFastClasses, 0.0f);
+    for (int d = 0; d < 2048; ++d) {
         const int c = cls_of[d];
         if (c < 0) continue;
+        const bool s0 = (d & 0x200) != 0, r0 = (d & 0x400) != 0;
         tab[c] = f.lut[(d >> 6) & 7];
-        tab[64 + c] = f.lut[d & 7];
-        tab[128 + c] = f.lut[(d >> 3) & 7];
+        tab[kFastClasses + c] = s0 ? -f.lut[d & 7] : f.lut[d & 7];
+        tab[2 * kFastClasses + c] = r0 ? -f.lut[(d >> 3) & 7] : f.lut[(d >> 3) & 7];
+        tab[3 * kFastClasses + c] = s0 ? 0.0f : 1.0f;
+        tab[4 * kFastClasses + c] = r0 ? 0.0f : 1.0f;
     }
     VS_HIP(hipMemcpy(f.cls8, c8.data(), n, hipMemcpyHostToDevice));
     VS_HIP(hipMemcpy(f.ctab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -633,7 +642,7 @@ int prepare_fast(vs_handle* h, int threads) {
         int rc;
         if ((rc = h->alloc(&f.kP, n)) || (rc = h->alloc(&f.gX, n)) || (rc = h->alloc(&f.gY, n)) ||
             (rc = h->alloc(&f.Uxf, n)) || (rc = h->alloc(&f.Uyf, n)) || (rc = h->alloc(&f.code, n)) ||
-            (rc = h->alloc(&f.cls8, n)) || (rc = h->alloc(&f.ctab, 3 * 64)))
+            (rc = h->alloc(&f.cls8, n)) || (rc = h->alloc(&f.ctab, 5 * kFastClasses)))
             return rc;
         const PhysTables& ph = h->desc[0].ph;
         f.lut[0] = 1.0f;

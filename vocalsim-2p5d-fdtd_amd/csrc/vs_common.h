@@ -214,12 +214,13 @@ struct PlaneSection {
 int plane_sections(const float* area_cm2, int n, std::vector<PlaneSection>& out, std::string& err);
 
 // Per-cell coefficients of the FP32 flux-form kernel (vs_fast.hip).
+constexpr int kFastClasses = 64;   // cell classes of one batch (class bytes 0..63)
 struct FastBufs {
     float* kP; float* gX; float* gY;     // pressure and face coefficients
     float* Uxf; float* Uyf;              // flux state U = V * depth
     uint16_t* code;                      // damping codes + face-form bits
-    uint8_t* cls8;                       // class byte: damping class (6 bits) + face-form bits
-    float* ctab;                         // per damping class: aP[64], ax[64], ay[64]
+    uint8_t* cls8;                       // class byte (< kFastClasses): damping triple + face-form bits
+    float* ctab;                         // per class: aP, ax (sign: !s), ay (sign: !r), s, r (kFastClasses each)
     float lut[8];                        // damping factor by code
 };
 

@@ -40,12 +40,12 @@ constexpr int kShiftXY = 3;
 constexpr int kShiftP = 9;
 constexpr uint32_t kCodeSelfOff = 1u << 12;
 constexpr uint32_t kCodeSrcX = 1u << 13;
-// The step kernel reads a class byte per cell instead: a batch-local damping
-// class (low 6 bits, class 0 = all damping factors zero, used for pad cells)
-// and the two face-form bits.
-constexpr uint32_t kClassMask = 0x3fu;
-constexpr uint32_t kClassSelfOff = 0x40u;
-constexpr uint32_t kClassSrcX = 0x80u;
+// The step kernel reads a class byte per cell instead: a batch-local class
+// (< kFastClasses) of the damping triple and the two face-form bits, indexing
+// the tables aP and (ax, ay) (class 0: every factor zero, used for pad cells).
+// The sign bit of the ax entry marks s = 0 (the cell's own pressure is left
+// out of its face updates), that of ay marks r = 0 (the x-face carries the
+// source); the magnitudes are the damping factors.
 
 // ---------------------------------------------------------------------------
 // Setup of the per-cell coefficients of the flux form (evaluated in double
@@ -147,6 +147,9 @@ __device__ inline uint32_t next_off(uint32_t vo, uint32_t step) {
     return vo;
 }
 
+// Sign-bit flags of the (ax, ay) class entries.
+__device__ __forceinline__ bool flag_off(float e) { return __float_as_int(e) < 0; }
+
 // The two phases of one step over a lane's run of K cells.  The pointers are
 // __restrict__: in the pressure-like phase only the lane's own P-region is
 // written (Uy of this lane and of the lane below are only read), in the face
@@ -155,7 +158,7 @@ __device__ inline uint32_t next_off(uint32_t vo, uint32_t step) {
 // the writes of earlier ones.
 template <int K, bool REC>
 __device__ __forceinline__ void fast_pressure(float* __restrict__ P, const float* __restrict__ UY,
-                                              const float* __restrict__ UYB, const float* __restrict__ plut,
+                                              const float* __restrict__ UYB, const float* __restrict__ ptab,
                                               const float (&Ux)[K], const float (&kP)[K], const uint32_t (&cw)[(K + 3) / 4],
                                               float uxl0, int mW, char* dr, uint32_t vo, uint32_t T4, uint32_t KT4) {
 #pragma clang fp contract(fast)
@@ -166,7 +169,7 @@ __device__ __forceinline__ void fast_pressure(float* __restrict__ P, const float
         const float uyb = (k + mW >= K) ? UYB[k + K + 1] : UYB[k];
         const float left = k > 0 ? Ux[k - 1] : uxl0;
         const float dv = (Ux[k] - left) + (uy - uyb);
-        const float aP = plut[class_of<K>(cw, k) & kClassMask];
+        const float aP = ptab[class_of<K>(cw, k)];
         const float pn = aP * p - kP[k] * dv;
         P[k] = pn;
         if (REC) {
@@ -179,7 +182,7 @@ __device__ __forceinline__ void fast_pressure(float* __restrict__ P, const float
 
 template <int K>
 __device__ __forceinline__ void fast_faces(const float* __restrict__ P, const float* __restrict__ PA,
-                                           float* __restrict__ UY, const float2* __restrict__ vtab, float (&Ux)[K],
+                                           float* __restrict__ UY, const float2* __restrict__ ftab, float (&Ux)[K],
                                            const float (&gX)[K], const float (&gY)[K],
                                            const uint32_t (&cw)[(K + 3) / 4], float ev, int mW) {
 #pragma clang fp contract(fast)
@@ -189,14 +192,13 @@ __device__ __forceinline__ void fast_faces(const float* __restrict__ P, const fl
         const float p = pn;
         pn = (k < K - 1) ? P[k + 1] : P[2 * K + 1];   // next lane's first cell
         const float pa = (k < mW) ? PA[k - K - 1] : PA[k];
-        const uint32_t cb = class_of<K>(cw, k);
-        const float2 axy = vtab[cb & kClassMask];
-        // walls with admittance: U = -g P_neighbour; excitation: Ux = g exeV
-        const float ps = (cb & kClassSelfOff) ? 0.f : p;
-        const float prx = (cb & kClassSrcX) ? -ev : pn;
+        const float2 c2 = ftab[class_of<K>(cw, k)];   // ax, ay with the s, r flags in their signs
+        // walls with admittance (s = 0): U = -g P_neighbour; excitation (s = r = 0): Ux = g exeV
+        const float ps = flag_off(c2.x) ? 0.f : p;
+        const float prx = flag_off(c2.y) ? -ev : pn;
         const float fx = ps - prx, fy = ps - pa;
-        Ux[k] = axy.x * Ux[k] + gX[k] * fx;
-        UY[k] = axy.y * UY[k] + gY[k] * fy;
+        Ux[k] = fabsf(c2.x) * Ux[k] + gX[k] * fx;
+        UY[k] = fabsf(c2.y) * UY[k] + gY[k] * fy;
     }
     UY[K] = Ux[K - 1];
 }
@@ -213,8 +215,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
 #pragma clang fp contract(fast)
     constexpr int S = 2 * K + 1;                 // record stride (floats)
     extern __shared__ float lds[];
-    __shared__ float2 vtab[64];                  // (ax, ay) by damping class
-    __shared__ float plut[64];                   // aP by damping class
+    __shared__ float2 ftab[kFastClasses];        // (ax, ay) by class, flags in the signs
+    __shared__ float ptab[kFastClasses];         // aP by class
     __shared__ double sexe[2];                   // source value of steps t, t + 1
     __shared__ TwoMassState tm;
     // threads [0, T) own cells; the last wave is the control wave (source,
@@ -249,8 +251,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         const float* pL = rec0 + (jL / K) * S + jL % K;
         const float* pE = rec0 + (jE / K) * S + jE % K;
         double pexc = 0.0;
-        plut[lane] = f.ctab[lane];
-        vtab[lane] = make_float2(f.ctab[64 + lane], f.ctab[128 + lane]);
+        ptab[lane] = f.ctab[lane];
+        ftab[lane] = make_float2(f.ctab[kFastClasses + lane], f.ctab[2 * kFastClasses + lane]);
         if (lane == 0) {
             pexc = b.pexc[tube];
             if (twomass) tm = b.tm[tube];
@@ -332,7 +334,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         {
             float* const rec = rec0 + tl * S;
             // Uy of the cell below: slot k + m of lane l + q; run end of lane l - 1
-            fast_pressure<K, REC>(rec, rec + K, rec + nbOff + K, plut, Ux, kP, cw, rec[2 * K - S], mW, dr, vo, T4,
+            fast_pressure<K, REC>(rec, rec + K, rec + nbOff + K, ptab, Ux, kP, cw, rec[2 * K - S], mW, dr, vo, T4,
                                   KT4);
         }
         __syncthreads();
@@ -344,7 +346,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
             for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
             float* const rec = rec0 + tl * S;
             // cell above: slot k - m of lane l - q
-            fast_faces<K>(rec, rec - nbOff, rec + K, vtab, Ux, gX, gY, cw, ev, mW);
+            fast_faces<K>(rec, rec - nbOff, rec + K, ftab, Ux, gX, gY, cw, ev, mW);
         }
         __syncthreads();
     }
@@ -384,20 +386,22 @@ __device__ __forceinline__ float adj_face(float a, float L, float g, float kn, f
 // Ly-region in the second).
 template <int K>
 __device__ __forceinline__ void rev_pressure(float* __restrict__ KP, const float* __restrict__ LY,
-                                             const float* __restrict__ LYB, const float* __restrict__ plut,
-                                             const float (&LX)[K], const float (&kP)[K],
+                                             const float* __restrict__ LYB, const float* __restrict__ ptab,
+                                             const float2* __restrict__ ftab, const float (&LX)[K], const float (&kP)[K],
                                              const uint32_t (&cw)[(K + 3) / 4], float lx0, int mW, int kL, float gl,
                                              char* ad, uint32_t vo, uint32_t T4) {
 #pragma clang fp contract(fast)
+    float left = lx0;   // r Lx of the cell on the left
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const uint32_t cb = class_of<K>(cw, k);
+        const float2 c2 = ftab[cb];   // ax, ay with the s, r flags in their signs
         const float kold = KP[k], ly = LY[k], lx = LX[k];
         const float lyb = (k + mW >= K) ? LYB[k + K + 1] : LYB[k];
-        const float left = k > 0 ? ((class_of<K>(cw, k - 1) & kClassSrcX) ? 0.f : LX[k - 1]) : lx0;
-        float br = ((cb & kClassSelfOff) ? 0.f : lx + ly) - left - lyb;
+        float br = (flag_off(c2.x) ? 0.f : lx + ly) - left - lyb;
         if (k == kL) br += gl;
-        const float kn = plut[cb & kClassMask] * kold + kP[k] * br;
+        const float kn = ptab[cb] * kold + kP[k] * br;
+        left = flag_off(c2.y) ? 0.f : lx;
         KP[k] = kn;
         slab_store(ad, vo, kn);
         vo = next_off(vo, T4);
@@ -406,7 +410,7 @@ __device__ __forceinline__ void rev_pressure(float* __restrict__ KP, const float
 
 template <int K>
 __device__ __forceinline__ void rev_faces(const float* __restrict__ KP, const float* __restrict__ KA,
-                                          float* __restrict__ LY, const float2* __restrict__ vtab, float (&LX)[K],
+                                          float* __restrict__ LY, const float2* __restrict__ ftab, float (&LX)[K],
                                           const float (&gX)[K], const float (&gY)[K],
                                           const uint32_t (&cw)[(K + 3) / 4], int mW) {
 #pragma clang fp contract(fast)
@@ -416,11 +420,11 @@ __device__ __forceinline__ void rev_faces(const float* __restrict__ KP, const fl
         const float kc = kn;
         kn = (k < K - 1) ? KP[k + 1] : KP[2 * K + 1];
         const float ka = (k < mW) ? KA[k - K - 1] : KA[k];
-        const float2 axy = vtab[class_of<K>(cw, k) & kClassMask];
-        LX[k] = adj_face(axy.x, LX[k], gX[k], kn, kc);
-        LY[k] = adj_face(axy.y, LY[k], gY[k], ka, kc);
+        const float2 c2 = ftab[class_of<K>(cw, k)];
+        LX[k] = adj_face(fabsf(c2.x), LX[k], gX[k], kn, kc);
+        LY[k] = adj_face(fabsf(c2.y), LY[k], gY[k], ka, kc);
     }
-    LY[K] = (class_of<K>(cw, K - 1) & kClassSrcX) ? 0.f : LX[K - 1];
+    LY[K] = flag_off(ftab[class_of<K>(cw, K - 1)].y) ? 0.f : LX[K - 1];
 }
 
 // ---------------------------------------------------------------------------
@@ -446,8 +450,8 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
 #pragma clang fp contract(fast)
     constexpr int S = 2 * K + 1;
     extern __shared__ float lds[];
-    __shared__ float2 vtab[64];
-    __shared__ float plut[64];
+    __shared__ float2 ftab[kFastClasses];
+    __shared__ float ptab[kFastClasses];
     const int T = blockDim.x, t = threadIdx.x;
     const int KT = K * T;
     float* const rec0 = lds + G * S;
@@ -467,8 +471,8 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
 
     for (int i = t; i < 2 * G * S; i += T) lds[i < G * S ? i : i + T * S] = 0.f;
     if (t < 64) {
-        plut[t] = f.ctab[t];
-        vtab[t] = make_float2(f.ctab[64 + t], f.ctab[128 + t]);
+        ptab[t] = f.ctab[t];
+        ftab[t] = make_float2(f.ctab[kFastClasses + t], f.ctab[2 * kFastClasses + t]);
     }
     float LX[K], kP[K], gX[K], gY[K];
     uint32_t cw[(K + 3) / 4];
@@ -495,7 +499,7 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
             a.start[(long long)blockIdx.x * 2 * KT + KT + k * T + t] = rec[K + k];
             if (++c == Wp) { c = 0; ++r; }
         }
-        rec[2 * K] = (class_of<K>(cw, K - 1) & kClassSrcX) ? 0.f : LX[K - 1];
+        rec[2 * K] = f.ctab[4 * kFastClasses + class_of<K>(cw, K - 1)] * LX[K - 1];   // r Lx of the run end
     }
     __syncthreads();
 
@@ -515,7 +519,8 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
         {
             float* const rec = rec0 + tl * S;
             // Ly of the cell below; r Lx of the previous lane's last cell
-            rev_pressure<K>(rec, rec + K, rec + nbOff + K, plut, LX, kP, cw, rec[2 * K - S], mW, kL, gl, ad, vo, T4);
+            rev_pressure<K>(rec, rec + K, rec + nbOff + K, ptab, ftab, LX, kP, cw, rec[2 * K - S], mW, kL, gl, ad, vo,
+                            T4);
         }
         __syncthreads();
         // ---------------- face phase ----------------
@@ -524,7 +529,7 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
 #pragma unroll
             for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
             float* const rec = rec0 + tl * S;
-            rev_faces<K>(rec, rec - nbOff, rec + K, vtab, LX, gX, gY, cw, mW);   // cell above: rec - nbOff
+            rev_faces<K>(rec, rec - nbOff, rec + K, ftab, LX, gX, gY, cw, mW);   // cell above: rec - nbOff
         }
         __syncthreads();
     }
@@ -578,9 +583,10 @@ void k_fast_adj_sums(const TubeDesc* __restrict__ tubes, const int* __restrict__
         const bool has_top = jt >= 0;
         const int stp = has_top ? (jt % K) * T + jt / K : s;
         const uint32_t cb = f.cls8[q];
-        const float ax = f.ctab[64 + (cb & kClassMask)], ay = f.ctab[128 + (cb & kClassMask)];
+        const float axe = f.ctab[kFastClasses + cb], aye = f.ctab[2 * kFastClasses + cb];
+        const float ax = fabsf(axe), ay = fabsf(aye);
+        const bool soff = flag_off(axe), roff = flag_off(aye);
         const float gx = f.gX[q], gy = f.gY[q];
-        const bool selfoff = (cb & kClassSelfOff) != 0, srcx = (cb & kClassSrcX) != 0;
         float lx = st0[s], ly = st0[KT + s];
         double sx = 0.0, sy = 0.0, sp = 0.0;
 #pragma unroll 2
@@ -588,8 +594,8 @@ void k_fast_adj_sums(const TubeDesc* __restrict__ tubes, const int* __restrict__
             const int o = (n - s0) * 2 * KT, oa = (n - s0) * KT;
             const float p = R[o + s], pr = R[o + sr], pt = has_top ? R[o + stp] : 0.f, dv = R[o + KT + s];
             const float kc = A[oa + s], kr = A[oa + sr], kt = has_top ? A[oa + stp] : 0.f;
-            const float ps = selfoff ? 0.f : p;
-            const float prx = srcx ? -src[n] : pr;
+            const float ps = soff ? 0.f : p;
+            const float prx = roff ? -src[n] : pr;
             const float fx = ps - prx, fy = ps - pt;
             sx += (double)lx * (double)fx;
             sy += (double)ly * (double)fy;
@@ -668,7 +674,9 @@ int fast_cells_per_lane(int max_padded_cells, int threads) {
 
 int fast_guard_lanes(int K, int max_padded_width) { return (max_padded_width + K - 1) / K + 1; }
 
-size_t fast_lds_static_bytes() { return sizeof(TwoMassState) + 64 * sizeof(float2) + 64 * sizeof(float) + 2 * sizeof(double) + 64; }
+size_t fast_lds_static_bytes() {
+    return sizeof(TwoMassState) + kFastClasses * (sizeof(float2) + sizeof(float)) + 2 * sizeof(double) + 64;
+}
 
 hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<float> b,
                              FastBufs f, hipStream_t st) {
