@@ -547,9 +547,9 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
     }
 }
 
-// Per-cell sums over the steps of one block.  Grid (gx, tubes): the gx
-// work-groups of a tube stride over its slots s = k * T + t (real cells
-// only), so the slab bases are work-group uniform.  From the recorded P' and
+// Per-cell sums over the steps of one block.  gx work-groups per tube
+// stride over its slots s = k * T + t (real cells only), so the slab bases
+// are work-group uniform.  From the recorded P' and
 // divergence and the reverse pass's Kp, each thread rebuilds, step by step
 // backwards:
 //   the forward factors  fx = s P' - (r P'_right or -ev),  fy = s P' - P'_top
@@ -559,11 +559,18 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
 // and adds Lx fx, Ly fy, -Kp div (double) to the cell-layout log-derivatives.
 // Neighbour values come from the same slab step (L2-resident).
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
-void k_fast_adj_sums(const TubeDesc* __restrict__ tubes, const int* __restrict__ order, int K, FastBufs f,
-                     FastAdj a, const float* __restrict__ source, int s0, int ns) {
+void k_fast_adj_sums(const TubeDesc* __restrict__ tubes, const int* __restrict__ order, int ntubes, int K,
+                     FastBufs f, FastAdj a, const float* __restrict__ source, int s0, int ns) {
 #pragma clang fp contract(fast)
     const int T = a.lanes, KT = K * T;
-    const int b = blockIdx.y;
+    // Work-groups are dealt to the 8 XCDs round-robin by linear id; the gx
+    // work-groups of tube b get ids w = b % 8 + 8 (x + gx (b / 8)), so all of
+    // them share one XCD and its L2, where a thread's right and top
+    // neighbour values are the lines its neighbours' threads just loaded.
+    const int gx = gridDim.x / (8 * ((ntubes + 7) / 8));
+    const int w = blockIdx.x;
+    const int b = (w & 7) + 8 * (w / (8 * gx)), x = (w >> 3) % gx;
+    if (b >= ntubes) return;
     const TubeDesc& D = tubes[order[b]];
     const int W = D.W, Wp = W - 1, Wi = W - 2, Hi = D.H - 2;
     const int tend = min(s0 + ns, D.num_steps);
@@ -571,7 +578,7 @@ void k_fast_adj_sums(const TubeDesc* __restrict__ tubes, const int* __restrict__
     const float* const A = a.adj + b * a.adj_tube;
     const float* const st0 = a.start + (long long)b * 2 * KT;
     const float* const src = source + D.src_off;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < KT; s += gridDim.x * blockDim.x) {
+    for (int s = x * blockDim.x + threadIdx.x; s < KT; s += gx * blockDim.x) {
         const int k = s / T, t = s - (s / T) * T;
         const int j = t * K + k;
         const int r = j / Wp, c = j - (j / Wp) * Wp;
@@ -589,11 +596,16 @@ void k_fast_adj_sums(const TubeDesc* __restrict__ tubes, const int* __restrict__
         const float gx = f.gX[q], gy = f.gY[q];
         float lx = st0[s], ly = st0[KT + s];
         double sx = 0.0, sy = 0.0, sp = 0.0;
+        // step rows walked backwards from uniform bases; the per-thread
+        // offsets (s, sr, stp) stay fixed, and the top loads are always in
+        // bounds (stp = s without a top neighbour)
+        const float* Rn = R + (long long)(tend - 1 - s0) * 2 * KT;
+        const float* An = A + (long long)(tend - 1 - s0) * KT;
 #pragma unroll 2
-        for (int n = tend - 1; n >= s0; --n) {
-            const int o = (n - s0) * 2 * KT, oa = (n - s0) * KT;
-            const float p = R[o + s], pr = R[o + sr], pt = has_top ? R[o + stp] : 0.f, dv = R[o + KT + s];
-            const float kc = A[oa + s], kr = A[oa + sr], kt = has_top ? A[oa + stp] : 0.f;
+        for (int n = tend - 1; n >= s0; --n, Rn -= 2 * KT, An -= KT) {
+            const float p = Rn[s], pr = Rn[sr], ptr = Rn[stp], dv = Rn[KT + s];
+            const float kc = An[s], kr = An[sr], ktr = An[stp];
+            const float pt = has_top ? ptr : 0.f, kt = has_top ? ktr : 0.f;
             const float ps = soff ? 0.f : p;
             const float prx = roff ? -src[n] : pr;
             const float fx = ps - prx, fy = ps - pt;
@@ -739,8 +751,9 @@ hipError_t launch_fast_reverse(int K, const TubeDesc* d_tubes, const int* d_orde
 
 hipError_t launch_fast_adj_sums(int K, const TubeDesc* d_tubes, const int* d_order, int n, FastBufs f, FastAdj a,
                                 const float* source, int s0, int ns, int groups, hipStream_t st) {
-    hipLaunchKernelGGL(k_fast_adj_sums, dim3(std::max(1, groups / n), n), dim3(1024), 0, st, d_tubes, d_order, K, f,
-                       a, source, s0, ns);
+    const int gx = std::max(1, groups / n);
+    hipLaunchKernelGGL(k_fast_adj_sums, dim3(8 * ((n + 7) / 8) * gx), dim3(1024), 0, st, d_tubes, d_order, n, K, f, a,
+                       source, s0, ns);
     return hipGetLastError();
 }
 
