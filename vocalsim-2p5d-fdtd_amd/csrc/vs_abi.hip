@@ -13,6 +13,8 @@
 #include <numeric>
 #include <string>
 #include <vector>
+#include <mutex>
+#include <array>
 
 namespace vs {
 namespace {
@@ -326,6 +328,7 @@ struct vs_handle {
     bool adj_onchip = false;   // the held forward pass ran on chip
     // on-chip FP32 reverse pass: lanes shape, slabs and scratch
     int adj_K = 0, adj_G = 0, adj_B = 0, adj_fast_S = 0, adj_sum_groups = 0;
+    int adj_fwd_threads = 0;   // work-group size of the checkpointing forward (timed once)
     FastAdj fa{};
     float* adj_rec[2] = {nullptr, nullptr};   // double-buffered record slabs
     float* adj_adj[2] = {nullptr, nullptr};
@@ -1198,6 +1201,7 @@ struct TuneKey {
     }
 };
 std::vector<std::pair<TuneKey, std::pair<int, int>>> g_tune_cache;
+std::mutex g_tune_mutex;
 }  // namespace
 
 int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) {
@@ -1206,6 +1210,7 @@ int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) 
     if (timed_steps <= 0) timed_steps = 64;
     const TuneKey key{h->n, h->max_interior, h->precision, h->kernel, h->cells_per_step};
     std::string rep = "{\"candidates\": [";
+    std::unique_lock<std::mutex> cache_lock(g_tune_mutex);
     for (auto& kv : g_tune_cache)
         if (kv.first == key) {
             h->threads = kv.second.first;
@@ -1228,6 +1233,7 @@ int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) 
     } else {
         threads = {1024, 768, 512, 256};
     }
+    cache_lock.unlock();
     if (threads.empty()) return fail(VS_E_INVALID, "no work-group shape fits this batch");
     const int spls[3] = {128, 256, 512};
     const int warm = 16;
@@ -1265,7 +1271,10 @@ int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) 
     h->timing = old_timing;
     h->use_graph = old_graph;
     const int rc = vs_reset(h);
-    g_tune_cache.push_back({key, {best_t, best_s}});
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        g_tune_cache.push_back({key, {best_t, best_s}});
+    }
     rep += "], \"threads\": " + std::to_string(best_t) + ", \"steps_per_launch\": " + std::to_string(best_s) +
            ", \"us_per_step\": " + std::to_string(best) + "}";
     if (report && report_len) std::snprintf(report, report_len, "%s", rep.c_str());
@@ -1533,11 +1542,71 @@ int adj_check(vs_handle* h) {
     return VS_OK;
 }
 
+// Work-group size of the checkpointing forward.  Its state is kept in cell
+// layout, so it need not share the lanes shape of the recording and reverse
+// launches; the fastest of the shapes that fit is timed once per handle (64
+// steps each, on the adjoint's own state, which the forward pass then clears).
+// (batch size, largest padded interior, widest padded row) -> forward shape,
+// shared by the handles of one process (a differentiable call builds a new
+// handle per call).
+std::vector<std::pair<std::array<int, 3>, int>> g_adj_shape_cache;
+std::mutex g_adj_shape_mutex;
+
+int adj_pick_forward_threads(vs_handle* h) {
+    if (h->adj_fwd_threads) return VS_OK;
+    const std::array<int, 3> key{h->n, h->max_padded, h->max_wp};
+    {
+        std::lock_guard<std::mutex> lk(g_adj_shape_mutex);
+        for (auto& kv : g_adj_shape_cache)
+            if (kv.first == key) {
+                h->adj_fwd_threads = kv.second;
+                return VS_OK;
+            }
+    }
+    const Bufs<float> b = adj_fast_bufs(h);
+    const FastBufs f = adj_fast_flux(h);
+    float best = 1e30f;
+    int best_t = kAdjThreads;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    VS_HIP(hipEventCreate(&e0));
+    VS_HIP(hipEventCreate(&e1));
+    int rc = VS_OK;
+    for (int t = kAdjThreads; t >= kAdjThreads - 4 * 64 && rc == VS_OK; t -= 64) {
+        int K = 0, G = 0;
+        if (!fast_fits(h, t, &K, &G)) continue;
+        hipError_t e = configure_fast(K);
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+            e = hipEventRecord(e0, h->stream);
+            if (e == hipSuccess) e = launch_fast_step(K, h->d_desc, h->d_order, h->n, b, f, 0, 64, t, G, nullptr, h->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float m = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
+            if (e == hipSuccess) ms = std::min(ms, m);
+        }
+        if (e != hipSuccess) rc = fail(VS_E_DEVICE, std::string("adjoint forward shape: ") + hipGetErrorString(e));
+        else if (ms < best) { best = ms; best_t = t; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == VS_OK) {
+        h->adj_fwd_threads = best_t;
+        std::lock_guard<std::mutex> lk(g_adj_shape_mutex);
+        g_adj_shape_cache.push_back({key, best_t});
+    }
+    return rc;
+}
+
 // Forward pass on chip: the FP32 flux-form step kernel in S-step launches,
 // the whole state copied to a checkpoint before each.
 int adj_forward_fast(vs_handle* h, int S, double* probe) {
     int rc;
-    if ((rc = adj_cells<float>(h)) != VS_OK || (rc = adj_fast_prepare(h, S)) != VS_OK) return rc;
+    if ((rc = adj_cells<float>(h)) != VS_OK || (rc = adj_fast_prepare(h, S)) != VS_OK ||
+        (rc = adj_pick_forward_threads(h)) != VS_OK)
+        return rc;
+    int FK = 0, FG = 0;
+    if (!fast_fits(h, h->adj_fwd_threads, &FK, &FG)) return fail(VS_E_INVALID, "forward shape no longer fits");
     AdjBufs<float>& a = h->af;
     const size_t cb = sizeof(float) * (size_t)h->total_cells;
     VS_HIP(hipMemsetAsync(a.P, 0, cb, h->stream));
@@ -1548,7 +1617,7 @@ int adj_forward_fast(vs_handle* h, int S, double* probe) {
     for (int t0 = 0; t0 < h->max_steps; t0 += S) {
         if ((rc = adj_state_copy(h, h->adj_ck + (long long)(t0 / S) * 3 * h->total_cells, true, h->stream)) != VS_OK)
             return rc;
-        hipError_t e = launch_fast_step(h->adj_K, h->d_desc, h->d_order, h->n, b, f, t0, S, kAdjThreads, h->adj_G,
+        hipError_t e = launch_fast_step(FK, h->d_desc, h->d_order, h->n, b, f, t0, S, h->adj_fwd_threads, FG,
                                         nullptr, h->stream);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint forward: ") + hipGetErrorString(e));
     }
