@@ -125,9 +125,14 @@ enum vs_option {
     VS_OPT_ADJOINT_KERNEL = 6,   /* FP32 reverse-time pass: 0 (default) on-chip kernels when every
                                     tube fits one work-group, 1: per-step global-memory kernels.
                                     FP64 always uses the per-step kernels. */
-    VS_OPT_ADJOINT_BLOCK = 7     /* on-chip reverse pass: most steps recorded per block (0, the
-                                    default: as many of the checkpoint spacing as fit in 16 GiB);
-                                    shorter blocks trade one extra forward pass for memory */
+    VS_OPT_ADJOINT_BLOCK = 7,    /* on-chip reverse pass: most steps recorded per block (0, the
+                                    default: as many of the checkpoint spacing as fit in half the
+                                    free device memory); shorter blocks trade one extra forward
+                                    pass for memory */
+    VS_OPT_EXACT_GROUPS = 8      /* exact step of a single-tube handle: work-groups of one
+                                    cooperative launch, the tube's cells split across them
+                                    (bit-identical results).  0, the default: one per ~1000
+                                    interior cells, at most 16; 1: one work-group; 2..64 */
 };
 
 enum vs_kernel {

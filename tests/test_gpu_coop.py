@@ -1,0 +1,59 @@
+"""Exact step of a single tube spread over several work-groups of one
+cooperative launch (VS_OPT_EXACT_GROUPS): the per-cell code is the same as
+the one-work-group kernel, so probes and whole fields must be bit-identical
+for any group count, in FP64 and FP32, with Gaussian and two-mass sources,
+and the FP64 run must equal the CPU oracle exactly."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from vocalsim import Solver, _abi, tubes
+
+pytestmark = pytest.mark.gpu
+
+
+def run(spec, precision, groups, threads=1024, steps=None, blocks=1):
+    with Solver(spec, precision=precision, threads=threads, steps_per_launch=max(1, (steps or spec.num_steps) // blocks)) as s:
+        s.set_option(_abi.OPT_EXACT_GROUPS, groups)
+        s.run(steps)
+        return s.probe(), s.field("pressure"), s.field("vx"), s.field("vy")
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("source", [tubes.SOURCE_GAUSSIAN, tubes.SOURCE_TWO_MASS])
+def test_groups_are_bit_identical(hip_device, precision, source):
+    spec = tubes.vowel_tube(2, num_steps=900, source=source)
+    ref = run(spec, precision, 1)
+    for groups in (2, 3, 8, 16):
+        got = run(spec, precision, groups, blocks=3)
+        print(f"[coop] precision={precision} source={source} groups={groups}", flush=True)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+
+
+def test_groups_match_the_oracle_fp64(hip_device):
+    spec = tubes.vowel_tube(5, num_steps=600, source=tubes.SOURCE_TWO_MASS)
+    probe, P, _, _ = run(spec, 64, 8, threads=512)
+    p_ref, _ = orc.OracleTube(spec, 64).run()
+    print(f"[coop] vs oracle max |diff| = {np.abs(probe - p_ref).max():.3e}", flush=True)
+    assert np.array_equal(probe, p_ref)
+
+
+def test_batches_keep_one_group_per_tube(hip_device):
+    a = tubes.vowel_tube(1, num_steps=400, source=tubes.SOURCE_GAUSSIAN)
+    b = tubes.vowel_tube(3, num_steps=400, source=tubes.SOURCE_GAUSSIAN)
+    with Solver([a, b], precision=64) as s:
+        s.set_option(_abi.OPT_EXACT_GROUPS, 8)
+        s.run()
+        pa, pb = s.probe(0), s.probe(1)
+    assert np.array_equal(pa, run(a, 64, 1, threads=256)[0])
+    assert np.array_equal(pb, run(b, 64, 1, threads=256)[0])
+
+
+def test_option_range(hip_device):
+    spec = tubes.vowel_tube(0, num_steps=10)
+    with Solver(spec, precision=64) as s:
+        assert s.get_option(_abi.OPT_EXACT_GROUPS) == 0
+        for bad in (-1, 65):
+            with pytest.raises(Exception):
+                s.set_option(_abi.OPT_EXACT_GROUPS, bad)

@@ -324,6 +324,10 @@ struct vs_handle {
     bool adj_forward = false;  // checkpoints hold a completed forward pass
     std::vector<void*> adj_slabs;
     int adj_kernel = 0;        // VS_OPT_ADJOINT_KERNEL
+    int exact_groups = 0;      // VS_OPT_EXACT_GROUPS (0: one per ~1000 cells of a single tube, at most 16)
+    unsigned* d_bar = nullptr; // grid barrier state of the cooperative exact step
+    void* d_state2 = nullptr;  // its second state buffers: P, Vx, Vy (Real, total_cells each)
+    int coop_ok = -1;          // device supports cooperative launches (-1: not asked yet)
     int adj_block = 0;         // VS_OPT_ADJOINT_BLOCK
     bool adj_onchip = false;   // the held forward pass ran on chip
     // on-chip FP32 reverse pass: lanes shape, slabs and scratch
@@ -675,9 +679,48 @@ int prepare_fast(vs_handle* h, int threads) {
 
 template <typename Real>
 hipError_t launch_any(vs_handle* h, Bufs<Real>& b, int t0, int ns, const int* d_t0 = nullptr);
+// The exact step of a single tube over several work-groups (one cooperative
+// launch) when VS_OPT_EXACT_GROUPS asks for it; graph replays keep the
+// one-work-group kernel (the results are bit-identical either way).
+// Work-groups of the cooperative exact step for this handle (1: none).
+int coop_groups(const vs_handle* h) {
+    if (h->exact_groups > 0) return h->exact_groups;
+    const int cells = (h->desc[0].H - 2) * (h->desc[0].W - 2);
+    return std::max(1, std::min(16, (cells + 500) / 1000));
+}
+template <typename Real>
+bool use_coop(vs_handle* h, const int* d_t0) {
+    if (h->n != 1 || d_t0 || coop_groups(h) < 2 || h->desc[0].W - 2 > kCoopMaxWidth) return false;
+    if (h->coop_ok < 0) {
+        int dev = 0, v = 0;
+        h->coop_ok = (hipGetDevice(&dev) == hipSuccess &&
+                      hipDeviceGetAttribute(&v, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess && v) ? 1 : 0;
+    }
+    return h->coop_ok == 1;
+}
+template <typename Real>
+hipError_t launch_exact(vs_handle* h, Bufs<Real>& b, int t0, int ns, const int* d_t0) {
+    if (use_coop<Real>(h, d_t0)) {
+        if (!h->d_bar) {
+            Real* st2 = nullptr;
+            if (h->alloc(&h->d_bar, 2) != VS_OK || h->alloc(&st2, 3 * (size_t)h->total_cells) != VS_OK)
+                return hipErrorOutOfMemory;
+            h->d_state2 = st2;
+        }
+        Real* st2 = static_cast<Real*>(h->d_state2);
+        const long long C = h->total_cells;
+        hipError_t e = launch_step_coop<Real>(h->d_desc, h->d_order, b, t0, ns, h->threads, coop_groups(h), h->d_bar,
+                                              st2, st2 + C, st2 + 2 * C, h->stream);
+        if (e != hipErrorCooperativeLaunchTooLarge) return e;
+        // the device cannot hold the groups at once: one work-group from now on
+        (void)hipGetLastError();
+        h->coop_ok = 0;
+    }
+    return launch_step<Real>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, d_t0, h->stream);
+}
 template <>
 hipError_t launch_any<double>(vs_handle* h, Bufs<double>& b, int t0, int ns, const int* d_t0) {
-    return launch_step<double>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, d_t0, h->stream);
+    return launch_exact<double>(h, b, t0, ns, d_t0);
 }
 template <>
 hipError_t launch_any<float>(vs_handle* h, Bufs<float>& b, int t0, int ns, const int* d_t0) {
@@ -687,7 +730,7 @@ hipError_t launch_any<float>(vs_handle* h, Bufs<float>& b, int t0, int ns, const
         return launch_fast_step(K, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, G, d_t0, h->stream);
     }
     h->flux_valid = false;
-    return launch_step<float>(h->d_desc, h->d_order, h->n, b, t0, ns, h->threads, h->kernel, d_t0, h->stream);
+    return launch_exact<float>(h, b, t0, ns, d_t0);
 }
 
 // Captures G step blocks (step kernel + counter bump) into one graph.
@@ -1027,6 +1070,16 @@ int vs_run(vs_handle* h, int n_steps) {
 int vs_sync(vs_handle* h) {
     if (check_handle(h)) return VS_E_INVALID;
     VS_HIP(hipStreamSynchronize(h->stream));
+    if (h->d_bar) {
+        unsigned* st = h->precision == 64 ? h->bd.status : h->bf.status;
+        unsigned status = 0;
+        VS_HIP(hipMemcpy(&status, st, sizeof(unsigned), hipMemcpyDeviceToHost));
+        if (status & kStatusBarrierTimeout) {
+            const unsigned clear = status & ~kStatusBarrierTimeout;
+            VS_HIP(hipMemcpy(st, &clear, sizeof(unsigned), hipMemcpyHostToDevice));
+            return fail(VS_E_DEVICE, "cooperative exact step: a grid barrier timed out (results invalid)");
+        }
+    }
     float ms = 0;
     VS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_ms = ms;
@@ -1100,6 +1153,9 @@ int vs_set_option(vs_handle* h, int option, int value) {
         if (value != 0 && value != 1) return fail(VS_E_INVALID, "adjoint kernel must be 0 (on chip) or 1 (per step)");
         if (value != h->adj_kernel) h->adj_forward = false;
         h->adj_kernel = value; return VS_OK;
+    case VS_OPT_EXACT_GROUPS:
+        if (value < 0 || value > 64) return fail(VS_E_INVALID, "exact groups must be in [0, 64]");
+        h->exact_groups = value; return VS_OK;
     case VS_OPT_ADJOINT_BLOCK:
         if (value < 0) return fail(VS_E_INVALID, "adjoint block must be >= 0");
         if (value != h->adj_block) {   // slabs are re-sized by the next forward pass
@@ -1129,6 +1185,7 @@ int vs_get_option(const vs_handle* h, int option, int* value) {
     case VS_OPT_ENGINE_VARIANT: *value = h->desc.empty() ? 0 : h->desc[0].variant; return VS_OK;
     case VS_OPT_ADJOINT_KERNEL: *value = h->adj_kernel; return VS_OK;
     case VS_OPT_ADJOINT_BLOCK: *value = h->adj_block; return VS_OK;
+    case VS_OPT_EXACT_GROUPS: *value = h->exact_groups; return VS_OK;
     default: return fail(VS_E_INVALID, "unknown option");
     }
 }

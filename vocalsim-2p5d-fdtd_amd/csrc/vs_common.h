@@ -22,6 +22,7 @@ constexpr uint8_t kSegWallBit = 0x10;
 // Status bits raised by device code (read back after setup / each run).
 constexpr unsigned kStatusSqrtDomain = 1u;   // negative depth sqrt argument
 constexpr unsigned kStatusNonFinite = 2u;    // non-finite probe sample
+constexpr unsigned kStatusBarrierTimeout = 4u;   // a cooperative step's grid barrier gave up waiting
 
 // Physics constants and per-type tables of one tube, stored in double.
 // FP32 tubes hold the float32-evaluated values widened exactly.
@@ -316,6 +317,13 @@ hipError_t launch_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n,
 template <typename Real>
 hipError_t launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<Real> b,
                        int t0, int nsteps, int threads, int kernel, const int* d_t0, hipStream_t st);
+// Exact step of a single tube over `groups` work-groups of one cooperative
+// launch (bar: two words of barrier state, zeroed by the launch; P2, Vx2,
+// Vy2: second state buffers, cell layout, zero frame ring).
+template <typename Real>
+hipError_t launch_step_coop(const TubeDesc* d_tubes, const int* d_order, Bufs<Real> b, int t0, int nsteps,
+                            int threads, int groups, unsigned* bar, Real* P2, Real* Vx2, Real* Vy2, hipStream_t st);
+constexpr int kCoopMaxWidth = 1024;   // widest interior row (W - 2) the cooperative step accepts
 template <typename Real>
 hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tube, Bufs<Real> b,
                          Real* out17, hipStream_t st);

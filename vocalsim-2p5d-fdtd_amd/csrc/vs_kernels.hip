@@ -362,6 +362,94 @@ __global__ void k_exact_classes(const TubeDesc* __restrict__ tubes, Bufs<Real> b
 // (table lookup or the two-mass recurrence in double, fed by the previous
 // step's P at the excitation cell) and records the probe.
 // ---------------------------------------------------------------------------
+// One tube's view for the two phases of the exact step.
+template <typename Real>
+struct ExactView {
+    Real* __restrict__ P;
+    Real* __restrict__ Vx;
+    Real* __restrict__ Vy;
+    const Real* __restrict__ dP;
+    const Real* __restrict__ dX;
+    const Real* __restrict__ dY;
+    const Real* __restrict__ zc;
+    const uint8_t* __restrict__ xc;
+    const Real* ctab;   // this tube's class coefficients (LDS)
+    int W, Wi;
+    Real K1;
+    bool dirichlet, eigen;
+};
+
+// New pressure of frame cell q (FDTDEngineCPU.cpp:316-339), from the state
+// in x (P, Vx, Vy of the step's input).
+template <typename Real>
+__device__ __forceinline__ Real exact_pressure_cell(const ExactView<Real>& x, int q) {
+    const int W = x.W;
+    const Real cxvx = x.Vx[q] * x.dX[q] - x.Vx[q - 1] * x.dX[q - 1];
+    const Real cyvy = x.Vy[q] * x.dY[q] - x.Vy[q + W] * x.dY[q + W];
+    const Real num = (x.P[q] * x.dP[q]) - (x.K1 * (cxvx + cyvy));
+    const Real* o = x.ctab + x.xc[q] * kExactSlots;
+    const Real sp = o[8];   // sigma'dt of the cell's own type
+    const Real den = x.eigen ? (sp * x.dP[q] + x.dP[q]) : ((1 + sp) * x.dP[q]);
+    Real v = num / den;
+    if (x.dirichlet && o[17] != 0) v = 0;
+    return v;
+}
+
+// New velocities of frame cell q in column c (FDTDEngineCPU.cpp:364-410),
+// from its input velocities in x and the new pressures pc (own), pr (right)
+// and pt (top); stored to vx_out, vy_out.
+template <typename Real>
+__device__ __forceinline__ void exact_velocity_cell(const ExactView<Real>& x, int q, int c, Real pc, Real pr, Real pt,
+                                                    Real ev, Real* vx_out, Real* vy_out) {
+    const Real* o = x.ctab + x.xc[q] * kExactSlots;
+    const Real z = (o[18] != 0) ? x.zc[c] : (Real)0;
+    const Real cxp = pr - pc;
+    const Real cyp = pt - pc;
+    Real vx = (o[2] * x.Vx[q]) - (o[6] * cxp);
+    Real vy = (o[3] * x.Vy[q]) - (o[7] * cyp);
+    vx = vx + (ev * o[11] * o[4]);
+    vy = vy + (ev * o[12] * o[5]);
+    Real bx = (o[14] * pc * o[0]) - (o[13] * pr * o[1]);
+    Real by = (o[16] * pc * o[0]) - (o[15] * pt * o[1]);
+    bx = bx * o[9] * z;
+    by = by * o[10] * z;
+    vx += (o[4] * bx);
+    vy += (o[5] * by);
+    vx /= (o[2] + o[4]);
+    vy /= (o[3] + o[5]);
+    *vx_out = vx;
+    *vy_out = vy;
+}
+
+// Pressure phase over the interior cells i = i0, i0 + B, ... < i1
+// (row-major, frame cell (r, c) of cell i0 given), updating P in place: a
+// cell's pressure update reads only its own P.
+template <typename Real>
+__device__ __forceinline__ void exact_pressure(const ExactView<Real>& x, int i0, int i1, int B, int r, int c,
+                                               int dr, int dc) {
+    const int W = x.W, Wi = x.Wi;
+    for (int i = i0; i < i1; i += B) {
+        const int q = r * W + c;
+        x.P[q] = exact_pressure_cell(x, q);
+        r += dr; c += dc;
+        if (c > Wi) { c -= Wi; ++r; }
+    }
+}
+
+// Velocity phase over the same cells, in place: a cell's velocity update
+// reads only its own V.
+template <typename Real>
+__device__ __forceinline__ void exact_velocity(const ExactView<Real>& x, Real ev, int i0, int i1, int B, int r,
+                                               int c, int dr, int dc) {
+    const int W = x.W, Wi = x.Wi;
+    for (int i = i0; i < i1; i += B) {
+        const int q = r * W + c;
+        exact_velocity_cell(x, q, c, x.P[q], x.P[q + 1], x.P[q - W], ev, &x.Vx[q], &x.Vy[q]);
+        r += dr; c += dc;
+        if (c > Wi) { c -= Wi; ++r; }
+    }
+}
+
 template <typename Real>
 __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict__ tubes,
                                                      const int* __restrict__ order, Bufs<Real> b,
@@ -400,8 +488,8 @@ __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict_
         if (twomass) tm = b.tm[tube];
     }
     __syncthreads();
-    const Real K1 = (Real)T.ph.K1;
-    const uint8_t* __restrict__ xc = b.xcls + T.cell_off;
+    const ExactView<Real> x{P, Vx, Vy, dP, dX, dY, zc, b.xcls + T.cell_off, ctab, W, Wi, (Real)T.ph.K1,
+                            dirichlet, eigen};
     // per-thread walk over interior cells i = tid + k * B in row-major order
     const int dr = B / Wi, dc = B - dr * Wi;
     const int r0 = tid / Wi, c0 = tid - r0 * Wi;
@@ -413,58 +501,13 @@ __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict_
             s_exeV = (Real)e;
             strace[t] = (double)(Real)e;
         }
-        // ---- pressure phase (FDTDEngineCPU.cpp:316-339) ----
-        {
-            int r = r0 + 1, c = c0 + 1;
-            for (int i = tid; i < NI; i += B) {
-                const int q = r * W + c;
-                const Real cxvx = Vx[q] * dX[q] - Vx[q - 1] * dX[q - 1];
-                const Real cyvy = Vy[q] * dY[q] - Vy[q + W] * dY[q + W];
-                const Real num = (P[q] * dP[q]) - (K1 * (cxvx + cyvy));
-                const Real* o = ctab + xc[q] * kExactSlots;
-                const Real sp = o[8];   // sigma'dt of the cell's own type
-                const Real den = eigen ? (sp * dP[q] + dP[q]) : ((1 + sp) * dP[q]);
-                Real v = num / den;
-                if (dirichlet && o[17] != 0) v = 0;
-                P[q] = v;
-                r += dr; c += dc;
-                if (c > Wi) { c -= Wi; ++r; }
-            }
-        }
+        exact_pressure(x, tid, NI, B, r0 + 1, c0 + 1, dr, dc);
         __syncthreads();
         if (tid == 0) {
             probe[t] = (double)P[T.listener];
             pexc = (double)P[T.excitation];
         }
-        const Real ev = s_exeV;
-        // ---- velocity phase (FDTDEngineCPU.cpp:364-410) ----
-        {
-            int r = r0 + 1, c = c0 + 1;
-            for (int i = tid; i < NI; i += B) {
-                const int q = r * W + c;
-                const Real* o = ctab + xc[q] * kExactSlots;
-                const Real z = (o[18] != 0) ? zc[c] : (Real)0;
-                const Real pc = P[q], pr = P[q + 1], pt = P[q - W];
-                const Real cxp = pr - pc;
-                const Real cyp = pt - pc;
-                Real vx = (o[2] * Vx[q]) - (o[6] * cxp);
-                Real vy = (o[3] * Vy[q]) - (o[7] * cyp);
-                vx = vx + (ev * o[11] * o[4]);
-                vy = vy + (ev * o[12] * o[5]);
-                Real bx = (o[14] * pc * o[0]) - (o[13] * pr * o[1]);
-                Real by = (o[16] * pc * o[0]) - (o[15] * pt * o[1]);
-                bx = bx * o[9] * z;
-                by = by * o[10] * z;
-                vx += (o[4] * bx);
-                vy += (o[5] * by);
-                vx /= (o[2] + o[4]);
-                vy /= (o[3] + o[5]);
-                Vx[q] = vx;
-                Vy[q] = vy;
-                r += dr; c += dc;
-                if (c > Wi) { c -= Wi; ++r; }
-            }
-        }
+        exact_velocity(x, s_exeV, tid, NI, B, r0 + 1, c0 + 1, dr, dc);
         __syncthreads();
     }
     if (tid == 0) {
@@ -473,7 +516,186 @@ __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict_
     }
 }
 
+// Barrier across the work-groups of one cooperative launch.  bar[0] counts
+// arrivals, bar[1] is the generation; the last arrival resets the count and
+// advances the generation.  Ordering comes from agent-scope fences, one of
+// each kind per work-group and barrier (each costs an L2 write-back or
+// invalidate): a release fence before arriving, an acquire fence after
+// leaving, and the last arrival's acquire before it releases the rest, so
+// every work-group's writes before the barrier happen before every read
+// after it.  Polling uses relaxed loads.  The wait is bounded: past
+// spin_limit polls (or once any work-group has timed out) it raises
+// kStatusBarrierTimeout and returns, so every wave exits.
+__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned groups, int spin_limit, unsigned* status) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned* count = bar;
+        unsigned* gen = bar + 1;
+        const unsigned my_gen = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == groups - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            for (int spins = 0; __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == my_gen;) {
+                if (++spins > spin_limit ||
+                    (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kStatusBarrierTimeout)) {
+                    atomicOr(status, kStatusBarrierTimeout);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+    }
+    __syncthreads();
+}
+
+// The exact step of one tube spread over gridDim.x work-groups of one
+// cooperative launch, with one grid barrier per step.  The state is double
+// buffered as in the reference (PV_N / PV_Nplus1): step t reads `cur` and
+// writes `nxt`, which swap after the barrier.  Work-group g owns the
+// contiguous interior cells [g chunk, (g + 1) chunk) and runs the per-cell
+// code of k_step_exact (bit-identical results).  Its velocity phase needs new
+// pressures of the row above its chunk and of the cell after its last one;
+// it computes those halo cells itself (same code, same inputs) into LDS, so
+// no barrier is needed between the phases.  Every work-group runs the
+// source on its own lane 0 (table lookup, or the two-mass recurrence fed by
+// the new pressure at the excitation cell after each barrier: identical
+// inputs, identical values); work-group 0 records the probe and the source
+// trace and keeps the final source state.
+constexpr int kCoopHalo = 1024;   // widest padded row a cooperative step supports (halo in LDS)
+
+template <typename Real>
+__global__ __launch_bounds__(1024) void k_step_exact_coop(const TubeDesc* __restrict__ tubes,
+                                                          const int* __restrict__ order, Bufs<Real> b, int t0,
+                                                          int nsteps, unsigned* bar, Real* P2, Real* Vx2, Real* Vy2,
+                                                          int spin_limit) {
+    const int tube = order[0];
+    const TubeDesc& T = tubes[tube];
+    __shared__ Real ctab[kExactClasses * kExactSlots];
+    __shared__ Real halo[kCoopHalo + 1];   // new P of cells i0 - Wi .. i0 - 1, then of cell i1
+    __shared__ TwoMassState tm;
+    __shared__ Real s_ev;
+    const int tid = threadIdx.x, B = blockDim.x, g = blockIdx.x;
+    const unsigned G = gridDim.x;
+    {
+        const Real* src = b.xtab + (long long)tube * kExactClasses * kExactSlots;
+        const int nv = b.xn[tube] * kExactSlots;
+        for (int i = tid; i < nv; i += B) ctab[i] = src[i];
+    }
+    const int H = T.H, W = T.W, Wi = W - 2, NI = (H - 2) * Wi;
+    const int tend = min(t0 + nsteps, T.num_steps);
+    const long long co = T.cell_off;
+    ExactView<Real> cur{b.P + co, b.Vx + co, b.Vy + co, b.dP + co, b.dX + co, b.dY + co, b.zcol + T.col_off,
+                        b.xcls + co, ctab, W, Wi, (Real)T.ph.K1, T.dirichlet != 0, T.variant == 1};
+    Real *nP = P2 + co, *nVx = Vx2 + co, *nVy = Vy2 + co;
+    const Real* __restrict__ src = b.source + T.src_off;
+    double* probe = b.probe + T.src_off;
+    double* strace = b.strace + T.src_off;
+    const bool twomass = T.source_type == 2;
+    double pexc = 0;
+    if (tid == 0) {
+        pexc = b.pexc[tube];
+        if (twomass) tm = b.tm[tube];
+    }
+    __syncthreads();
+    const int chunk = (NI + (int)G - 1) / (int)G;
+    const int c0i = g * chunk, c1i = min(NI, (g + 1) * chunk);   // this work-group's cells
+    const int hb = max(0, c0i - Wi);                                // first halo cell above
+    const int dr = B / Wi, dc = B - dr * Wi;
+    const int r0 = (c0i + tid) / Wi, cc0 = (c0i + tid) - r0 * Wi;
+    for (int t = t0; t < tend; ++t) {
+        if (tid == 0) {
+            double e;
+            if (twomass) { tm.p1 = pexc; e = twomass_step(tm, T.srate_mul); }
+            else e = (double)src[t];
+            s_ev = (Real)e;
+            if (g == 0) strace[t] = (double)(Real)e;
+        }
+        // pressure: own cells into nxt, halo cells into LDS
+        {
+            int r = r0 + 1, c = cc0 + 1;
+            for (int i = c0i + tid; i < c1i; i += B) {
+                const int q = r * W + c;
+                nP[q] = exact_pressure_cell(cur, q);
+                r += dr; c += dc;
+                if (c > Wi) { c -= Wi; ++r; }
+            }
+            for (int i = hb + tid; i < c0i; i += B) {
+                const int q = (i / Wi + 1) * W + (i - (i / Wi) * Wi) + 1;
+                halo[i - (c0i - Wi)] = exact_pressure_cell(cur, q);
+            }
+            if (tid == 0 && c1i < NI) {
+                const int q = (c1i / Wi + 1) * W + (c1i - (c1i / Wi) * Wi) + 1;
+                halo[kCoopHalo] = exact_pressure_cell(cur, q);
+            }
+        }
+        __syncthreads();
+        // velocity: own cells, new P from nxt (own cells), the LDS halo, or
+        // the zero frame ring
+        {
+            const Real ev = s_ev;
+            int r = r0 + 1, c = cc0 + 1;
+            for (int i = c0i + tid; i < c1i; i += B) {
+                const int q = r * W + c;
+                // the right / top neighbour is in nxt (this work-group's cells
+                // or the zero frame ring) or in the LDS halo; both loads are
+                // issued (the nxt index falls back to the cell itself when
+                // the halo holds the value) and the value is selected
+                const bool rg = c == Wi || i + 1 < c1i, tg = r == 1 || i - Wi >= c0i;
+                const Real pc = nP[q];
+                const Real prg = nP[rg ? q + 1 : q], prh = halo[kCoopHalo];
+                const Real ptg = nP[tg ? q - W : q], pth = halo[tg ? 0 : i - c0i];
+                const Real pr = rg ? prg : prh;
+                const Real pt = tg ? ptg : pth;
+                exact_velocity_cell(cur, q, c, pc, pr, pt, ev, &nVx[q], &nVy[q]);
+                r += dr; c += dc;
+                if (c > Wi) { c -= Wi; ++r; }
+            }
+        }
+        grid_barrier(bar, G, spin_limit, b.status);
+        if (tid == 0) {
+            pexc = (double)nP[T.excitation];
+            if (g == 0) probe[t] = (double)nP[T.listener];
+        }
+        Real* tp = cur.P; cur.P = nP; nP = tp;
+        tp = cur.Vx; cur.Vx = nVx; nVx = tp;
+        tp = cur.Vy; cur.Vy = nVy; nVy = tp;
+    }
+    // the final state is in cur; bring it home if that is the second buffer
+    if (((tend - t0) & 1) && tend > t0) {
+        Real *hP = b.P + co, *hVx = b.Vx + co, *hVy = b.Vy + co;
+        int r = r0 + 1, c = cc0 + 1;
+        for (int i = c0i + tid; i < c1i; i += B) {
+            const int q = r * W + c;
+            hP[q] = cur.P[q];
+            hVx[q] = cur.Vx[q];
+            hVy[q] = cur.Vy[q];
+            r += dr; c += dc;
+            if (c > Wi) { c -= Wi; ++r; }
+        }
+    }
+    if (g == 0 && tid == 0) {
+        b.pexc[tube] = pexc;
+        if (twomass) b.tm[tube] = tm;
+    }
+}
+
 }  // namespace
+
+template <typename Real>
+hipError_t launch_step_coop(const TubeDesc* d_tubes, const int* d_order, Bufs<Real> b, int t0, int nsteps,
+                            int threads, int groups, unsigned* bar, Real* P2, Real* Vx2, Real* Vy2, hipStream_t st) {
+    int spin_limit = 1 << 22;   // about a quarter of a second of polling
+    void* args[] = {(void*)&d_tubes, (void*)&d_order, (void*)&b, (void*)&t0, (void*)&nsteps,
+                    (void*)&bar, (void*)&P2, (void*)&Vx2, (void*)&Vy2, (void*)&spin_limit};
+    hipError_t e = hipMemsetAsync(bar, 0, 2 * sizeof(unsigned), st);
+    if (e != hipSuccess) return e;
+    return hipLaunchCooperativeKernel((const void*)k_step_exact_coop<Real>, dim3(groups), dim3(threads), args, 0, st);
+}
 
 template <typename Real>
 hipError_t launch_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, int n, Bufs<Real> b,
@@ -543,6 +765,8 @@ hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tu
                                            Real*, Real*, Real*, hipStream_t);                       \
     template hipError_t launch_step<Real>(const TubeDesc*, const int*, int, Bufs<Real>, int, int,    \
                                           int, int, const int*, hipStream_t);                       \
+    template hipError_t launch_step_coop<Real>(const TubeDesc*, const int*, Bufs<Real>, int, int, int, \
+                                               int, unsigned*, Real*, Real*, Real*, hipStream_t);  \
     template hipError_t launch_coeffs<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,       \
                                             Real*, hipStream_t);                                    \
     template hipError_t launch_depth_raw<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,    \
