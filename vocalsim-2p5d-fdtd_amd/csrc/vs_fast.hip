@@ -9,8 +9,9 @@
 //         distinct banks):  [ P[0..K) | Uy[0..K) | Ux of the run's last cell ]
 //         with G zero guard records on each side, so every neighbour read of
 //         a real cell lands on a real cell, a pad cell or a zero guard;
-//         static tables: (ax, ay) by damping-code pair, aP by damping code.
-//   VGPRs Ux[K], kP[K], gX[K], gY[K] and the packed per-cell codes.
+//         static tables by cell class: aP, and (ax, ay) with the two
+//         face-form flags in their sign bits.
+//   VGPRs Ux[K], kP[K], gX[K], gY[K] and the class bytes (4 per VGPR).
 //
 // Update in flux form (U = V * depth), algebraically equal to the reference
 // step (FDTDEngineCPU.cpp:316-405) with every per-cell coefficient
@@ -20,10 +21,14 @@
 //   Uy' = ay * Uy + gY * (P' - P'_top)
 // aP = m/(1+sigma_c), kP = m*K1/((1+sigma_c)*dP) (m = 0 on Dirichlet cells);
 // ax = mbx/(mbx+Mx), gX = dX*Cx/(mbx+Mx) (Cx = mbx^2 dt/(rho ds)), same in y.
-// Cells whose face terms differ from that form ("special": walls carrying an
-// admittance, whose face velocity is -z*Nin*P_neighbour, and the excitation
-// column, whose x-face carries the source velocity) override the result with
-// coefficients read from HBM.
+// Cells whose face terms differ from that form (walls carrying an admittance,
+// whose face velocity is -z*Nin*P_neighbour, and the excitation column, whose
+// x-face carries the source velocity) keep it with their own gX, gY and two
+// flags of their class: s = 0 drops the cell's own pressure from its faces,
+// r = 0 puts the source in place of the right neighbour's pressure.
+//
+// The same file holds the on-chip reverse pass of this step (the recording
+// forward launch, k_fast_reverse and k_fast_adj_sums).
 
 #include "vs_common.h"
 
@@ -31,11 +36,12 @@ namespace vs {
 
 namespace {
 
-// Per-cell code (16 bits): bits [3, 9) hold code_x + 8 code_y, which times 8
-// is the byte offset of (ax, ay) in the pair table; bits [9, 12) the pressure
-// damping code; kCodeSelfOff drops the cell's own pressure from its face
-// updates (walls carrying an admittance: U = -g * P_neighbour); kCodeSrcX
-// makes the x-face carry the source (excitation column: Ux = g * exeV).
+// Per-cell code written at setup (16 bits): bits [3, 9) hold code_x +
+// 8 code_y, bits [9, 12) the pressure damping code; kCodeSelfOff drops the
+// cell's own pressure from its face updates (walls carrying an admittance:
+// U = -g * P_neighbour); kCodeSrcX makes the x-face carry the source
+// (excitation column: Ux = g * exeV).  The host maps the distinct codes of a
+// batch to class bytes.
 constexpr int kShiftXY = 3;
 constexpr int kShiftP = 9;
 constexpr uint32_t kCodeSelfOff = 1u << 12;

@@ -354,13 +354,8 @@ __global__ void k_exact_classes(const TubeDesc* __restrict__ tubes, Bufs<Real> b
 }
 
 // ---------------------------------------------------------------------------
-// Time step, reference operation order.  One work-group owns one tube for a
-// block of steps; P and V are updated in place with a work-group barrier
-// between the pressure and velocity phases (a cell's pressure update reads
-// only its own P, a cell's velocity update only its own V, so in-place
-// updates equal the reference's double buffer).  Lane 0 runs the source
-// (table lookup or the two-mass recurrence in double, fed by the previous
-// step's P at the excitation cell) and records the probe.
+// Time step, reference operation order: the per-cell updates, the
+// one-work-group-per-tube kernel and the cooperative single-tube kernel.
 // ---------------------------------------------------------------------------
 // One tube's view for the two phases of the exact step.
 template <typename Real>
@@ -450,6 +445,12 @@ __device__ __forceinline__ void exact_velocity(const ExactView<Real>& x, Real ev
     }
 }
 
+// One work-group owns one tube for a block of steps; P and V are updated in
+// place with a work-group barrier between the pressure and velocity phases
+// (a cell's pressure update reads only its own P, a cell's velocity update
+// only its own V, so in-place updates equal the reference's double buffer).
+// Lane 0 runs the source (table lookup or the two-mass recurrence in double,
+// fed by the previous step's P at the excitation cell) and records the probe.
 template <typename Real>
 __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict__ tubes,
                                                      const int* __restrict__ order, Bufs<Real> b,
