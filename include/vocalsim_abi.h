@@ -129,10 +129,19 @@ enum vs_option {
                                     default: as many of the checkpoint spacing as fit in half the
                                     free device memory); shorter blocks trade one extra forward
                                     pass for memory */
-    VS_OPT_EXACT_GROUPS = 8      /* exact step of a single-tube handle: work-groups of one
-                                    cooperative launch, the tube's cells split across them
-                                    (bit-identical results).  0, the default: one per ~1000
-                                    interior cells, at most 16; 1: one work-group; 2..64 */
+    VS_OPT_EXACT_GROUPS = 8,     /* exact step of a single-tube handle: 1024-thread
+                                    work-groups of one cooperative launch (VS_OPT_THREADS
+                                    does not apply), the tube's columns split across them
+                                    (bit-identical results).  0, the default: the fewest (at
+                                    most 48) whose columns plus a halo of 2 x
+                                    VS_OPT_EXACT_STEPS hold one cell per thread (one
+                                    work-group when the tube itself does); 1: one
+                                    work-group; 2..64 */
+    VS_OPT_EXACT_STEPS = 9       /* steps of that cooperative launch between two grid
+                                    barriers: each work-group recomputes halo columns to run
+                                    them without exchanging data.  0, the default: up to 8,
+                                    as many as 48 work-groups allow; 1..32 (lowered where
+                                    the LDS tile or the source placement needs it) */
 };
 
 enum vs_kernel {

@@ -1,8 +1,11 @@
 """Exact step of a single tube spread over several work-groups of one
-cooperative launch (VS_OPT_EXACT_GROUPS): the per-cell code is the same as
-the one-work-group kernel, so probes and whole fields must be bit-identical
-for any group count, in FP64 and FP32, with Gaussian and two-mass sources,
-and the FP64 run must equal the CPU oracle exactly."""
+cooperative launch (VS_OPT_EXACT_GROUPS), several steps per grid barrier
+(VS_OPT_EXACT_STEPS) with recomputed halo columns: the per-cell code is the
+same as the one-work-group kernel, so probes and whole fields must be
+bit-identical for any group and step count, in FP64 and FP32, with Gaussian
+and two-mass sources, with launches that end inside a block of steps, and
+with the source in the middle of the domain; the FP64 run must equal the
+CPU oracle exactly."""
 import numpy as np
 import pytest
 
@@ -12,9 +15,10 @@ from vocalsim import Solver, _abi, tubes
 pytestmark = pytest.mark.gpu
 
 
-def run(spec, precision, groups, threads=1024, steps=None, blocks=1):
+def run(spec, precision, groups, threads=1024, steps=None, blocks=1, per_barrier=0):
     with Solver(spec, precision=precision, threads=threads, steps_per_launch=max(1, (steps or spec.num_steps) // blocks)) as s:
         s.set_option(_abi.OPT_EXACT_GROUPS, groups)
+        s.set_option(_abi.OPT_EXACT_STEPS, per_barrier)
         s.run(steps)
         return s.probe(), s.field("pressure"), s.field("vx"), s.field("vy")
 
@@ -27,6 +31,32 @@ def test_groups_are_bit_identical(hip_device, precision, source):
     for groups in (2, 3, 8, 16):
         got = run(spec, precision, groups, blocks=3)
         print(f"[coop] precision={precision} source={source} groups={groups}", flush=True)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision,source", [(64, tubes.SOURCE_TWO_MASS), (32, tubes.SOURCE_GAUSSIAN)])
+def test_steps_per_barrier_are_bit_identical(hip_device, precision, source):
+    # 901 steps in launches of 300: every launch ends inside a block of
+    # steps for 4 and 8, and the last launch is a single step
+    spec = tubes.vowel_tube(4, num_steps=901, source=source)
+    ref = run(spec, precision, 1)
+    for groups, per_barrier in ((2, 2), (8, 3), (14, 4), (16, 8), (24, 1), (0, 0), (0, 6)):
+        got = run(spec, precision, groups, blocks=3, per_barrier=per_barrier)
+        print(f"[coop] precision={precision} source={source} groups={groups} steps={per_barrier}", flush=True)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("source", [tubes.SOURCE_GAUSSIAN, tubes.SOURCE_TWO_MASS])
+def test_source_in_the_middle(hip_device, source):
+    # open space: the point source sits in column 176 of 355, so some group
+    # counts put a column boundary next to it and lower the steps per barrier
+    spec = tubes.open_space(num_steps=400, source_type=source)
+    ref = run(spec, 64, 1)
+    for groups in (2, 7, 8, 15, 16, 32):
+        got = run(spec, 64, groups, blocks=2, per_barrier=4)
+        print(f"[coop] open space source={source} groups={groups}", flush=True)
         for a, b in zip(got, ref):
             assert np.array_equal(a, b)
 
@@ -57,3 +87,9 @@ def test_option_range(hip_device):
         for bad in (-1, 65):
             with pytest.raises(Exception):
                 s.set_option(_abi.OPT_EXACT_GROUPS, bad)
+        assert s.get_option(_abi.OPT_EXACT_STEPS) == 0
+        for bad in (-1, 33):
+            with pytest.raises(Exception):
+                s.set_option(_abi.OPT_EXACT_STEPS, bad)
+        s.set_option(_abi.OPT_EXACT_STEPS, 32)
+        assert s.get_option(_abi.OPT_EXACT_STEPS) == 32

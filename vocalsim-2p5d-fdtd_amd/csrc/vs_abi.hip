@@ -324,7 +324,8 @@ struct vs_handle {
     bool adj_forward = false;  // checkpoints hold a completed forward pass
     std::vector<void*> adj_slabs;
     int adj_kernel = 0;        // VS_OPT_ADJOINT_KERNEL
-    int exact_groups = 0;      // VS_OPT_EXACT_GROUPS (0: one per ~1000 cells of a single tube, at most 16)
+    int exact_groups = 0;      // VS_OPT_EXACT_GROUPS (0: automatic, coop_plan)
+    int exact_steps = 0;       // VS_OPT_EXACT_STEPS (0: automatic, coop_plan)
     unsigned* d_bar = nullptr; // grid barrier state of the cooperative exact step
     void* d_state2 = nullptr;  // its second state buffers: P, Vx, Vy (Real, total_cells each)
     int coop_ok = -1;          // device supports cooperative launches (-1: not asked yet)
@@ -512,7 +513,7 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
         srct.insert(srct.end(), tab.begin(), tab.end());
         const double q = specs[k].tm_q > 0 ? specs[k].tm_q : 1.0;
         const double ps = specs[k].tm_ps > 0 ? specs[k].tm_ps : (double)686.46f;
-        twomass_init(tms[k], q, ps);
+        twomass_init(tms[k], q, ps, specs[k].srate_mul);
     }
     // largest tubes first
     std::vector<int> order(h->n);
@@ -679,18 +680,56 @@ int prepare_fast(vs_handle* h, int threads) {
 
 template <typename Real>
 hipError_t launch_any(vs_handle* h, Bufs<Real>& b, int t0, int ns, const int* d_t0 = nullptr);
-// The exact step of a single tube over several work-groups (one cooperative
-// launch) when VS_OPT_EXACT_GROUPS asks for it; graph replays keep the
+// The exact step of a single tube over several work-groups of one
+// cooperative launch, several steps per grid barrier; graph replays keep the
 // one-work-group kernel (the results are bit-identical either way).
-// Work-groups of the cooperative exact step for this handle (1: none).
-int coop_groups(const vs_handle* h) {
-    if (h->exact_groups > 0) return h->exact_groups;
-    const int cells = (h->desc[0].H - 2) * (h->desc[0].W - 2);
-    return std::max(1, std::min(16, (cells + 500) / 1000));
+struct CoopPlan {
+    int groups = 1, steps = 1;
+    bool one = false;   // one cell per lane (coop_one_cell_per_lane)
+    size_t tile = 0;
+};
+// Work-groups and steps per barrier for this handle (groups 1: none).  By
+// default: the most steps per barrier up to kCoopSteps for which at most
+// kCoopAutoGroups work-groups hold their first pressure regions (own
+// columns and 2s - 1 more) at one cell per lane (coop_one_cell_per_lane),
+// with the fewest such work-groups; tubes of at most one cell per lane stay
+// on one work-group.  VS_OPT_EXACT_GROUPS / VS_OPT_EXACT_STEPS override
+// either.  Steps are then lowered until the LDS tile fits and every source
+// work-group sees the excitation column (coop_source_ok).
+constexpr int kCoopSteps = 8;
+constexpr int kCoopAutoGroups = 48;
+constexpr int kCoopThreads = 1024;   // work-group size of the cooperative step (VS_OPT_THREADS is for the others)
+template <typename Real>
+CoopPlan coop_plan(const vs_handle* h) {
+    CoopPlan p;
+    const TubeDesc& T = h->desc[0];
+    const int H = T.H, W = T.W, Hi = H - 2, Wi = W - 2, ex = T.excitation % W;
+    if (Hi < 1 || Wi < 2 || h->exact_groups == 1) return p;
+    int s = h->exact_steps > 0 ? h->exact_steps : kCoopSteps;
+    int G = h->exact_groups;
+    if (G == 0) {
+        if ((long long)Hi * Wi <= kCoopThreads) return p;
+        for (int ss = s; ss >= 1 && G == 0; --ss) {
+            for (int g = 2; g <= kCoopAutoGroups; ++g) {
+                if (coop_one_cell_per_lane(H, W, g, ss, kCoopThreads)) { G = g; s = ss; break; }
+            }
+        }
+        if (G == 0) { G = kCoopAutoGroups; s = 1; }
+    }
+    G = std::min(G, Wi);
+    G = (Wi + coop_chunk(W, G) - 1) / coop_chunk(W, G);   // no empty work-groups
+    if (G < 2) return p;
+    while (s > 1 && (!coop_source_ok(W, G, s, ex) || coop_tile_bytes<Real>(H, W, G, s) > kCoopTileMax)) --s;
+    if (coop_tile_bytes<Real>(H, W, G, s) > kCoopTileMax) return p;
+    p.groups = G;
+    p.steps = s;
+    p.one = coop_one_cell_per_lane(H, W, G, s, kCoopThreads);
+    p.tile = coop_tile_bytes<Real>(H, W, G, s);
+    return p;
 }
 template <typename Real>
 bool use_coop(vs_handle* h, const int* d_t0) {
-    if (h->n != 1 || d_t0 || coop_groups(h) < 2 || h->desc[0].W - 2 > kCoopMaxWidth) return false;
+    if (h->n != 1 || d_t0 || coop_plan<Real>(h).groups < 2) return false;
     if (h->coop_ok < 0) {
         int dev = 0, v = 0;
         h->coop_ok = (hipGetDevice(&dev) == hipSuccess &&
@@ -709,8 +748,9 @@ hipError_t launch_exact(vs_handle* h, Bufs<Real>& b, int t0, int ns, const int* 
         }
         Real* st2 = static_cast<Real*>(h->d_state2);
         const long long C = h->total_cells;
-        hipError_t e = launch_step_coop<Real>(h->d_desc, h->d_order, b, t0, ns, h->threads, coop_groups(h), h->d_bar,
-                                              st2, st2 + C, st2 + 2 * C, h->stream);
+        const CoopPlan p = coop_plan<Real>(h);
+        hipError_t e = launch_step_coop<Real>(h->d_desc, h->d_order, b, t0, ns, kCoopThreads, p.groups, p.steps, p.one,
+                                              p.tile, h->d_bar, st2, st2 + C, st2 + 2 * C, h->stream);
         if (e != hipErrorCooperativeLaunchTooLarge) return e;
         // the device cannot hold the groups at once: one work-group from now on
         (void)hipGetLastError();
@@ -1154,8 +1194,11 @@ int vs_set_option(vs_handle* h, int option, int value) {
         if (value != h->adj_kernel) h->adj_forward = false;
         h->adj_kernel = value; return VS_OK;
     case VS_OPT_EXACT_GROUPS:
-        if (value < 0 || value > 64) return fail(VS_E_INVALID, "exact groups must be in [0, 64]");
+        if (value < 0 || value > kCoopMaxGroups) return fail(VS_E_INVALID, "exact groups must be in [0, 64]");
         h->exact_groups = value; return VS_OK;
+    case VS_OPT_EXACT_STEPS:
+        if (value < 0 || value > kCoopMaxSteps) return fail(VS_E_INVALID, "exact steps must be in [0, 32]");
+        h->exact_steps = value; return VS_OK;
     case VS_OPT_ADJOINT_BLOCK:
         if (value < 0) return fail(VS_E_INVALID, "adjoint block must be >= 0");
         if (value != h->adj_block) {   // slabs are re-sized by the next forward pass
@@ -1186,6 +1229,7 @@ int vs_get_option(const vs_handle* h, int option, int* value) {
     case VS_OPT_ADJOINT_KERNEL: *value = h->adj_kernel; return VS_OK;
     case VS_OPT_ADJOINT_BLOCK: *value = h->adj_block; return VS_OK;
     case VS_OPT_EXACT_GROUPS: *value = h->exact_groups; return VS_OK;
+    case VS_OPT_EXACT_STEPS: *value = h->exact_steps; return VS_OK;
     default: return fail(VS_E_INVALID, "unknown option");
     }
 }

@@ -60,11 +60,18 @@ struct TwoMassState {
     double mu, rho, q, gs, m1, m2, d1, d2, k1, k2, kc, etak1, etak2, h1, h2,
         etah1, etah2, Ag01, Ag02, lg, r1_open, r2_open, r1_close, r2_close, A1,
         ps, p1, ug_old, ug_curr, ug_next, x1_old, x1_curr, x2_old, x2_curr;
+    // Sub-expressions of the step that do not depend on the state, or only
+    // on whether a fold is closed (suffix c) or open (suffix o), evaluated
+    // once by twomass_init with the step's own operations: sr = 44100 *
+    // srate_mul, sr2 = sr * sr, thr = -Ag0 / (2 lg), g = Ag0 / (2 lg),
+    // a11, a22 and a12 of the 2x2 system, cb = 2 m + r / sr.
+    double sr, sr2, thr1, thr2, g1, g2, a11c, a11o, a22c, a22o, a12, cb1c, cb1o, cb2c, cb2o;
 };
 
 // Parameter set-up (TwoMassModel.cpp:4-87).  The float sub-expressions of the
 // reference (units are float32 constants) are kept as float.
-inline void twomass_init(TwoMassState& s, double q, double ps) {
+inline void twomass_init(TwoMassState& s, double q, double ps, int srate_mul) {
+#pragma clang fp contract(off)
     const float CM = 1e-2f, GRAM = 1e-3f, DYN = 1e-5f, KG = 1.0f, M = 1.0f, SEC = 1.0f;
     s.mu = 1.86e-4 * (DYN * SEC / (CM * CM));
     s.rho = 1.14 * (KG / (M * M * M));
@@ -96,23 +103,51 @@ inline void twomass_init(TwoMassState& s, double q, double ps) {
     s.p1 = 0;
     s.ug_old = s.ug_curr = s.ug_next = 0;
     s.x1_old = s.x1_curr = s.x2_old = s.x2_curr = 0;
+    // TwoMassModel.cpp:90-231, the terms of the step that are constants
+    const double sr = 44100 * static_cast<double>(srate_mul), lg = s.lg;
+    s.sr = sr;
+    s.sr2 = sr * sr;
+    s.thr1 = -s.Ag01 / (2 * lg);
+    s.thr2 = -s.Ag02 / (2 * lg);
+    s.g1 = s.Ag01 / (2 * lg);
+    s.g2 = s.Ag02 / (2 * lg);
+    const double h1c = s.h1, h1o = 0.0f, h2c = s.h2, h2o = 0.0f;
+    s.a11c = (s.k1 + h1c + s.kc) / (sr * sr) + s.r1_close / sr + s.m1;
+    s.a11o = (s.k1 + h1o + s.kc) / (sr * sr) + s.r1_open / sr + s.m1;
+    s.a22c = (s.k2 + h2c + s.kc) / (sr * sr) + s.r2_close / sr + s.m2;
+    s.a22o = (s.k2 + h2o + s.kc) / (sr * sr) + s.r2_open / sr + s.m2;
+    s.a12 = -s.kc / (sr * sr);
+    s.cb1c = 2 * s.m1 + s.r1_close / sr;
+    s.cb1o = 2 * s.m1 + s.r1_open / sr;
+    s.cb2c = 2 * s.m2 + s.r2_close / sr;
+    s.cb2o = 2 * s.m2 + s.r2_open / sr;
 }
 
-// One step of the recurrence (TwoMassModel.cpp:90-231).  Rounds every
-// operation separately (no contraction) so host and device agree bitwise.
-__host__ __device__ inline double twomass_step(TwoMassState& s, int srate_mul) {
+// One step of the recurrence (TwoMassModel.cpp:90-231), in three parts so
+// that a caller can compute the first two before the pressure input p1 is
+// known: twomass_solve (the new fold positions), twomass_flow (flow
+// resistance and inertance at those positions) and twomass_finish (the new
+// glottal flow from p1, committing the state).  twomass_step runs the three
+// in a row.  Every operation is rounded separately (no contraction), in the
+// reference's order, so host and device agree bitwise.
+struct TwoMassNext {
+    double x1n, x2n, Rtot, Ltot;
+    bool closed;
+};
+
+__host__ __device__ inline void twomass_solve(const TwoMassState& s, TwoMassNext& n) {
 #pragma clang fp contract(off)
-    const double sr = 44100 * static_cast<double>(srate_mul);
+    const double sr = s.sr;
     const double lg = s.lg, rho = s.rho, mu = s.mu, d1 = s.d1, d2 = s.d2;
     const double x1 = s.x1_curr, x2 = s.x2_curr, ug = s.ug_curr;
-    double Ag1 = s.Ag01 + 2 * lg * x1;
-    double Ag2 = s.Ag02 + 2 * lg * x2;
+    const double Ag1 = s.Ag01 + 2 * lg * x1;
+    const double Ag2 = s.Ag02 + 2 * lg * x2;
     const double Rv1 = (12 * mu * lg * lg * d1) / (Ag1 * Ag1 * Ag1);
     const double Lg1 = (rho * d1) / (Ag1);
     const double Rv2 = (12 * mu * lg * lg * d2) / (Ag2 * Ag2 * Ag2);
     const double Lg2 = (rho * d2) / (Ag2);
-    const bool open1 = x1 > -s.Ag01 / (2 * lg);
-    const bool open2 = x2 > -s.Ag02 / (2 * lg);
+    const bool open1 = x1 > s.thr1;
+    const bool open2 = x2 > s.thr2;
     double pm1;
     if (open1 && open2)
         pm1 = s.ps - 1.37 * (rho / 2) * (ug / Ag1) * (ug / Ag1) - 0.5 * (Rv1 * ug + Lg1 * sr * (ug - s.ug_old));
@@ -130,39 +165,58 @@ __host__ __device__ inline double twomass_step(TwoMassState& s, int srate_mul) {
         pm2 = 0;
     }
     const double f2 = pm2 * d2 * lg;
-    double h1u, r1u, h2u, r2u;
-    if (x1 + s.Ag01 / (2 * lg) < 0) { h1u = s.h1; r1u = s.r1_close; } else { h1u = 0.0f; r1u = s.r1_open; }
-    if (x2 + s.Ag02 / (2 * lg) < 0) { h2u = s.h2; r2u = s.r2_close; } else { h2u = 0.0f; r2u = s.r2_open; }
-    const double a11 = (s.k1 + h1u + s.kc) / (sr * sr) + r1u / sr + s.m1;
-    const double a12 = -s.kc / (sr * sr);
+    const bool closed1 = x1 + s.g1 < 0, closed2 = x2 + s.g2 < 0;
+    const double h1u = closed1 ? s.h1 : 0.0f, h2u = closed2 ? s.h2 : 0.0f;
+    const double a11 = closed1 ? s.a11c : s.a11o;
+    const double a12 = s.a12;
     const double a21 = a12;
-    const double a22 = (s.k2 + h2u + s.kc) / (sr * sr) + r2u / sr + s.m2;
-    const double g1 = s.Ag01 / (2 * lg), g2 = s.Ag02 / (2 * lg);
+    const double a22 = closed2 ? s.a22c : s.a22o;
+    const double g1 = s.g1, g2 = s.g2;
     const double s1p = s.k1 * s.etak1 * x1 * x1 * x1 + h1u * (g1 + s.etah1 * (g1 + x1) * (g1 + x1) * (g1 + x1));
     const double s2p = s.k2 * s.etak2 * x2 * x2 * x2 + h2u * (g2 + s.etah2 * (g2 + x2) * (g2 + x2) * (g2 + x2));
-    const double b1 = (2 * s.m1 + r1u / sr) * x1 - s.m1 * s.x1_old - s1p / (sr * sr) + f1 / (sr * sr);
-    const double b2 = (2 * s.m2 + r2u / sr) * x2 - s.m2 * s.x2_old - s2p / (sr * sr) + f2 / (sr * sr);
+    const double b1 = (closed1 ? s.cb1c : s.cb1o) * x1 - s.m1 * s.x1_old - s1p / s.sr2 + f1 / s.sr2;
+    const double b2 = (closed2 ? s.cb2c : s.cb2o) * x2 - s.m2 * s.x2_old - s2p / s.sr2 + f2 / s.sr2;
     double det = a11 * a22 - a21 * a12;
     if (det == 0) det = 1;
-    const double x1n = (a22 * b1 - a12 * b2) / det;
-    const double x2n = (a11 * b2 - a21 * b1) / det;
-    s.x1_old = x1; s.x1_curr = x1n;
-    s.x2_old = x2; s.x2_curr = x2n;
-    Ag1 = s.Ag01 + 2 * lg * x1n;
-    Ag2 = s.Ag02 + 2 * lg * x2n;
-    double ugn;
-    if (Ag1 < 0 || Ag2 < 0) {
-        ugn = 0;
-    } else {
-        const double Rtot = (rho / 2) * __builtin_fabs(ug) * ((0.37 / (Ag1 * Ag1)) + (1 - 2 * (Ag2 / s.A1) * (1 - Ag2 / s.A1)) / (Ag2 * Ag2))
-                            + (12 * mu * lg * lg * (d1 / (Ag1 * Ag1 * Ag1) + d2 / (Ag2 * Ag2 * Ag2)));
-        const double Ltot = rho * (d1 / Ag1 + d2 / Ag2);
-        ugn = ((s.ps - s.p1) / sr + Ltot * ug) / (Rtot / sr + Ltot);
+    n.x1n = (a22 * b1 - a12 * b2) / det;
+    n.x2n = (a11 * b2 - a21 * b1) / det;
+}
+
+__host__ __device__ inline void twomass_flow(const TwoMassState& s, TwoMassNext& n) {
+#pragma clang fp contract(off)
+    const double lg = s.lg, rho = s.rho, mu = s.mu, d1 = s.d1, d2 = s.d2;
+    const double ug = s.ug_curr;
+    const double Ag1 = s.Ag01 + 2 * lg * n.x1n;
+    const double Ag2 = s.Ag02 + 2 * lg * n.x2n;
+    n.closed = Ag1 < 0 || Ag2 < 0;
+    n.Rtot = n.Ltot = 0;
+    if (!n.closed) {
+        n.Rtot = (rho / 2) * __builtin_fabs(ug) * ((0.37 / (Ag1 * Ag1)) + (1 - 2 * (Ag2 / s.A1) * (1 - Ag2 / s.A1)) / (Ag2 * Ag2))
+                 + (12 * mu * lg * lg * (d1 / (Ag1 * Ag1 * Ag1) + d2 / (Ag2 * Ag2 * Ag2)));
+        n.Ltot = rho * (d1 / Ag1 + d2 / Ag2);
     }
+}
+
+__host__ __device__ inline double twomass_finish(TwoMassState& s, const TwoMassNext& n, double p1) {
+#pragma clang fp contract(off)
+    const double sr = s.sr;
+    const double ug = s.ug_curr;
+    s.p1 = p1;
+    s.x1_old = s.x1_curr; s.x1_curr = n.x1n;
+    s.x2_old = s.x2_curr; s.x2_curr = n.x2n;
+    const double ugn = n.closed ? 0 : ((s.ps - s.p1) / sr + n.Ltot * ug) / (n.Rtot / sr + n.Ltot);
     s.ug_old = ug;
     s.ug_curr = ugn;
     s.ug_next = ugn;
     return ugn;
+}
+
+// One step with the pressure input already in s.p1.
+__host__ __device__ inline double twomass_step(TwoMassState& s) {
+    TwoMassNext n;
+    twomass_solve(s, n);
+    twomass_flow(s, n);
+    return twomass_finish(s, n, s.p1);
 }
 
 // Device buffers of a batch (packed over tubes).
@@ -318,12 +372,65 @@ template <typename Real>
 hipError_t launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<Real> b,
                        int t0, int nsteps, int threads, int kernel, const int* d_t0, hipStream_t st);
 // Exact step of a single tube over `groups` work-groups of one cooperative
-// launch (bar: two words of barrier state, zeroed by the launch; P2, Vx2,
-// Vy2: second state buffers, cell layout, zero frame ring).
+// launch, s steps per grid barrier (one: coop_one_cell_per_lane; bar: two
+// words of barrier state, zeroed by the launch; P2, Vx2, Vy2: second state
+// buffers, cell layout, zero frame ring; tile_bytes: coop_tile_bytes, at
+// most kCoopTileMax).
 template <typename Real>
 hipError_t launch_step_coop(const TubeDesc* d_tubes, const int* d_order, Bufs<Real> b, int t0, int nsteps,
-                            int threads, int groups, unsigned* bar, Real* P2, Real* Vx2, Real* Vy2, hipStream_t st);
-constexpr int kCoopMaxWidth = 1024;   // widest interior row (W - 2) the cooperative step accepts
+                            int threads, int groups, int s, bool one, size_t tile_bytes, unsigned* bar, Real* P2,
+                            Real* Vx2, Real* Vy2, hipStream_t st);
+// LDS tile of one work-group of that step: P, Vx, Vy of H rows by the
+// widest column range (own columns plus s on each side).
+template <typename Real>
+size_t coop_tile_bytes(int H, int W, int groups, int s);
+constexpr size_t kCoopTileMax = 144 * 1024;   // dynamic LDS a cooperative step's tile may use
+constexpr int kCoopMaxGroups = 64;             // most work-groups of a cooperative step
+constexpr int kCoopMaxSteps = 32;              // most steps per grid barrier
+
+// Column partition of that step: work-group g of `groups` owns the frame
+// columns [*a, *e) (empty when *a >= *e) of a W-wide frame.
+__host__ __device__ inline int coop_chunk(int W, int groups) { return (W - 2 + groups - 1) / groups; }
+__host__ __device__ inline void coop_columns(int W, int groups, int g, int* a, int* e) {
+    *a = 1 + g * coop_chunk(W, groups);
+    const int end = *a + coop_chunk(W, groups);
+    *e = end < W - 1 ? end : W - 1;
+}
+// The work-group owning column c (ring columns go to the nearest owner).
+__host__ __device__ inline int coop_owner(int W, int groups, int c) {
+    const int cc = c < 1 ? 1 : (c > W - 2 ? W - 2 : c);
+    return (cc - 1) / coop_chunk(W, groups);
+}
+// Does every work-group's first pressure region (own columns, s - 1 more on
+// the left and s on the right, within the interior) hold at most one cell
+// per lane of the waves after the first of a `threads`-lane work-group.
+__host__ __device__ inline bool coop_one_cell_per_lane(int H, int W, int groups, int s, int threads) {
+    const int w = coop_chunk(W, groups) + 2 * s - 1;
+    return (long long)(H - 2) * (w < W - 2 ? w : W - 2) <= threads - 64;
+}
+// Does work-group g run the source: its first velocity region (own columns
+// and s - 1 more on each side) reaches the excitation column ex or the one
+// left of it (the cells with source coefficients), or it owns ex.
+__host__ __device__ inline bool coop_runs_source(int W, int groups, int s, int g, int ex) {
+    int a, e;
+    coop_columns(W, groups, g, &a, &e);
+    if (a >= e) return false;
+    const int lo = a - (s - 1), hi = e + (s - 1);
+    return g == coop_owner(W, groups, ex) || (ex >= lo && ex < hi) || (ex - 1 >= lo && ex - 1 < hi);
+}
+// A source work-group reads the excitation cell's pressure of sub-steps
+// 1 .. s-1 from its own tile, so for s >= 2 every source work-group needs
+// the excitation column within [a - 1, e + 2).
+__host__ __device__ inline bool coop_source_ok(int W, int groups, int s, int ex) {
+    if (s < 2) return true;
+    for (int g = 0; g < groups; ++g) {
+        if (!coop_runs_source(W, groups, s, g, ex)) continue;
+        int a, e;
+        coop_columns(W, groups, g, &a, &e);
+        if (ex < a - 1 || ex >= e + 2) return false;
+    }
+    return true;
+}
 template <typename Real>
 hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tube, Bufs<Real> b,
                          Real* out17, hipStream_t st);

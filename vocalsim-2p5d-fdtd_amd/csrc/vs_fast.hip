@@ -264,7 +264,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
             if (twomass) tm = b.tm[tube];
             if (t0 < tend) {
                 double e;
-                if (twomass) { tm.p1 = pexc; e = twomass_step(tm, D.srate_mul); }
+                if (twomass) { tm.p1 = pexc; e = twomass_step(tm); }
                 else e = (double)b.source[D.src_off + t0];
                 sexe[t0 & 1] = (double)(float)e;
             }
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 b.strace[D.src_off + ts] = sexe[ts & 1];
                 if (ts + 1 < tend) {
                     double e;
-                    if (twomass) { tm.p1 = pexc; e = twomass_step(tm, D.srate_mul); }
+                    if (twomass) { tm.p1 = pexc; e = twomass_step(tm); }
                     else e = (double)b.source[D.src_off + ts + 1];
                     sexe[(ts + 1) & 1] = (double)(float)e;
                 }

@@ -7,6 +7,8 @@
 
 #include "vs_common.h"
 
+#include <atomic>
+
 namespace vs {
 
 namespace {
@@ -374,34 +376,53 @@ struct ExactView {
     bool dirichlet, eigen;
 };
 
-// New pressure of frame cell q (FDTDEngineCPU.cpp:316-339), from the state
-// in x (P, Vx, Vy of the step's input).
+// New pressure of one cell (FDTDEngineCPU.cpp:316-339) from its pressure
+// p, velocities vx, vy, the left cell's vxl and the lower cell's vyb, its
+// depths dp, dx, dy, the left cell's dxl and the lower cell's dyb, and its
+// class coefficients o.
 template <typename Real>
-__device__ __forceinline__ Real exact_pressure_cell(const ExactView<Real>& x, int q) {
-    const int W = x.W;
-    const Real cxvx = x.Vx[q] * x.dX[q] - x.Vx[q - 1] * x.dX[q - 1];
-    const Real cyvy = x.Vy[q] * x.dY[q] - x.Vy[q + W] * x.dY[q + W];
-    const Real num = (x.P[q] * x.dP[q]) - (x.K1 * (cxvx + cyvy));
-    const Real* o = x.ctab + x.xc[q] * kExactSlots;
+__device__ __forceinline__ Real exact_pressure_v(const ExactView<Real>& x, const Real* o, Real p, Real vx, Real vxl,
+                                                 Real vy, Real vyb, Real dp, Real dx, Real dxl, Real dy, Real dyb) {
+    const Real cxvx = vx * dx - vxl * dxl;
+    const Real cyvy = vy * dy - vyb * dyb;
+    const Real num = (p * dp) - (x.K1 * (cxvx + cyvy));
     const Real sp = o[8];   // sigma'dt of the cell's own type
-    const Real den = x.eigen ? (sp * x.dP[q] + x.dP[q]) : ((1 + sp) * x.dP[q]);
+    const Real den = x.eigen ? (sp * dp + dp) : ((1 + sp) * dp);
     Real v = num / den;
     if (x.dirichlet && o[17] != 0) v = 0;
     return v;
 }
 
-// New velocities of frame cell q in column c (FDTDEngineCPU.cpp:364-410),
-// from its input velocities in x and the new pressures pc (own), pr (right)
-// and pt (top); stored to vx_out, vy_out.
+// New pressure of frame cell q.  The step's input state is read from sP,
+// sVx, sVy at index s with row stride sW (the frame itself, or a tile of
+// it); depths and classes at frame index q.
 template <typename Real>
-__device__ __forceinline__ void exact_velocity_cell(const ExactView<Real>& x, int q, int c, Real pc, Real pr, Real pt,
-                                                    Real ev, Real* vx_out, Real* vy_out) {
-    const Real* o = x.ctab + x.xc[q] * kExactSlots;
-    const Real z = (o[18] != 0) ? x.zc[c] : (Real)0;
+__device__ __forceinline__ Real exact_pressure_at(const ExactView<Real>& x, const Real* sP, const Real* sVx,
+                                                  const Real* sVy, int s, int sW, int q) {
+    const int W = x.W;
+    return exact_pressure_v(x, x.ctab + x.xc[q] * kExactSlots, sP[s], sVx[s], sVx[s - 1], sVy[s], sVy[s + sW],
+                            x.dP[q], x.dX[q], x.dX[q - 1], x.dY[q], x.dY[q + W]);
+}
+
+// New pressure of frame cell q from the state in x (P, Vx, Vy of the step's
+// input).
+template <typename Real>
+__device__ __forceinline__ Real exact_pressure_cell(const ExactView<Real>& x, int q) {
+    return exact_pressure_at(x, x.P, x.Vx, x.Vy, q, x.W, q);
+}
+
+// New velocities of one cell (FDTDEngineCPU.cpp:364-410) from its input
+// velocities vxi, vyi, the new pressures pc (own), pr (right) and pt (top),
+// the source value ev, its column's z_inv zcol and its class coefficients
+// o; stored to vx_out, vy_out.
+template <typename Real>
+__device__ __forceinline__ void exact_velocity_v(const Real* o, Real vxi, Real vyi, Real pc, Real pr, Real pt,
+                                                 Real ev, Real zcol, Real* vx_out, Real* vy_out) {
+    const Real z = (o[18] != 0) ? zcol : (Real)0;
     const Real cxp = pr - pc;
     const Real cyp = pt - pc;
-    Real vx = (o[2] * x.Vx[q]) - (o[6] * cxp);
-    Real vy = (o[3] * x.Vy[q]) - (o[7] * cyp);
+    Real vx = (o[2] * vxi) - (o[6] * cxp);
+    Real vy = (o[3] * vyi) - (o[7] * cyp);
     vx = vx + (ev * o[11] * o[4]);
     vy = vy + (ev * o[12] * o[5]);
     Real bx = (o[14] * pc * o[0]) - (o[13] * pr * o[1]);
@@ -414,6 +435,22 @@ __device__ __forceinline__ void exact_velocity_cell(const ExactView<Real>& x, in
     vy /= (o[3] + o[5]);
     *vx_out = vx;
     *vy_out = vy;
+}
+
+// New velocities of frame cell q in column c from its input velocities vxi,
+// vyi and the new pressures pc, pr, pt.
+template <typename Real>
+__device__ __forceinline__ void exact_velocity_at(const ExactView<Real>& x, int q, int c, Real vxi, Real vyi, Real pc,
+                                                  Real pr, Real pt, Real ev, Real* vx_out, Real* vy_out) {
+    const Real* o = x.ctab + x.xc[q] * kExactSlots;
+    exact_velocity_v(o, vxi, vyi, pc, pr, pt, ev, (o[18] != 0) ? x.zc[c] : (Real)0, vx_out, vy_out);
+}
+
+// The same with the input velocities read from x at q.
+template <typename Real>
+__device__ __forceinline__ void exact_velocity_cell(const ExactView<Real>& x, int q, int c, Real pc, Real pr, Real pt,
+                                                    Real ev, Real* vx_out, Real* vy_out) {
+    exact_velocity_at(x, q, c, x.Vx[q], x.Vy[q], pc, pr, pt, ev, vx_out, vy_out);
 }
 
 // Pressure phase over the interior cells i = i0, i0 + B, ... < i1
@@ -497,7 +534,7 @@ __global__ __launch_bounds__(1024) void k_step_exact(const TubeDesc* __restrict_
     for (int t = t0; t < tend; ++t) {
         if (tid == 0) {
             double e;
-            if (twomass) { tm.p1 = pexc; e = twomass_step(tm, T.srate_mul); }
+            if (twomass) { tm.p1 = pexc; e = twomass_step(tm); }
             else e = (double)src[t];
             s_exeV = (Real)e;
             strace[t] = (double)(Real)e;
@@ -555,132 +592,223 @@ __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned groups, int
 }
 
 // The exact step of one tube spread over gridDim.x work-groups of one
-// cooperative launch, with one grid barrier per step.  The state is double
-// buffered as in the reference (PV_N / PV_Nplus1): step t reads `cur` and
-// writes `nxt`, which swap after the barrier.  Work-group g owns the
-// contiguous interior cells [g chunk, (g + 1) chunk) and runs the per-cell
-// code of k_step_exact (bit-identical results).  Its velocity phase needs new
-// pressures of the row above its chunk and of the cell after its last one;
-// it computes those halo cells itself (same code, same inputs) into LDS, so
-// no barrier is needed between the phases.  Every work-group runs the
-// source on its own lane 0 (table lookup, or the two-mass recurrence fed by
-// the new pressure at the excitation cell after each barrier: identical
-// inputs, identical values); work-group 0 records the probe and the source
-// trace and keeps the final source state.
-constexpr int kCoopHalo = 1024;   // widest padded row a cooperative step supports (halo in LDS)
-
-template <typename Real>
+// cooperative launch, s steps per grid barrier.  Work-group g owns the
+// frame columns [a, e) of every interior row (coop_columns).  Each period of
+// s steps
+//  1. copies rows 0..H-1 of the columns [a - s, e + s) of the step's input
+//     state into an LDS tile;
+//  2. advances the tile s steps in place: sub-step k updates pressures on
+//     the columns [a - (s-1-k), e + (s-k)) and then velocities on
+//     [a - (s-1-k), e + (s-1-k)).  A pressure reads velocities of its own
+//     and the left column, a velocity pressures of its own and the right
+//     column, so each region holds every input of the next one; the
+//     columns outside [a, e) recompute cells that neighbouring work-groups
+//     own, with the same code and inputs, hence the same values;
+//  3. stores its own columns into the other state buffer (double buffered
+//     as the reference's PV_N / PV_Nplus1) and meets the other work-groups
+//     at one grid barrier.
+// The per-cell code is that of k_step_exact, so results are bit-identical.
+// Only the excitation column and the one left of it carry the source
+// (coefficients 11 and 12).  The work-groups whose velocity regions reach
+// them (coop_runs_source) run the source on lane 0, the two-mass recurrence
+// fed by the excitation cell's new pressure from their own tile (identical
+// inputs, identical values); the others' cells have zero source
+// coefficients.  The work-group owning the listener column records the
+// probe, the one owning the excitation column the source trace and the
+// final source state.
+// ONE: the first pressure region holds at most one cell per lane of waves
+// 1 and up, so each of those lanes keeps one cell for the whole launch,
+// with its depths, class and z_inv in registers and its own P, Vx, Vy in
+// registers within a period; only the neighbours' values come from the
+// tile.  Wave 0 is then the control wave: it runs the two-mass recurrence
+// one step ahead, overlapped with the cell waves -- the fold positions of
+// step n + 1 during the pressure phase of step n, the flow coefficients and
+// (once the new excitation pressure is in the tile) the flow of step n + 1
+// during the velocity phase of step n, into the other slot of a
+// double-buffered source value.
+template <typename Real, bool ONE>
 __global__ __launch_bounds__(1024) void k_step_exact_coop(const TubeDesc* __restrict__ tubes,
                                                           const int* __restrict__ order, Bufs<Real> b, int t0,
-                                                          int nsteps, unsigned* bar, Real* P2, Real* Vx2, Real* Vy2,
-                                                          int spin_limit) {
+                                                          int nsteps, int s, unsigned* bar, Real* P2, Real* Vx2,
+                                                          Real* Vy2, int spin_limit) {
+    extern __shared__ __align__(16) unsigned char coop_tile[];
     const int tube = order[0];
     const TubeDesc& T = tubes[tube];
     __shared__ Real ctab[kExactClasses * kExactSlots];
-    __shared__ Real halo[kCoopHalo + 1];   // new P of cells i0 - Wi .. i0 - 1, then of cell i1
     __shared__ TwoMassState tm;
-    __shared__ Real s_ev;
+    __shared__ TwoMassNext nx;   // ONE: the next step's recurrence before its pressure input
+    __shared__ Real s_ev[2];     // source value; ONE: of steps with even / odd k
     const int tid = threadIdx.x, B = blockDim.x, g = blockIdx.x;
-    const unsigned G = gridDim.x;
+    const int G = (int)gridDim.x;
     {
         const Real* src = b.xtab + (long long)tube * kExactClasses * kExactSlots;
         const int nv = b.xn[tube] * kExactSlots;
         for (int i = tid; i < nv; i += B) ctab[i] = src[i];
     }
-    const int H = T.H, W = T.W, Wi = W - 2, NI = (H - 2) * Wi;
+    const int H = T.H, W = T.W, Hi = H - 2;
     const int tend = min(t0 + nsteps, T.num_steps);
+    int a, e;
+    coop_columns(W, G, g, &a, &e);
+    const int L = max(0, a - s), R = min(W, e + s), TW = max(0, R - L);
+    Real* tP = reinterpret_cast<Real*>(coop_tile);
+    Real* tVx = tP + H * TW;
+    Real* tVy = tVx + H * TW;
     const long long co = T.cell_off;
-    ExactView<Real> cur{b.P + co, b.Vx + co, b.Vy + co, b.dP + co, b.dX + co, b.dY + co, b.zcol + T.col_off,
-                        b.xcls + co, ctab, W, Wi, (Real)T.ph.K1, T.dirichlet != 0, T.variant == 1};
+    const ExactView<Real> x{b.P + co, b.Vx + co, b.Vy + co, b.dP + co, b.dX + co, b.dY + co, b.zcol + T.col_off,
+                            b.xcls + co, ctab, W, W - 2, (Real)T.ph.K1, T.dirichlet != 0, T.variant == 1};
+    Real *cP = b.P + co, *cVx = b.Vx + co, *cVy = b.Vy + co;
     Real *nP = P2 + co, *nVx = Vx2 + co, *nVy = Vy2 + co;
     const Real* __restrict__ src = b.source + T.src_off;
     double* probe = b.probe + T.src_off;
     double* strace = b.strace + T.src_off;
     const bool twomass = T.source_type == 2;
-    double pexc = 0;
-    if (tid == 0) {
+    const int ex = T.excitation % W, ey = T.excitation / W;
+    const int lx = T.listener % W, ly = T.listener / W;
+    const bool source = coop_runs_source(W, G, s, g, ex);
+    const bool keeps_source = g == coop_owner(W, G, ex);
+    const bool keeps_probe = g == coop_owner(W, G, lx);
+    double pexc = 0;   // lane 0 of a source work-group: the source's pressure input
+    if (tid == 0 && source) {
         pexc = b.pexc[tube];
         if (twomass) tm = b.tm[tube];
     }
-    __syncthreads();
-    const int chunk = (NI + (int)G - 1) / (int)G;
-    const int c0i = g * chunk, c1i = min(NI, (g + 1) * chunk);   // this work-group's cells
-    const int hb = max(0, c0i - Wi);                                // first halo cell above
-    const int dr = B / Wi, dc = B - dr * Wi;
-    const int r0 = (c0i + tid) / Wi, cc0 = (c0i + tid) - r0 * Wi;
-    for (int t = t0; t < tend; ++t) {
-        if (tid == 0) {
-            double e;
-            if (twomass) { tm.p1 = pexc; e = twomass_step(tm, T.srate_mul); }
-            else e = (double)src[t];
-            s_ev = (Real)e;
-            if (g == 0) strace[t] = (double)(Real)e;
+    // ONE: this lane's cell (r1, c1) of the first pressure region, if any
+    const int pl0 = max(1, a - (s - 1)), w0 = max(0, min(W - 1, e + s) - pl0);
+    const int cell = tid - 64;
+    const bool mine = ONE && cell >= 0 && cell < Hi * w0;
+    const int r1 = mine ? 1 + cell / w0 : 1, c1 = mine ? pl0 + (cell - (r1 - 1) * w0) : 1;
+    const int s1 = r1 * TW + (c1 - L), q1 = r1 * W + c1;
+    Real dp1 = 0, dx1 = 0, dxl1 = 0, dy1 = 0, dyb1 = 0, z1 = 0;
+    const Real* o1 = ctab;
+    if (mine) {
+        dp1 = x.dP[q1]; dx1 = x.dX[q1]; dxl1 = x.dX[q1 - 1]; dy1 = x.dY[q1]; dyb1 = x.dY[q1 + W];
+        o1 = ctab + x.xc[q1] * kExactSlots;
+        z1 = x.zc[c1];
+    }
+    for (int t = t0; t < tend; t += s) {
+        const int sp = min(s, tend - t);
+        for (int i = tid; i < H * TW; i += B) {
+            const int r = i / TW, c = i - r * TW;
+            const int q = r * W + L + c;
+            tP[i] = cP[q];
+            tVx[i] = cVx[q];
+            tVy[i] = cVy[q];
         }
-        // pressure: own cells into nxt, halo cells into LDS
-        {
-            int r = r0 + 1, c = cc0 + 1;
-            for (int i = c0i + tid; i < c1i; i += B) {
-                const int q = r * W + c;
-                nP[q] = exact_pressure_cell(cur, q);
-                r += dr; c += dc;
-                if (c > Wi) { c -= Wi; ++r; }
-            }
-            for (int i = hb + tid; i < c0i; i += B) {
-                const int q = (i / Wi + 1) * W + (i - (i / Wi) * Wi) + 1;
-                halo[i - (c0i - Wi)] = exact_pressure_cell(cur, q);
-            }
-            if (tid == 0 && c1i < NI) {
-                const int q = (c1i / Wi + 1) * W + (c1i - (c1i / Wi) * Wi) + 1;
-                halo[kCoopHalo] = exact_pressure_cell(cur, q);
-            }
-        }
+        if (tid == 0 && source && t > t0) pexc = (double)cP[T.excitation];
         __syncthreads();
-        // velocity: own cells, new P from nxt (own cells), the LDS halo, or
-        // the zero frame ring
-        {
-            const Real ev = s_ev;
-            int r = r0 + 1, c = cc0 + 1;
-            for (int i = c0i + tid; i < c1i; i += B) {
-                const int q = r * W + c;
-                // the right / top neighbour is in nxt (this work-group's cells
-                // or the zero frame ring) or in the LDS halo; both loads are
-                // issued (the nxt index falls back to the cell itself when
-                // the halo holds the value) and the value is selected
-                const bool rg = c == Wi || i + 1 < c1i, tg = r == 1 || i - Wi >= c0i;
-                const Real pc = nP[q];
-                const Real prg = nP[rg ? q + 1 : q], prh = halo[kCoopHalo];
-                const Real ptg = nP[tg ? q - W : q], pth = halo[tg ? 0 : i - c0i];
-                const Real pr = rg ? prg : prh;
-                const Real pt = tg ? ptg : pth;
-                exact_velocity_cell(cur, q, c, pc, pr, pt, ev, &nVx[q], &nVy[q]);
-                r += dr; c += dc;
-                if (c > Wi) { c -= Wi; ++r; }
+        Real p1 = 0, vx1 = 0, vy1 = 0;   // ONE: this lane's cell
+        if (mine) { p1 = tP[s1]; vx1 = tVx[s1]; vy1 = tVy[s1]; }
+        for (int k = 0; k < sp; ++k) {
+            const int tt = t + k;
+            if (tid == 0 && source) {
+                if constexpr (ONE) {
+                    if (k == 0) {   // later steps of the period were finished one phase ahead
+                        double ev;
+                        if (twomass) {
+                            if (t == t0) { twomass_solve(tm, nx); twomass_flow(tm, nx); }
+                            ev = twomass_finish(tm, nx, pexc);
+                        } else {
+                            ev = (double)src[tt];
+                        }
+                        s_ev[0] = (Real)ev;
+                        if (keeps_source) strace[tt] = (double)(Real)ev;
+                    }
+                    if (twomass && tt + 1 < tend) twomass_solve(tm, nx);
+                } else {
+                    double ev;
+                    if (twomass) { tm.p1 = pexc; ev = twomass_step(tm); }
+                    else ev = (double)src[tt];
+                    s_ev[0] = (Real)ev;
+                    if (keeps_source) strace[tt] = (double)(Real)ev;
+                }
+            }
+            const int pl = max(1, a - (sp - 1 - k)), pr = min(W - 1, e + (sp - k));
+            if constexpr (ONE) {
+                if (mine && c1 >= pl && c1 < pr) {
+                    p1 = exact_pressure_v(x, o1, p1, vx1, tVx[s1 - 1], vy1, tVy[s1 + TW], dp1, dx1, dxl1, dy1, dyb1);
+                    tP[s1] = p1;
+                }
+            } else {
+                const int pw = pr - pl;
+                for (int i = tid; i < Hi * pw; i += B) {
+                    const int rr = i / pw, r = 1 + rr, c = pl + (i - rr * pw);
+                    const int si = r * TW + (c - L);
+                    tP[si] = exact_pressure_at(x, tP, tVx, tVy, si, TW, r * W + c);
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                if (keeps_probe) probe[tt] = (double)tP[ly * TW + (lx - L)];
+                if constexpr (ONE) {
+                    if (source && tt + 1 < tend) {
+                        if (twomass) twomass_flow(tm, nx);
+                        if (k + 1 < sp) {
+                            const double pin = (double)tP[ey * TW + (ex - L)];
+                            const double ev = twomass ? twomass_finish(tm, nx, pin) : (double)src[tt + 1];
+                            s_ev[(k + 1) & 1] = (Real)ev;
+                            if (keeps_source) strace[tt + 1] = (double)(Real)ev;
+                        }
+                    }
+                } else {
+                    if (source && k + 1 < sp) pexc = (double)tP[ey * TW + (ex - L)];
+                }
+            }
+            {
+                const Real ev = source ? s_ev[ONE ? (k & 1) : 0] : (Real)0;
+                const int vl = pl, vr = min(W - 1, e + (sp - 1 - k));
+                if constexpr (ONE) {
+                    if (mine && c1 >= vl && c1 < vr) {
+                        exact_velocity_v(o1, vx1, vy1, p1, tP[s1 + 1], tP[s1 - TW], ev, z1, &vx1, &vy1);
+                        tVx[s1] = vx1;
+                        tVy[s1] = vy1;
+                    }
+                } else {
+                    const int vw = vr - vl;
+                    for (int i = tid; i < Hi * vw; i += B) {
+                        const int rr = i / vw, r = 1 + rr, c = vl + (i - rr * vw);
+                        const int si = r * TW + (c - L);
+                        exact_velocity_at(x, r * W + c, c, tVx[si], tVy[si], tP[si], tP[si + 1], tP[si - TW], ev,
+                                          &tVx[si], &tVy[si]);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if constexpr (ONE) {
+            if (mine && c1 >= a && c1 < e) {
+                nP[q1] = p1;
+                nVx[q1] = vx1;
+                nVy[q1] = vy1;
+            }
+        } else {
+            const int ow = max(0, e - a);
+            for (int i = tid; i < Hi * ow; i += B) {
+                const int rr = i / ow, r = 1 + rr, c = a + (i - rr * ow);
+                const int si = r * TW + (c - L), q = r * W + c;
+                nP[q] = tP[si];
+                nVx[q] = tVx[si];
+                nVy[q] = tVy[si];
             }
         }
-        grid_barrier(bar, G, spin_limit, b.status);
-        if (tid == 0) {
-            pexc = (double)nP[T.excitation];
-            if (g == 0) probe[t] = (double)nP[T.listener];
-        }
-        Real* tp = cur.P; cur.P = nP; nP = tp;
-        tp = cur.Vx; cur.Vx = nVx; nVx = tp;
-        tp = cur.Vy; cur.Vy = nVy; nVy = tp;
+        grid_barrier(bar, (unsigned)G, spin_limit, b.status);
+        Real* tp = cP; cP = nP; nP = tp;
+        tp = cVx; cVx = nVx; nVx = tp;
+        tp = cVy; cVy = nVy; nVy = tp;
     }
-    // the final state is in cur; bring it home if that is the second buffer
-    if (((tend - t0) & 1) && tend > t0) {
+    // the final state is in cP; bring it home if that is the second buffer
+    if (cP != b.P + co) {
+        const int ow = max(0, e - a);
         Real *hP = b.P + co, *hVx = b.Vx + co, *hVy = b.Vy + co;
-        int r = r0 + 1, c = cc0 + 1;
-        for (int i = c0i + tid; i < c1i; i += B) {
+        for (int i = tid; i < Hi * ow; i += B) {
+            const int rr = i / ow, r = 1 + rr, c = a + (i - rr * ow);
             const int q = r * W + c;
-            hP[q] = cur.P[q];
-            hVx[q] = cur.Vx[q];
-            hVy[q] = cur.Vy[q];
-            r += dr; c += dc;
-            if (c > Wi) { c -= Wi; ++r; }
+            hP[q] = cP[q];
+            hVx[q] = cVx[q];
+            hVy[q] = cVy[q];
         }
     }
-    if (g == 0 && tid == 0) {
-        b.pexc[tube] = pexc;
+    if (tid == 0 && keeps_source) {
+        if (tend > t0) b.pexc[tube] = (double)cP[T.excitation];
         if (twomass) b.tm[tube] = tm;
     }
 }
@@ -688,14 +816,30 @@ __global__ __launch_bounds__(1024) void k_step_exact_coop(const TubeDesc* __rest
 }  // namespace
 
 template <typename Real>
+size_t coop_tile_bytes(int H, int W, int groups, int s) {
+    return 3 * sizeof(Real) * (size_t)H * (size_t)std::min(W, coop_chunk(W, groups) + 2 * s);
+}
+
+template <typename Real>
 hipError_t launch_step_coop(const TubeDesc* d_tubes, const int* d_order, Bufs<Real> b, int t0, int nsteps,
-                            int threads, int groups, unsigned* bar, Real* P2, Real* Vx2, Real* Vy2, hipStream_t st) {
+                            int threads, int groups, int s, bool one, size_t tile_bytes, unsigned* bar, Real* P2,
+                            Real* Vx2, Real* Vy2, hipStream_t st) {
+    static std::atomic<int> configured[2] = {0, 0};   // dynamic LDS limit set, per instantiation
+    if (!configured[sizeof(Real) == 8].load()) {
+        for (const void* f : {(const void*)k_step_exact_coop<Real, true>, (const void*)k_step_exact_coop<Real, false>}) {
+            hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCoopTileMax);
+            if (e != hipSuccess) return e;
+        }
+        configured[sizeof(Real) == 8].store(1);
+    }
+    if (tile_bytes > kCoopTileMax || s < 1) return hipErrorInvalidValue;
+    const void* fn = one ? (const void*)k_step_exact_coop<Real, true> : (const void*)k_step_exact_coop<Real, false>;
     int spin_limit = 1 << 22;   // about a quarter of a second of polling
-    void* args[] = {(void*)&d_tubes, (void*)&d_order, (void*)&b, (void*)&t0, (void*)&nsteps,
+    void* args[] = {(void*)&d_tubes, (void*)&d_order, (void*)&b, (void*)&t0, (void*)&nsteps, (void*)&s,
                     (void*)&bar, (void*)&P2, (void*)&Vx2, (void*)&Vy2, (void*)&spin_limit};
     hipError_t e = hipMemsetAsync(bar, 0, 2 * sizeof(unsigned), st);
     if (e != hipSuccess) return e;
-    return hipLaunchCooperativeKernel((const void*)k_step_exact_coop<Real>, dim3(groups), dim3(threads), args, 0, st);
+    return hipLaunchCooperativeKernel(fn, dim3(groups), dim3(threads), args, tile_bytes, st);
 }
 
 template <typename Real>
@@ -767,7 +911,9 @@ hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tu
     template hipError_t launch_step<Real>(const TubeDesc*, const int*, int, Bufs<Real>, int, int,    \
                                           int, int, const int*, hipStream_t);                       \
     template hipError_t launch_step_coop<Real>(const TubeDesc*, const int*, Bufs<Real>, int, int, int, \
-                                               int, unsigned*, Real*, Real*, Real*, hipStream_t);  \
+                                               int, int, bool, size_t, unsigned*, Real*, Real*, Real*, \
+                                               hipStream_t);                                       \
+    template size_t coop_tile_bytes<Real>(int, int, int, int);                                      \
     template hipError_t launch_coeffs<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,       \
                                             Real*, hipStream_t);                                    \
     template hipError_t launch_depth_raw<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,    \
