@@ -222,7 +222,7 @@ def main():
         adjoint = adjoint_leg(args, np, torch, vocalsim, tubes, _abi)
 
     # ---------------- CPU baseline (oracle, one core) ----------------
-    cpu = None
+    cpu = cpu_structured = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
@@ -241,6 +241,23 @@ def main():
                "sample": f"FP64 oracle (flat arrays, -O2, 1 thread), 44-section vowel tube "
                          f"{t.H}x{t.W}: {n_cpu} steps in {runs} runs of up to {n_full} steps, {dt:.1f} s"}
         log(f"[bench] CPU oracle: {cpu['value']:.3e} cell-updates/s over {n_cpu} steps")
+        # the reference engine's own layout and precision: vector<vector<cell>>
+        # state, a deep copy per step and the inert wall sweep, FP32, timed
+        # over the step loop only (FDTDEngineCPU.cpp:254-292)
+        ts = orc.OracleTube(spec64, 32)
+        n_s, dt_s, runs_s = 0, 0.0, 0
+        while dt_s < args.cpu_seconds / 2 and runs_s < 20:
+            n = min(n_full, 400 if runs_s == 0 else n_full)
+            ts.reset()
+            _, secs = ts.run_structured(n)
+            dt_s += secs
+            n_s += n
+            runs_s += 1
+        cpu_structured = {"value": c64 * n_s / dt_s, "unit": "grid-cell-updates/s", "cores": 1,
+                          "sample": f"reference-structured FP32 engine (vector<vector<cell>>, per-step copy, "
+                                    f"wall sweep, -O2, 1 thread), same tube: {n_s} steps in {runs_s} runs, "
+                                    f"{dt_s:.1f} s of step loop"}
+        log(f"[bench] CPU reference-structured FP32: {cpu_structured['value']:.3e} cell-updates/s over {n_s} steps")
 
     out = {
         "metric": "grid-cell-updates/s (batched FP32 vs plain FP64) at <=1 Hz F1-F4 error",
@@ -270,6 +287,7 @@ def main():
                      "kernel": "k_fast_step<K> (batched FP32)", "mean_launch_ms": mean_launch_ms,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "cpu_reference_structured": cpu_structured,
         "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
