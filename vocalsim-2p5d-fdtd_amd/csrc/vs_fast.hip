@@ -32,6 +32,8 @@
 
 #include "vs_common.h"
 
+#include <atomic>
+
 namespace vs {
 
 namespace {
@@ -68,6 +70,7 @@ __device__ inline int sigma_code(int type) {
     }
 }
 
+#ifndef VS_FAST_SPLIT_K   // set by vs_fast_k19.hip, which builds one K of the step kernel only
 __global__ void k_fast_setup(const TubeDesc* __restrict__ tubes, Bufs<float> b, FastBufs f) {
     const TubeDesc& T = tubes[blockIdx.y];
     const int H = T.H, W = T.W, Wi = W - 2;
@@ -131,6 +134,8 @@ __global__ void k_vel_to_flux(const TubeDesc* __restrict__ tubes, Bufs<float> b,
     f.Uxf[q] = b.Vx[q] * b.dX[q];
     f.Uyf[q] = b.Vy[q] * b.dY[q];
 }
+
+#endif   // VS_FAST_SPLIT_K
 
 template <int K>
 __device__ inline uint32_t class_of(const uint32_t* cw, int k) {
@@ -553,6 +558,7 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
     }
 }
 
+#ifndef VS_FAST_SPLIT_K
 // Per-cell sums over the steps of one block.  gx work-groups per tube
 // stride over its slots s = k * T + t (real cells only), so the slab bases
 // are work-group uniform.  From the recorded P' and
@@ -627,30 +633,43 @@ void k_fast_adj_sums(const TubeDesc* __restrict__ tubes, const int* __restrict__
     }
 }
 
+#endif   // VS_FAST_SPLIT_K
+
+// Raises a kernel's dynamic LDS limit to the on-chip kernels' share of the
+// 160 KB (once per kernel; configure_fast does it ahead of graph capture).
+hipError_t allow_fast_lds(const void* fn, std::atomic<int>& done) {
+    if (done.load()) return hipSuccess;
+    const int dyn = 160 * 1024 - (int)fast_lds_static_bytes();
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    if (e == hipSuccess) done.store(1);
+    return e;
+}
+
+template <int K, bool REC>
+hipError_t configure_step() {
+    static std::atomic<int> done{0};
+    return allow_fast_lds((const void*)k_fast_step<K, REC>, done);
+}
+
+template <int K>
+hipError_t configure_rev() {
+    static std::atomic<int> done{0};
+    return allow_fast_lds((const void*)k_fast_reverse<K>, done);
+}
+
 template <int K>
 hipError_t configure_k() {
-    static bool configured = false;
-    if (!configured) {
-        const int dyn = 160 * 1024 - (int)fast_lds_static_bytes();
-        hipError_t e = hipFuncSetAttribute((const void*)k_fast_step<K, false>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)k_fast_step<K, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    dyn);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)k_fast_reverse<K>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    dyn);
-        if (e != hipSuccess) return e;
-        configured = true;
-    }
-    return hipSuccess;
+    hipError_t e = configure_step<K, false>();
+    if (e == hipSuccess) e = configure_step<K, true>();
+    if (e == hipSuccess) e = configure_rev<K>();
+    return e;
 }
 
 template <int K>
 hipError_t launch_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
                     int t0, int nsteps, int threads, int guard, const int* d_t0, hipStream_t st) {
     const size_t lds = fast_lds_bytes(K, threads, guard);
-    hipError_t e = configure_k<K>();
+    hipError_t e = configure_step<K, false>();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_fast_step<K, false>), dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0,
                        nsteps, d_t0, guard, (float*)nullptr, 0LL, (float*)nullptr);
@@ -661,7 +680,7 @@ template <int K>
 hipError_t launch_rec_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
                         FastAdj a, int t0, int nsteps, int threads, int guard, hipStream_t st) {
     const size_t lds = fast_lds_bytes(K, threads, guard);
-    hipError_t e = configure_k<K>();
+    hipError_t e = configure_step<K, true>();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_fast_step<K, true>), dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0,
                        nsteps, (const int*)nullptr, guard, a.rec, a.rec_tube, a.sink);
@@ -672,7 +691,7 @@ template <int K>
 hipError_t launch_rev_k(const TubeDesc* d_tubes, const int* d_order, int n, FastBufs f, FastAdj a, int s0,
                         int ns, int guard, hipStream_t st) {
     const size_t lds = fast_lds_bytes(K, a.lanes + 64, guard);
-    hipError_t e = configure_k<K>();
+    hipError_t e = configure_rev<K>();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_fast_reverse<K>, dim3(n), dim3(a.lanes), lds, st, d_tubes, d_order, f, a, s0, ns, guard);
     return hipGetLastError();
@@ -680,6 +699,20 @@ hipError_t launch_rev_k(const TubeDesc* d_tubes, const int* d_order, int n, Fast
 
 }  // namespace
 
+// K = kFastSplitK: the forward and recording step kernels of that K are
+// built in their own translation unit (vs_fast_k19.hip) under another
+// instruction scheduler.  Under the default one this K alone gets a
+// loop-carried register rotation (about 100 moves per step) and spills;
+// under the other, K = 15..18 and 23..24 would.  Its reverse kernel stays
+// here.
+constexpr int kFastSplitK = 19;
+hipError_t split_configure_step(bool rec);
+hipError_t split_launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
+                             int t0, int nsteps, int threads, int guard, const int* d_t0, hipStream_t st);
+hipError_t split_launch_record(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
+                               FastAdj a, int t0, int nsteps, int threads, int guard, hipStream_t st);
+
+#ifndef VS_FAST_SPLIT_K
 size_t fast_lds_bytes(int K, int threads, int guard) {
     const int T = threads - 64;   // cell threads
     return (size_t)(T + 2 * guard) * (2 * K + 1) * sizeof(float);
@@ -710,9 +743,15 @@ hipError_t configure_fast(int K) {
     switch (K) {
 #define VS_K(k) case k: return configure_k<k>();
         VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
-        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(19) VS_K(20)
+        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(20)
         VS_K(21) VS_K(22) VS_K(23) VS_K(24)
 #undef VS_K
+    case kFastSplitK: {
+        hipError_t e = split_configure_step(false);
+        if (e == hipSuccess) e = split_configure_step(true);
+        if (e == hipSuccess) e = configure_rev<kFastSplitK>();
+        return e;
+    }
     default: return hipErrorInvalidValue;
     }
 }
@@ -723,9 +762,11 @@ hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, 
     switch (K) {
 #define VS_K(k) case k: return launch_k<k>(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard, d_t0, st);
         VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
-        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(19) VS_K(20)
+        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(20)
         VS_K(21) VS_K(22) VS_K(23) VS_K(24)
 #undef VS_K
+    case kFastSplitK:
+        return split_launch_step(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard, d_t0, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -736,9 +777,11 @@ hipError_t launch_fast_record(int K, const TubeDesc* d_tubes, const int* d_order
     switch (K) {
 #define VS_K(k) case k: return launch_rec_k<k>(d_tubes, d_order, n, b, f, a, t0, nsteps, threads, guard, st);
         VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
-        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(19) VS_K(20)
+        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(20)
         VS_K(21) VS_K(22) VS_K(23) VS_K(24)
 #undef VS_K
+    case kFastSplitK:
+        return split_launch_record(d_tubes, d_order, n, b, f, a, t0, nsteps, threads, guard, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -762,5 +805,26 @@ hipError_t launch_fast_adj_sums(int K, const TubeDesc* d_tubes, const int* d_ord
                        source, s0, ns);
     return hipGetLastError();
 }
+
+
+#else   // VS_FAST_SPLIT_K: this unit provides the split K only
+
+static_assert(VS_FAST_SPLIT_K == kFastSplitK, "vs_fast_k19.hip builds kFastSplitK");
+
+hipError_t split_configure_step(bool rec) {
+    return rec ? configure_step<VS_FAST_SPLIT_K, true>() : configure_step<VS_FAST_SPLIT_K, false>();
+}
+
+hipError_t split_launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
+                             int t0, int nsteps, int threads, int guard, const int* d_t0, hipStream_t st) {
+    return launch_k<VS_FAST_SPLIT_K>(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard, d_t0, st);
+}
+
+hipError_t split_launch_record(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
+                               FastAdj a, int t0, int nsteps, int threads, int guard, hipStream_t st) {
+    return launch_rec_k<VS_FAST_SPLIT_K>(d_tubes, d_order, n, b, f, a, t0, nsteps, threads, guard, st);
+}
+
+#endif   // VS_FAST_SPLIT_K
 
 }  // namespace vs
