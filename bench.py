@@ -19,6 +19,7 @@ BYTES_FP32 = 37   # P, Vx, Vy read+write, dP, dX, dY read (4 B each), 1 B class
 BYTES_FP64 = 73
 HBM_PEAK_GBS = 8000.0
 BLOCK = 256
+GRAPH_BLOCKS = 4   # 256-step launches per captured step graph (the library's default)
 # HBM bytes per 256-step launch of k_fast_step<20> on the 256-tube batch at 960
 # threads (the autotuned shape), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # passes, each scaled by the same counter over a plain copy kernel of known size
@@ -35,7 +36,8 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed 256-step blocks")
-    ap.add_argument("--warmup", type=int, default=2, help="untimed 256-step blocks")
+    ap.add_argument("--warmup", type=int, default=4,
+                    help="untimed 256-step blocks (4 or more also captures the step graph before timing)")
     ap.add_argument("--tubes", type=int, default=256)
     ap.add_argument("--threads", type=int, default=0, help="0: autotuned / default")
     ap.add_argument("--kernel", type=int, default=-1, help="-1: best available")
@@ -154,6 +156,9 @@ def main():
                                             for c in rep.get("candidates", [])) + f" -> threads {threads}")
     cells = s.cells_per_step
     log(f"[bench] interior cells per step: {cells}, threads {threads}")
+    # 256-step blocks replayed as hipGraphs of GRAPH_BLOCKS launches (the
+    # remainder, if any, as ordinary launches); the warm-up captures the graph
+    s.set_option(_abi.OPT_USE_GRAPH, 1)
     s.run(args.warmup * BLOCK)
     torch.cuda.synchronize()
     s.set_option(_abi.OPT_TIMING, 0)
@@ -273,9 +278,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"batch of {args.tubes} ragged elliptical tubes 13-19 cm, 30-50 sections, "
-                               f"half Gaussian / half two-mass, FP32 state, 256-step blocks",
+                               f"half Gaussian / half two-mass, FP32 state, 256-step blocks replayed as "
+                               f"hipGraphs of {GRAPH_BLOCKS}",
                    "tubes": args.tubes, "cells_per_step": cells, "block_steps": BLOCK,
-                   "kernel": kernel, "threads": threads},
+                   "graph_blocks": GRAPH_BLOCKS, "kernel": kernel, "threads": threads},
         "fp64_plain": {"value": fp64_rate, "cells_per_step": c64,
                        "workload": "44-section vowel tube, two-mass 800 Pa, FP64, one tube"},
         "ratio_batched_fp32_over_plain_fp64": value / fp64_rate,
