@@ -251,9 +251,12 @@ struct Bufs {
 // Exact step: the 17 coefficients of the reference (FDTDSolver.cu:860-1056)
 // depend only on the (cur, right, top) cell types, so each tube holds them
 // per class: slots 0..16 the coefficients, 17 = Dirichlet cell, 18 = wall
-// cell carrying a boundary segment (z_inv applies).
+// cell carrying a boundary segment (z_inv applies), 19 = flags of the terms
+// the velocity update cannot skip (kExactSourceTerms, kExactWallTerms).
 constexpr int kExactClasses = 64;
-constexpr int kExactSlots = 20;
+constexpr int kExactSlots = 20;        // 17 coefficients, noPressure, wall, term flags
+constexpr int kExactSourceTerms = 1;   // term flags: the class has source coefficients
+constexpr int kExactWallTerms = 2;     //             the class carries a wall segment
 
 // Single-plane-symmetric cross-section of one tube section (vs_plane.hip):
 // the aligned closed contour (3997 points: 1999 on the upper half and their

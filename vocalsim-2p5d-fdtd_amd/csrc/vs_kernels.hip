@@ -352,7 +352,10 @@ __global__ void k_exact_classes(const TubeDesc* __restrict__ tubes, Bufs<Real> b
     for (int k = 0; k < 17; ++k) dst[k] = o[k];
     dst[17] = (Real)(tc == kNoPressure ? 1 : 0);
     dst[18] = (Real)wall;
-    dst[19] = 0;
+    // terms the velocity update may skip: bit 0 set when the source
+    // coefficients (11, 12) are not both zero, bit 1 when the cell carries a
+    // wall segment (z_inv applies)
+    dst[19] = (Real)(((o[11] != 0 || o[12] != 0) ? kExactSourceTerms : 0) | (wall ? kExactWallTerms : 0));
 }
 
 // ---------------------------------------------------------------------------
@@ -414,23 +417,33 @@ __device__ __forceinline__ Real exact_pressure_cell(const ExactView<Real>& x, in
 // New velocities of one cell (FDTDEngineCPU.cpp:364-410) from its input
 // velocities vxi, vyi, the new pressures pc (own), pr (right) and pt (top),
 // the source value ev, its column's z_inv zcol and its class coefficients
-// o; stored to vx_out, vy_out.
+// o; stored to vx_out, vy_out.  The source term (ev * o11 * o4) and the
+// wall term (o4 * (...) * o9 * z) are added only where the class flags in
+// o[19] say they can be non-zero: elsewhere each is a finite value times a
+// zero factor, and adding it changes nothing but, at most, the sign of a
+// zero velocity, which no later operation can turn into a different
+// non-zero value (the step only divides by positive class constants).
 template <typename Real>
 __device__ __forceinline__ void exact_velocity_v(const Real* o, Real vxi, Real vyi, Real pc, Real pr, Real pt,
                                                  Real ev, Real zcol, Real* vx_out, Real* vy_out) {
-    const Real z = (o[18] != 0) ? zcol : (Real)0;
+    const int terms = (int)o[19];
     const Real cxp = pr - pc;
     const Real cyp = pt - pc;
     Real vx = (o[2] * vxi) - (o[6] * cxp);
     Real vy = (o[3] * vyi) - (o[7] * cyp);
-    vx = vx + (ev * o[11] * o[4]);
-    vy = vy + (ev * o[12] * o[5]);
-    Real bx = (o[14] * pc * o[0]) - (o[13] * pr * o[1]);
-    Real by = (o[16] * pc * o[0]) - (o[15] * pt * o[1]);
-    bx = bx * o[9] * z;
-    by = by * o[10] * z;
-    vx += (o[4] * bx);
-    vy += (o[5] * by);
+    if (terms & kExactSourceTerms) {
+        vx = vx + (ev * o[11] * o[4]);
+        vy = vy + (ev * o[12] * o[5]);
+    }
+    if (terms & kExactWallTerms) {
+        const Real z = zcol;
+        Real bx = (o[14] * pc * o[0]) - (o[13] * pr * o[1]);
+        Real by = (o[16] * pc * o[0]) - (o[15] * pt * o[1]);
+        bx = bx * o[9] * z;
+        by = by * o[10] * z;
+        vx += (o[4] * bx);
+        vy += (o[5] * by);
+    }
     vx /= (o[2] + o[4]);
     vy /= (o[3] + o[5]);
     *vx_out = vx;
