@@ -651,6 +651,12 @@ __global__ __launch_bounds__(1024) void k_step_exact_coop(const TubeDesc* __rest
     __shared__ TwoMassState tm;
     __shared__ TwoMassNext nx;   // ONE: the next step's recurrence before its pressure input
     __shared__ Real s_ev[2];     // source value; ONE: of steps with even / odd k
+    // ONE: the period's table source values, and its probe and source-trace
+    // samples, kept in LDS and moved once per period: a work-group barrier
+    // waits for a wave's outstanding global loads and stores, so none may
+    // be issued inside the steps
+    __shared__ Real s_srcv[kCoopMaxSteps];
+    __shared__ double s_rec[2][kCoopMaxSteps];
     const int tid = threadIdx.x, B = blockDim.x, g = blockIdx.x;
     const int G = (int)gridDim.x;
     {
@@ -708,6 +714,7 @@ __global__ __launch_bounds__(1024) void k_step_exact_coop(const TubeDesc* __rest
             tVy[i] = cVy[q];
         }
         if (tid == 0 && source && t > t0) pexc = (double)cP[T.excitation];
+        if (ONE && source && !twomass && tid < sp) s_srcv[tid] = src[t + tid];
         __syncthreads();
         Real p1 = 0, vx1 = 0, vy1 = 0;   // ONE: this lane's cell
         if (mine) { p1 = tP[s1]; vx1 = tVx[s1]; vy1 = tVy[s1]; }
@@ -721,10 +728,10 @@ __global__ __launch_bounds__(1024) void k_step_exact_coop(const TubeDesc* __rest
                             if (t == t0) { twomass_solve(tm, nx); twomass_flow(tm, nx); }
                             ev = twomass_finish(tm, nx, pexc);
                         } else {
-                            ev = (double)src[tt];
+                            ev = (double)s_srcv[0];
                         }
                         s_ev[0] = (Real)ev;
-                        if (keeps_source) strace[tt] = (double)(Real)ev;
+                        s_rec[1][0] = (double)(Real)ev;
                     }
                     if (twomass && tt + 1 < tend) twomass_solve(tm, nx);
                 } else {
@@ -751,18 +758,19 @@ __global__ __launch_bounds__(1024) void k_step_exact_coop(const TubeDesc* __rest
             }
             __syncthreads();
             if (tid == 0) {
-                if (keeps_probe) probe[tt] = (double)tP[ly * TW + (lx - L)];
                 if constexpr (ONE) {
+                    s_rec[0][k] = (double)tP[ly * TW + (lx - L)];
                     if (source && tt + 1 < tend) {
                         if (twomass) twomass_flow(tm, nx);
                         if (k + 1 < sp) {
                             const double pin = (double)tP[ey * TW + (ex - L)];
-                            const double ev = twomass ? twomass_finish(tm, nx, pin) : (double)src[tt + 1];
+                            const double ev = twomass ? twomass_finish(tm, nx, pin) : (double)s_srcv[k + 1];
                             s_ev[(k + 1) & 1] = (Real)ev;
-                            if (keeps_source) strace[tt + 1] = (double)(Real)ev;
+                            s_rec[1][k + 1] = (double)(Real)ev;
                         }
                     }
                 } else {
+                    if (keeps_probe) probe[tt] = (double)tP[ly * TW + (lx - L)];
                     if (source && k + 1 < sp) pexc = (double)tP[ey * TW + (ex - L)];
                 }
             }
@@ -792,6 +800,10 @@ __global__ __launch_bounds__(1024) void k_step_exact_coop(const TubeDesc* __rest
                 nP[q1] = p1;
                 nVx[q1] = vx1;
                 nVy[q1] = vy1;
+            }
+            if (tid < sp) {
+                if (keeps_probe) probe[t + tid] = s_rec[0][tid];
+                if (keeps_source) strace[t + tid] = s_rec[1][tid];
             }
         } else {
             const int ow = max(0, e - a);
