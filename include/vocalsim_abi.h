@@ -140,8 +140,10 @@ enum vs_option {
     VS_OPT_EXACT_STEPS = 9       /* steps of that cooperative launch between two grid
                                     barriers: each work-group recomputes halo columns to run
                                     them without exchanging data.  0, the default: up to 8,
-                                    as many as 48 work-groups allow; 1..32 (lowered where
-                                    the LDS tile or the source placement needs it) */
+                                    as many as 48 work-groups allow at one cell per thread
+                                    (with VS_OPT_EXACT_GROUPS set: as many as that count
+                                    allows, else 1); 1..32 (lowered where the LDS tile or
+                                    the source placement needs it) */
 };
 
 enum vs_kernel {

@@ -694,8 +694,10 @@ struct CoopPlan {
 // columns and 2s - 1 more) at one cell per lane (coop_one_cell_per_lane),
 // with the fewest such work-groups; tubes of at most one cell per lane stay
 // on one work-group.  VS_OPT_EXACT_GROUPS / VS_OPT_EXACT_STEPS override
-// either.  Steps are then lowered until the LDS tile fits and every source
-// work-group sees the excitation column (coop_source_ok).
+// either; with the groups given and the steps not, the steps are the most
+// (up to kCoopSteps) that keep one cell per lane, else 1.  Steps are then
+// lowered until the LDS tile fits and every source work-group sees the
+// excitation column (coop_source_ok).
 constexpr int kCoopSteps = 8;
 constexpr int kCoopAutoGroups = 48;
 constexpr int kCoopThreads = 1024;   // work-group size of the cooperative step (VS_OPT_THREADS is for the others)
@@ -719,6 +721,12 @@ CoopPlan coop_plan(const vs_handle* h) {
     G = std::min(G, Wi);
     G = (Wi + coop_chunk(W, G) - 1) / coop_chunk(W, G);   // no empty work-groups
     if (G < 2) return p;
+    if (h->exact_groups > 0 && h->exact_steps == 0) {
+        // groups given, steps not: the most steps that keep one cell per lane
+        s = 1;
+        for (int ss = kCoopSteps; ss > 1; --ss)
+            if (coop_one_cell_per_lane(H, W, G, ss, kCoopThreads)) { s = ss; break; }
+    }
     while (s > 1 && (!coop_source_ok(W, G, s, ex) || coop_tile_bytes<Real>(H, W, G, s) > kCoopTileMax)) --s;
     if (coop_tile_bytes<Real>(H, W, G, s) > kCoopTileMax) return p;
     p.groups = G;

@@ -759,7 +759,7 @@ __global__ __launch_bounds__(1024) void k_step_exact_coop(const TubeDesc* __rest
             __syncthreads();
             if (tid == 0) {
                 if constexpr (ONE) {
-                    s_rec[0][k] = (double)tP[ly * TW + (lx - L)];
+                    if (keeps_probe) s_rec[0][k] = (double)tP[ly * TW + (lx - L)];
                     if (source && tt + 1 < tend) {
                         if (twomass) twomass_flow(tm, nx);
                         if (k + 1 < sp) {
