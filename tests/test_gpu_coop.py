@@ -61,6 +61,24 @@ def test_source_in_the_middle(hip_device, source):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_graph_replays_mix_with_cooperative_launches(hip_device, precision):
+    # graph replays keep the one-work-group kernel, the remainders between
+    # them run cooperatively; the source state handed from one to the other
+    # (excitation pressure, two-mass state) must keep the run bit-identical
+    spec = tubes.vowel_tube(6, num_steps=1500)
+    ref = run(spec, precision, 1)
+    print(f"[coop] graph mix precision={precision}", flush=True)
+    with Solver(spec, precision=precision, threads=1024, steps_per_launch=128) as s:
+        s.set_option(_abi.OPT_USE_GRAPH, 1)
+        for n in (300, 700, 500):   # cooperative; a 512-step replay + 188 cooperative; cooperative
+            s.run(n)
+        assert s.steps_done == 1500
+        got = s.probe(), s.field("pressure"), s.field("vx"), s.field("vy")
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
 def test_groups_match_the_oracle_fp64(hip_device):
     spec = tubes.vowel_tube(5, num_steps=600, source=tubes.SOURCE_TWO_MASS)
     probe, P, _, _ = run(spec, 64, 8, threads=512)
