@@ -611,21 +611,47 @@ void k_fast_adj_sums(const TubeDesc* __restrict__ tubes, const int* __restrict__
         // step rows walked backwards from uniform bases; the per-thread
         // offsets (s, sr, stp) stay fixed, and the top loads are always in
         // bounds (stp = s without a top neighbour)
-        const float* Rn = R + (long long)(tend - 1 - s0) * 2 * KT;
-        const float* An = A + (long long)(tend - 1 - s0) * KT;
-#pragma unroll 2
-        for (int n = tend - 1; n >= s0; --n, Rn -= 2 * KT, An -= KT) {
-            const float p = Rn[s], pr = Rn[sr], ptr = Rn[stp], dv = Rn[KT + s];
-            const float kc = An[s], kr = An[sr], ktr = An[stp];
-            const float pt = has_top ? ptr : 0.f, kt = has_top ? ktr : 0.f;
-            const float ps = soff ? 0.f : p;
-            const float prx = roff ? -src[n] : pr;
+        // the step rows are loaded two steps ahead of their sums (the source
+        // value included, so that no later load forces a wait for all of
+        // them): the loop is unrolled by three over the rows r0, r1, r2, so a
+        // row keeps its registers from its load to its use, with the rows of
+        // the next two steps in flight.  The step index is clamped at the
+        // block's first step (every load in bounds).
+        struct Row { float p, pr, ptr, dv, kc, kr, ktr, ev; };
+        auto load = [&](int n) {
+            const int m = max(n, s0);
+            const long long i = (long long)(m - s0);
+            const float* Rn = R + i * 2 * KT;
+            const float* An = A + i * KT;
+            return Row{Rn[s], Rn[sr], Rn[stp], Rn[KT + s], An[s], An[sr], An[stp], src[m]};
+        };
+        auto sum = [&](const Row& c) {
+            const float pt = has_top ? c.ptr : 0.f, kt = has_top ? c.ktr : 0.f;
+            const float ps = soff ? 0.f : c.p;
+            const float prx = roff ? -c.ev : c.pr;
             const float fx = ps - prx, fy = ps - pt;
             sx += (double)lx * (double)fx;
             sy += (double)ly * (double)fy;
-            sp += (double)kc * (double)dv;
-            lx = adj_face(ax, lx, gx, kr, kc);
-            ly = adj_face(ay, ly, gy, kt, kc);
+            sp += (double)c.kc * (double)c.dv;
+            lx = adj_face(ax, lx, gx, c.kr, c.kc);
+            ly = adj_face(ay, ly, gy, kt, c.kc);
+        };
+        // (a scheduling barrier after each row's loads keeps the compiler
+        // from sinking them next to their use)
+        Row r0 = load(tend - 1), r1 = load(tend - 2), r2;
+        for (int n = tend - 1;; n -= 3) {
+            r2 = load(n - 2);
+            __builtin_amdgcn_sched_barrier(0);
+            sum(r0);
+            if (n - 1 < s0) break;
+            r0 = load(n - 3);
+            __builtin_amdgcn_sched_barrier(0);
+            sum(r1);
+            if (n - 2 < s0) break;
+            r1 = load(n - 4);
+            __builtin_amdgcn_sched_barrier(0);
+            sum(r2);
+            if (n - 3 < s0) break;
         }
         a.Lx[q] += sx;
         a.Ly[q] += sy;
