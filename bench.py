@@ -127,7 +127,7 @@ def main():
         copy_calibration(args.copy_mib)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(tubes would shard by batch).", file=sys.stderr)
         sys.exit(2)
     import numpy as np

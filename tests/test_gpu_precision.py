@@ -71,9 +71,11 @@ def test_fp32_formant_error_below_1hz(runs):
                      "B64": B64.tolist(), "dB": (B32 - B64).tolist()})
         print(f"  tube {i:2d} relL2 {rows[-1]['rel_l2']:.2e} F {np.round(F64, 1)} "
               f"dF {np.round(F32 - F64, 4)} dB {np.round(B32 - B64, 3)}", flush=True)
-    os.makedirs("gpurun_out", exist_ok=True)
-    with open("gpurun_out/precision_f1f4.json", "w") as f:
-        json.dump(rows, f, indent=1)
+    out_dir = os.environ.get("VOCALSIM_RESULTS_DIR")   # optional per-tube table
+    if out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+        with open(os.path.join(out_dir, "precision_f1f4.json"), "w") as f:
+            json.dump(rows, f, indent=1)
     dF = np.array([r["dF"] for r in rows])
     assert np.all(np.isfinite(dF))
     assert np.max(np.abs(dF)) <= 1.0
