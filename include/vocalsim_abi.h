@@ -137,13 +137,17 @@ enum vs_option {
                                     VS_OPT_EXACT_STEPS hold one cell per thread (one
                                     work-group when the tube itself does); 1: one
                                     work-group; 2..64 */
-    VS_OPT_EXACT_STEPS = 9       /* steps of that cooperative launch between two grid
+    VS_OPT_EXACT_STEPS = 9,      /* steps of that cooperative launch between two grid
                                     barriers: each work-group recomputes halo columns to run
                                     them without exchanging data.  0, the default: up to 8,
                                     as many as 48 work-groups allow at one cell per thread
                                     (with VS_OPT_EXACT_GROUPS set: as many as that count
                                     allows, else 1); 1..32 (lowered where the LDS tile or
                                     the source placement needs it) */
+    VS_OPT_BARRIER_SPIN = 10     /* polls a grid barrier of that cooperative launch makes
+                                    before it gives up: the launch then finishes without
+                                    synchronising and vs_sync reports VS_E_DEVICE.  0, the
+                                    default: about a quarter of a second; >= 1: that many */
 };
 
 enum vs_kernel {

@@ -79,6 +79,29 @@ def test_graph_replays_mix_with_cooperative_launches(hip_device, precision):
         assert np.array_equal(a, b)
 
 
+def test_barrier_timeout_is_reported_and_recoverable(hip_device):
+    # the grid barrier's wait is bounded: with a limit of one poll it gives
+    # up at once (every index is independent of the data, so the launch
+    # still runs to its end), vs_sync reports the timeout, and after a reset
+    # at the default limit the handle reproduces the reference bit for bit
+    spec = tubes.vowel_tube(2, num_steps=600, source=tubes.SOURCE_GAUSSIAN)
+    ref = run(spec, 64, 1)
+    with Solver(spec, precision=64, threads=1024, steps_per_launch=300) as s:
+        assert s.get_option(_abi.OPT_BARRIER_SPIN) == 0
+        with pytest.raises(Exception):
+            s.set_option(_abi.OPT_BARRIER_SPIN, -1)
+        s.set_option(_abi.OPT_BARRIER_SPIN, 1)
+        print("[coop] barrier spin limit 1", flush=True)
+        with pytest.raises(_abi.VocalSimError):
+            s.run()
+        s.set_option(_abi.OPT_BARRIER_SPIN, 0)
+        s.reset()
+        s.run()
+        got = s.probe(), s.field("pressure"), s.field("vx"), s.field("vy")
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
 def test_groups_match_the_oracle_fp64(hip_device):
     spec = tubes.vowel_tube(5, num_steps=600, source=tubes.SOURCE_TWO_MASS)
     probe, P, _, _ = run(spec, 64, 8, threads=512)

@@ -326,6 +326,7 @@ struct vs_handle {
     int adj_kernel = 0;        // VS_OPT_ADJOINT_KERNEL
     int exact_groups = 0;      // VS_OPT_EXACT_GROUPS (0: automatic, coop_plan)
     int exact_steps = 0;       // VS_OPT_EXACT_STEPS (0: automatic, coop_plan)
+    int barrier_spin = 0;      // VS_OPT_BARRIER_SPIN (0: kCoopSpinDefault)
     unsigned* d_bar = nullptr; // grid barrier state of the cooperative exact step
     void* d_state2 = nullptr;  // its second state buffers: P, Vx, Vy (Real, total_cells each)
     int coop_ok = -1;          // device supports cooperative launches (-1: not asked yet)
@@ -758,7 +759,8 @@ hipError_t launch_exact(vs_handle* h, Bufs<Real>& b, int t0, int ns, const int* 
         const long long C = h->total_cells;
         const CoopPlan p = coop_plan<Real>(h);
         hipError_t e = launch_step_coop<Real>(h->d_desc, h->d_order, b, t0, ns, kCoopThreads, p.groups, p.steps, p.one,
-                                              p.tile, h->d_bar, st2, st2 + C, st2 + 2 * C, h->stream);
+                                              p.tile, h->d_bar, st2, st2 + C, st2 + 2 * C,
+                                              h->barrier_spin > 0 ? h->barrier_spin : kCoopSpinDefault, h->stream);
         if (e != hipErrorCooperativeLaunchTooLarge) return e;
         // the device cannot hold the groups at once: one work-group from now on
         (void)hipGetLastError();
@@ -1207,6 +1209,9 @@ int vs_set_option(vs_handle* h, int option, int value) {
     case VS_OPT_EXACT_STEPS:
         if (value < 0 || value > kCoopMaxSteps) return fail(VS_E_INVALID, "exact steps must be in [0, 32]");
         h->exact_steps = value; return VS_OK;
+    case VS_OPT_BARRIER_SPIN:
+        if (value < 0) return fail(VS_E_INVALID, "barrier spin must be >= 0");
+        h->barrier_spin = value; return VS_OK;
     case VS_OPT_ADJOINT_BLOCK:
         if (value < 0) return fail(VS_E_INVALID, "adjoint block must be >= 0");
         if (value != h->adj_block) {   // slabs are re-sized by the next forward pass
@@ -1238,6 +1243,7 @@ int vs_get_option(const vs_handle* h, int option, int* value) {
     case VS_OPT_ADJOINT_BLOCK: *value = h->adj_block; return VS_OK;
     case VS_OPT_EXACT_GROUPS: *value = h->exact_groups; return VS_OK;
     case VS_OPT_EXACT_STEPS: *value = h->exact_steps; return VS_OK;
+    case VS_OPT_BARRIER_SPIN: *value = h->barrier_spin; return VS_OK;
     default: return fail(VS_E_INVALID, "unknown option");
     }
 }

@@ -378,11 +378,12 @@ hipError_t launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<
 // launch, s steps per grid barrier (one: coop_one_cell_per_lane; bar: two
 // words of barrier state, zeroed by the launch; P2, Vx2, Vy2: second state
 // buffers, cell layout, zero frame ring; tile_bytes: coop_tile_bytes, at
-// most kCoopTileMax).
+// most kCoopTileMax; spin_limit: polls of a barrier before it gives up).
 template <typename Real>
 hipError_t launch_step_coop(const TubeDesc* d_tubes, const int* d_order, Bufs<Real> b, int t0, int nsteps,
                             int threads, int groups, int s, bool one, size_t tile_bytes, unsigned* bar, Real* P2,
-                            Real* Vx2, Real* Vy2, hipStream_t st);
+                            Real* Vx2, Real* Vy2, int spin_limit, hipStream_t st);
+constexpr int kCoopSpinDefault = 1 << 22;   // grid-barrier polls before giving up: about a quarter of a second
 // LDS tile of one work-group of that step: P, Vx, Vy of H rows by the
 // widest column range (own columns plus s on each side).
 template <typename Real>

@@ -848,7 +848,7 @@ size_t coop_tile_bytes(int H, int W, int groups, int s) {
 template <typename Real>
 hipError_t launch_step_coop(const TubeDesc* d_tubes, const int* d_order, Bufs<Real> b, int t0, int nsteps,
                             int threads, int groups, int s, bool one, size_t tile_bytes, unsigned* bar, Real* P2,
-                            Real* Vx2, Real* Vy2, hipStream_t st) {
+                            Real* Vx2, Real* Vy2, int spin_limit, hipStream_t st) {
     static std::atomic<int> configured[2] = {0, 0};   // dynamic LDS limit set, per instantiation
     if (!configured[sizeof(Real) == 8].load()) {
         for (const void* f : {(const void*)k_step_exact_coop<Real, true>, (const void*)k_step_exact_coop<Real, false>}) {
@@ -859,7 +859,6 @@ hipError_t launch_step_coop(const TubeDesc* d_tubes, const int* d_order, Bufs<Re
     }
     if (tile_bytes > kCoopTileMax || s < 1) return hipErrorInvalidValue;
     const void* fn = one ? (const void*)k_step_exact_coop<Real, true> : (const void*)k_step_exact_coop<Real, false>;
-    int spin_limit = 1 << 22;   // about a quarter of a second of polling
     void* args[] = {(void*)&d_tubes, (void*)&d_order, (void*)&b, (void*)&t0, (void*)&nsteps, (void*)&s,
                     (void*)&bar, (void*)&P2, (void*)&Vx2, (void*)&Vy2, (void*)&spin_limit};
     hipError_t e = hipMemsetAsync(bar, 0, 2 * sizeof(unsigned), st);
@@ -936,7 +935,7 @@ hipError_t launch_coeffs(const TubeDesc* d_tubes, const TubeDesc* h_tube, int tu
     template hipError_t launch_step<Real>(const TubeDesc*, const int*, int, Bufs<Real>, int, int,    \
                                           int, int, const int*, hipStream_t);                       \
     template hipError_t launch_step_coop<Real>(const TubeDesc*, const int*, Bufs<Real>, int, int, int, \
-                                               int, int, bool, size_t, unsigned*, Real*, Real*, Real*, \
+                                               int, int, bool, size_t, unsigned*, Real*, Real*, Real*, int, \
                                                hipStream_t);                                       \
     template size_t coop_tile_bytes<Real>(int, int, int, int);                                      \
     template hipError_t launch_coeffs<Real>(const TubeDesc*, const TubeDesc*, int, Bufs<Real>,       \
