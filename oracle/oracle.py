@@ -74,6 +74,8 @@ def lib():
         L.vso_run_structured.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D]
         L.vso_float_num_steps.restype = ctypes.c_int
         L.vso_float_num_steps.argtypes = [ctypes.c_float, ctypes.c_int]
+        L.vso_twomass_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                      _D, ctypes.c_int, _D]
         L.vso_coeff_triple.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
         L.vso_plane_contour.argtypes = [ctypes.c_float, _F, _F, ctypes.c_int, _D]
         L.vso_polygon_area.restype = ctypes.c_float
@@ -175,6 +177,17 @@ class OracleTube:
 
 def float_num_steps(audio_dur: float, srate_mul: int = 15) -> int:
     return int(lib().vso_float_num_steps(audio_dur, srate_mul))
+
+
+def twomass_run(p1, precision: int = 32, q: float = 1.0, ps: float = 686.46, srate_mul: int = 15,
+                abs_double: bool = False) -> np.ndarray:
+    """Volume velocity of the two-mass recurrence fed the supraglottal
+    pressures p1 (one per step)."""
+    p1 = np.ascontiguousarray(p1, dtype=np.float64)
+    ug = np.zeros(len(p1))
+    if lib().vso_twomass_run(precision, q, ps, srate_mul, int(abs_double), _dptr(p1), len(p1), _dptr(ug)) != 0:
+        raise OracleError("vso_twomass_run: bad arguments")
+    return ug
 
 
 def coeff_triple(cur: int, right: int, top: int, precision: int = 64, srate_mul: int = 15) -> np.ndarray:

@@ -536,6 +536,7 @@ struct TwoMass {
         r1_close, r2_close, A1, Ag1, Ag2, ps, p1, f_mass1, f_mass2, p_mass1, p_mass2,
         Rv1, Rv2, Lg1, Lg2, ug_old, ug_curr, ug_next, x1_old, x1_curr, x1_next,
         x2_old, x2_curr, x2_next, a11, a12, a21, a22, b1, b2, s1p, s2p;
+    bool abs_double = false;   // fabs resolution of the compiler (see run())
 
     void init(Real q, Real ps_in) {
         using namespace fu;
@@ -632,7 +633,14 @@ struct TwoMass {
         if (Ag1 < 0 || Ag2 < 0) {
             ug_next = 0;
         } else {
-            Real Rtot = (rho / 2) * std::fabs(ug_curr) * ((0.37 / (Ag1 * Ag1)) + (1 - 2 * (Ag2 / A1) * (1 - Ag2 / A1)) / (Ag2 * Ag2))
+            // `(rho / 2)*fabs(ug_curr)` (TwoMassModel.cpp:221): MSVC, the
+            // reference's compiler, resolves fabs(float) to the float overload,
+            // so the product rounds to float before meeting the double 0.37
+            // term; glibc's global fabs under g++ is double fabs, which keeps
+            // the product exact (abs_double = true reproduces that build)
+            const double lead = abs_double ? (double)(rho / 2) * std::fabs((double)ug_curr)
+                                           : (double)((rho / 2) * std::fabs(ug_curr));
+            Real Rtot = lead * ((0.37 / (Ag1 * Ag1)) + (1 - 2 * (Ag2 / A1) * (1 - Ag2 / A1)) / (Ag2 * Ag2))
                         + (12 * mu * lg * lg * (d1 / (Ag1 * Ag1 * Ag1) + d2 / (Ag2 * Ag2 * Ag2)));
             Real Ltot = rho * (d1 / Ag1 + d2 / Ag2);
             ug_next = ((ps - p1) / srate + Ltot * ug_curr) / (Rtot / srate + Ltot);
@@ -1235,6 +1243,29 @@ int vso_run_structured(void* h, int n_steps, double* probe, double* seconds) {
 
 int vso_float_num_steps(float audio_dur, int srate_mul) {
     return float_num_steps(audio_dur, phys_f32(srate_mul).dt);
+}
+
+/* The two-mass recurrence alone, driven as FDTDEngineCPU.cpp:259-267 drives
+   it: p1 = the supraglottal pressure of each step, then one run, then the
+   volume velocity.  q = 1 and ps = 686.46 are the reference's own parameters
+   (TwoMassModel.cpp:10, 61). */
+int vso_twomass_run(int precision, double q, double ps, int srate_mul, int abs_double, const double* p1, int n,
+                    double* ug) {
+    if (n < 0 || (n > 0 && (!p1 || !ug))) return -1;
+    if (precision == 32) {
+        TwoMass<float> m;
+        m.init((float)q, (float)ps);
+        m.abs_double = abs_double != 0;
+        for (int i = 0; i < n; ++i) { m.p1 = (float)p1[i]; m.run(srate_mul); ug[i] = m.ug_next; }
+    } else if (precision == 64) {
+        TwoMass<double> m;
+        m.init(q, ps);
+        m.abs_double = abs_double != 0;
+        for (int i = 0; i < n; ++i) { m.p1 = p1[i]; m.run(srate_mul); ug[i] = m.ug_next; }
+    } else {
+        return -1;
+    }
+    return 0;
 }
 
 int vso_coeff_triple(int precision, int srate_mul, int cur, int right, int top, double* out17) {

@@ -96,6 +96,12 @@ int   vso_run_structured(void* h, int n_steps, double* probe, double* seconds);
 /* Reference float32 time-step count for a duration (SolverParams.cu:59-66). */
 int   vso_float_num_steps(float audio_dur, int srate_mul);
 
+/* The two-mass recurrence alone (TwoMassModel.cpp:90-231), fed p1[i] before
+ * step i as FDTDEngineCPU.cpp:259-267 does; writes the volume velocity of
+ * every step.  abs_double selects g++'s resolution of fabs(float) (double)
+ * instead of MSVC's (float), the reference's compiler. */
+int   vso_twomass_run(int precision, double q, double ps, int srate_mul, int abs_double, const double* p1, int n,
+                      double* ug);
 /* 17 solver coefficients of one (cur, right, top) type triple. */
 int   vso_coeff_triple(int precision, int srate_mul, int cur, int right, int top, double* out17);
 

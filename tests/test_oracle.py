@@ -1,6 +1,9 @@
 """Pins of the CPU oracle (oracle/vs_oracle.cpp) against what the reference
-and the mathematics fix.  The reference ships no tests or golden vectors, so
-the oracle is pinned by: values the reference's formulas produce and that are
+and the mathematics fix.  The reference ships no tests or golden vectors.
+The one part of it that compiles alone, the two-mass model
+(src/TwoMassModel.cpp), pins the oracle's recurrence bit for bit through
+golden vectors written by that compiled code (tests/golden/twomass/).  The
+rest of the oracle is pinned by: values the reference's formulas produce and that are
 quoted independently (z_inv, frame sizes, float32 step counts); closed forms
 (chord depths of circles and ellipses, the Gaussian pulse, quarter-wave
 resonances of a uniform tube); invariants (mirror symmetry, sealed walls,
@@ -8,7 +11,9 @@ border and Dirichlet cells stay zero, bounded energy of a lossless tube,
 power-of-two depth scaling); and an independent vectorised numpy restatement
 of one time step on small frames (catches index/sign/operand mistakes).
 """
+import ctypes
 import math
+import os
 
 import numpy as np
 import pytest
@@ -256,6 +261,52 @@ def test_two_mass_pitch_scales_with_q():
 def test_two_mass_no_pressure_no_flow():
     src = _two_mass_flow(1e-30, 1.0, n=2000)
     assert np.max(np.abs(src)) < 1e-20
+
+
+GOLD_TM = os.path.join(os.path.dirname(__file__), "golden", "twomass", "twomass_ref_fp32.npz")
+REF_TM = os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "libref_twomass.so")
+
+
+def test_two_mass_matches_reference_golden_vectors():
+    # Volume velocity of the reference's own TwoMassModel.cpp, compiled with
+    # g++ (tests/golden/twomass/make_twomass_golden.py), for 16 000 steps of a
+    # zero and a noise supraglottal pressure.  The oracle's FP32 recurrence,
+    # with fabs(float) resolved as g++ resolves it, equals it bit for bit.
+    g = np.load(GOLD_TM)
+    n = len(g["ug_zero"])
+    for feed, p1 in (("zero", np.zeros(n)), ("noise", g["p1_noise"].astype(np.float64))):
+        ug = orc.twomass_run(p1, precision=32, abs_double=True).astype(np.float32)
+        bad = np.nonzero(ug != g["ug_" + feed])[0]
+        assert bad.size == 0, f"{feed}: first mismatch at step {bad[0]}"
+    # the run covers the folds' first closure (flow exactly 0) and reopening
+    z = np.nonzero(g["ug_zero"] == 0)[0]
+    assert 0 < z[0] < z[-1] < n - 1000 and (g["ug_zero"][z[-1] + 1:] > 0).all()
+
+
+def test_two_mass_fabs_resolution_matters():
+    # MSVC (the reference's compiler) resolves fabs(float) to the float
+    # overload, g++ to glibc's double fabs; the oracle defaults to MSVC's.
+    # They agree until the first rounding of (rho/2)*|ug| differs, then drift.
+    g = np.load(GOLD_TM)
+    ug = orc.twomass_run(np.zeros(len(g["ug_zero"])), precision=32).astype(np.float32)
+    ref = g["ug_zero"]
+    assert np.array_equal(ug[:200], ref[:200])
+    assert not np.array_equal(ug, ref)
+    assert np.max(np.abs(ug - ref)) < 1e-2 * np.max(ref)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TM), reason="oracle/_ref not built (make -C oracle ref)")
+def test_two_mass_matches_reference_build():
+    # the same check live against oracle/_ref/libref_twomass.so, on a feed the
+    # golden vectors do not hold: a two-tone pressure, 20 000 steps
+    lib = ctypes.CDLL(REF_TM)
+    n = 20000
+    t = np.arange(n) / 661500.0
+    p1 = (300 * np.sin(2 * np.pi * 500 * t) + 100 * np.sin(2 * np.pi * 1700 * t)).astype(np.float32)
+    ref = np.zeros(n, np.float32)
+    assert lib.ref_twomass_run(15, p1.ctypes.data_as(ctypes.c_void_p), n, ref.ctypes.data_as(ctypes.c_void_p)) == 0
+    ug = orc.twomass_run(p1.astype(np.float64), precision=32, abs_double=True).astype(np.float32)
+    assert np.array_equal(ug, ref)
 
 
 # ---------------------------------------------------------------------- step
