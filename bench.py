@@ -20,12 +20,12 @@ BYTES_FP64 = 73
 HBM_PEAK_GBS = 8000.0
 BLOCK = 256
 GRAPH_BLOCKS = 4   # 256-step launches per captured step graph (the library's default)
-# HBM bytes per 256-step launch of k_fast_step<20> on the 256-tube batch at 960
+# HBM bytes per 256-step launch of k_fast_step<19> on the 256-tube batch at 1024
 # threads (the autotuned shape), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # passes, each scaled by the same counter over a plain copy kernel of known size
 # (FETCH x2.000, WRITE x1.000); profiles/k_fast_step_batch256_fp32_pmc.md.
-PMC_TRAFFIC_BYTES = 1.851e9
-PMC_THREADS = 960
+PMC_TRAFFIC_BYTES = 1.834e9
+PMC_THREADS = 1024
 
 
 def log(msg):

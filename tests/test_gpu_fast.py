@@ -105,22 +105,26 @@ def test_fast_rejects_fp64(hip_device):
             s.set_option(_abi.OPT_KERNEL, _abi.K_FAST)
 
 
-def test_fast_cells_per_lane_19(hip_device):
-    # tube 213 of the bench batch: 17 485 padded cells, so K = 19 at 1024
-    # threads (the instantiation built in its own unit under another
-    # scheduler) and K = 20 at 960; the flux kernel's results do not depend
-    # on the work-group shape, and both track the FP64 oracle
-    spec = tubes.batch_tubes(256, seed=0, num_steps=700)[213]
+def test_fast_odd_cells_per_lane(hip_device):
+    # tube 31 of the bench batch: 16 185 padded cells, so K = 17, 19, 21 and
+    # 23 at 1024, 960, 896 and 832 threads (K is rounded up to odd; K = 17 is
+    # the instantiation built in its own unit under another scheduler).  The
+    # flux kernel's results do not depend on the work-group shape, and all
+    # track the FP64 oracle
+    spec = tubes.batch_tubes(256, seed=0, num_steps=700)[31]
+    want = {1024: 17, 960: 19, 896: 21, 832: 23}
     runs = {}
-    for threads in (1024, 960):
-        print(f"[fast K19] threads={threads}", flush=True)
+    for threads, K in want.items():
+        print(f"[fast odd K] threads={threads} K={K}", flush=True)
         with Solver(spec, precision=32, threads=threads, kernel=_abi.K_FAST, steps_per_launch=256) as s:
             d = s.dims[0]
             padded = (d["frameH"] - 2) * (d["frameW"] - 1)
-            assert -(-padded // (threads - 64)) == (19 if threads == 1024 else 20)
+            k = -(-padded // (threads - 64))
+            assert k + (k % 2 == 0) == K
             s.run()
             runs[threads] = (s.probe(), s.field("pressure"), s.field("vx"), s.field("vy"))
-    for a, b in zip(runs[1024], runs[960]):
-        assert np.array_equal(a, b)
+    for threads in (960, 896, 832):
+        for a, b in zip(runs[1024], runs[threads]):
+            assert np.array_equal(a, b)
     p64, _ = orc.OracleTube(spec, 64).run()
     assert rel_l2(runs[1024][0], p64) <= 1e-4

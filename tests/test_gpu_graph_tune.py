@@ -29,6 +29,22 @@ def test_graph_replay_equals_launches(hip_device, precision, kernel):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_run_that_launches_nothing_syncs(hip_device, graph):
+    # a zero-step run on a fresh handle (and a graph-mode run with no step
+    # left) records no events: vs_sync succeeds and reports zero time
+    spec = tubes.uniform_tube(num_steps=300)
+    print(f"[zero-step run] graph={graph}", flush=True)
+    with Solver(spec, precision=32, threads=256, steps_per_launch=128) as s:
+        s.set_option(_abi.OPT_USE_GRAPH, graph)
+        s.run(0)
+        assert s.timing()[0] == 0.0 and s.steps_done == 0
+        s.run(300)
+        assert s.steps_done == 300
+        s.run(50)   # nothing left to run
+        assert s.steps_done == 300 and s.timing()[0] == 0.0
+
+
 def test_fast_kernel_independent_of_workgroup_shape(hip_device):
     specs = tubes.batch_tubes(6, seed=6, num_steps=500)
     probes = []

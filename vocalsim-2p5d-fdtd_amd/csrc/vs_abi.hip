@@ -294,6 +294,7 @@ struct vs_handle {
     long long total_cells = 0, total_cols = 0, total_walls = 0, total_secs = 0, total_src = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ev_recorded = false;  // the last run recorded ev0 .. ev1 (a run that launches nothing does not)
     double last_ms = 0, last_mean_launch_ms = 0;
     int last_launches = 0;
     std::vector<TwoMassState> tm0;   // initial two-mass states (for vs_reset)
@@ -621,11 +622,11 @@ int build_fast_classes(vs_handle* h) {
 }
 
 // Whether the batch fits the on-chip kernels at `threads` threads per tube
-// (cells per lane K <= 24, LDS within the device limit, one srate_mul).
+// (cells per lane K <= kFastMaxK, LDS within the device limit, one srate_mul).
 bool fast_fits(const vs_handle* h, int threads, int* K_out, int* G_out) {
     if (h->precision != VS_FP32 || threads < 128) return false;
     const int K = fast_cells_per_lane(h->max_padded, threads);
-    if (K > 24) return false;
+    if (K > kFastMaxK) return false;
     const int G = fast_guard_lanes(K, h->max_wp);
     if ((long long)(fast_lds_bytes(K, threads, G) + fast_lds_static_bytes()) > h->lds_limit) return false;
     for (const TubeDesc& d : h->desc)
@@ -640,7 +641,7 @@ int prepare_fast(vs_handle* h, int threads) {
     if (h->precision != VS_FP32) return fail(VS_E_INVALID, "the flux-form kernel is FP32 only");
     if (threads < 128) return fail(VS_E_INVALID, "the on-chip kernel needs at least 128 threads");
     const int K = fast_cells_per_lane(h->max_padded, threads);
-    if (K > 24) return fail(VS_E_INVALID, "tube too large for the on-chip kernel at this work-group size");
+    if (K > kFastMaxK) return fail(VS_E_INVALID, "tube too large for the on-chip kernel at this work-group size");
     const int G = fast_guard_lanes(K, h->max_wp);
     if ((long long)(fast_lds_bytes(K, threads, G) + fast_lds_static_bytes()) > h->lds_limit)
         return fail(VS_E_INVALID, "on-chip kernel needs " + std::to_string(fast_lds_bytes(K, threads, G)) +
@@ -819,6 +820,8 @@ int run_steps(vs_handle* h, int n_steps) {
         if (rc != VS_OK) return rc;
     }
     bool graph_ran = false;
+    h->ev_recorded = false;
+    h->last_launches = 0;
     if (h->use_graph && !h->timing) {
         // whole graph replays (graph_blocks x steps_per_launch steps each), then
         // the remainder with ordinary launches
@@ -835,6 +838,7 @@ int run_steps(vs_handle* h, int n_steps) {
             VS_HIP(hipEventRecord(h->ev0, h->stream));
             for (int k = 0; k < reps; ++k) VS_HIP(hipGraphLaunch(h->graph_exec, h->stream));
             VS_HIP(hipEventRecord(h->ev1, h->stream));
+            h->ev_recorded = true;
             if (h->kernel != VS_K_FAST) h->flux_valid = false;
             h->steps_done = start + reps * per;
             h->last_launches = reps * h->graph_blocks;
@@ -847,6 +851,7 @@ int run_steps(vs_handle* h, int n_steps) {
     }
     const int start = h->steps_done;
     const int end = std::min(h->max_steps, start + n_steps);
+    if (end <= start) return VS_OK;   // nothing to run (no graph replay ran either)
     int launches = 0;
     double launch_ms = 0;
     std::vector<hipEvent_t> evs;
@@ -864,6 +869,7 @@ int run_steps(vs_handle* h, int n_steps) {
         ++launches;
     }
     VS_HIP(hipEventRecord(h->ev1, h->stream));
+    h->ev_recorded = true;
     h->steps_done = end;
     h->last_launches = launches + (graph_ran ? h->last_launches : 0);
     if (h->timing) {
@@ -1131,7 +1137,7 @@ int vs_sync(vs_handle* h) {
         }
     }
     float ms = 0;
-    VS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (h->ev_recorded) VS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_ms = ms;
     return VS_OK;
 }
@@ -1341,7 +1347,7 @@ int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) 
     if (h->kernel == VS_K_FAST) {
         for (int t = 1024; t >= 128; t -= 64) {
             const int K = fast_cells_per_lane(h->max_padded, t);
-            if (K <= 24 && (long long)(fast_lds_bytes(K, t, fast_guard_lanes(K, h->max_wp)) +
+            if (K <= kFastMaxK && (long long)(fast_lds_bytes(K, t, fast_guard_lanes(K, h->max_wp)) +
                                        fast_lds_static_bytes()) <= h->lds_limit)
                 threads.push_back(t);
         }

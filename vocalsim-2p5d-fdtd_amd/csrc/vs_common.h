@@ -344,9 +344,11 @@ hipError_t launch_audio(const double* d_probe, const AudioJob* d_jobs, int njobs
 // Kernel launchers (vs_kernels.hip, vs_fast.hip, vs_adjoint.hip).
 hipError_t launch_dtft(const double* d_x, long long stride, int nsig, int n, double fs, int hann,
                        const double* d_f, int nf, double* d_re, double* d_im, hipStream_t st);
-// On-chip FP32 kernel sizing: cells per lane K for the batch's largest padded
-// interior Hi * (Wi + 1), zero guard lanes per LDS row for its widest padded
-// row Wi + 1, and the dynamic LDS bytes of a launch.
+// On-chip FP32 kernel sizing: cells per lane K (odd, at most kFastMaxK) for the
+// batch's largest padded interior Hi * (Wi + 1), zero guard lanes on each
+// side of the LDS regions for its widest padded row Wi + 1, and the dynamic
+// LDS bytes of a launch.
+constexpr int kFastMaxK = 23;
 int fast_cells_per_lane(int max_padded_cells, int threads);
 int fast_guard_lanes(int K, int max_padded_width);
 size_t fast_lds_bytes(int K, int threads, int guard);

@@ -5,10 +5,13 @@
 // The tube interior is indexed with one zero pad column per row,
 // j = r * Wp + c (Wp = Wi + 1), and cut into runs of K consecutive cells;
 // thread t owns cells j = t*K + k, k < K.
-//   LDS   one record per lane, stride S = 2K + 1 floats (odd: lanes hit
-//         distinct banks):  [ P[0..K) | Uy[0..K) | Ux of the run's last cell ]
-//         with G zero guard records on each side, so every neighbour read of
-//         a real cell lands on a real cell, a pad cell or a zero guard;
+//   LDS   three flat regions, each with G zero guard lanes on both sides:
+//         P[j] and Uy[j] (cell j at float j: lane t's run at t*K, so with K
+//         odd the 64 lanes of a wave hit distinct banks), and the Ux of each
+//         run's last cell (lane t at float t).  The cells below and above,
+//         j +- Wp, are then a uniform offset from the lane's run, so their
+//         reads pair like the lane's own.  The guards mean every neighbour
+//         read of a real cell lands on a real cell, a pad cell or a zero;
 //         static tables by cell class: aP, and (ax, ay) with the two
 //         face-form flags in their sign bits.
 //   VGPRs Ux[K], kP[K], gX[K], gY[K] and the class bytes (4 per VGPR).
@@ -70,7 +73,7 @@ __device__ inline int sigma_code(int type) {
     }
 }
 
-#ifndef VS_FAST_SPLIT_K   // set by vs_fast_k19.hip, which builds one K of the step kernel only
+#ifndef VS_FAST_SPLIT_K   // set by vs_fast_k17.hip, which builds one K of the step kernel only
 __global__ void k_fast_setup(const TubeDesc* __restrict__ tubes, Bufs<float> b, FastBufs f) {
     const TubeDesc& T = tubes[blockIdx.y];
     const int H = T.H, W = T.W, Wi = W - 2;
@@ -171,13 +174,13 @@ template <int K, bool REC>
 __device__ __forceinline__ void fast_pressure(float* __restrict__ P, const float* __restrict__ UY,
                                               const float* __restrict__ UYB, const float* __restrict__ ptab,
                                               const float (&Ux)[K], const float (&kP)[K], const uint32_t (&cw)[(K + 3) / 4],
-                                              float uxl0, int mW, char* dr, uint32_t vo, uint32_t T4, uint32_t KT4) {
+                                              float uxl0, char* dr, uint32_t vo, uint32_t T4, uint32_t KT4) {
 #pragma clang fp contract(fast)
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const float p = P[k];
         const float uy = UY[k];
-        const float uyb = (k + mW >= K) ? UYB[k + K + 1] : UYB[k];
+        const float uyb = UYB[k];
         const float left = k > 0 ? Ux[k - 1] : uxl0;
         const float dv = (Ux[k] - left) + (uy - uyb);
         const float aP = ptab[class_of<K>(cw, k)];
@@ -193,16 +196,17 @@ __device__ __forceinline__ void fast_pressure(float* __restrict__ P, const float
 
 template <int K>
 __device__ __forceinline__ void fast_faces(const float* __restrict__ P, const float* __restrict__ PA,
-                                           float* __restrict__ UY, const float2* __restrict__ ftab, float (&Ux)[K],
+                                           float* __restrict__ UY, float* __restrict__ UXE,
+                                           const float2* __restrict__ ftab, float (&Ux)[K],
                                            const float (&gX)[K], const float (&gY)[K],
-                                           const uint32_t (&cw)[(K + 3) / 4], float ev, int mW) {
+                                           const uint32_t (&cw)[(K + 3) / 4], float ev) {
 #pragma clang fp contract(fast)
     float pn = P[0];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const float p = pn;
-        pn = (k < K - 1) ? P[k + 1] : P[2 * K + 1];   // next lane's first cell
-        const float pa = (k < mW) ? PA[k - K - 1] : PA[k];
+        pn = P[k + 1];   // P[K]: the next lane's first cell
+        const float pa = PA[k];
         const float2 c2 = ftab[class_of<K>(cw, k)];   // ax, ay with the s, r flags in their signs
         // walls with admittance (s = 0): U = -g P_neighbour; excitation (s = r = 0): Ux = g exeV
         const float ps = flag_off(c2.x) ? 0.f : p;
@@ -211,7 +215,7 @@ __device__ __forceinline__ void fast_faces(const float* __restrict__ P, const fl
         Ux[k] = fabsf(c2.x) * Ux[k] + gX[k] * fx;
         UY[k] = fabsf(c2.y) * UY[k] + gY[k] * fy;
     }
-    UY[K] = Ux[K - 1];
+    *UXE = Ux[K - 1];
 }
 
 // REC: also write the forward factors of every step (the values that
@@ -224,7 +228,6 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                                                     float* __restrict__ frec, long long slab_tube,
                                                     float* __restrict__ fsink) {
 #pragma clang fp contract(fast)
-    constexpr int S = 2 * K + 1;                 // record stride (floats)
     extern __shared__ float lds[];
     __shared__ float2 ftab[kFastClasses];        // (ax, ay) by class, flags in the signs
     __shared__ float ptab[kFastClasses];         // aP by class
@@ -234,24 +237,28 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
     // two-mass recurrence, probe).  Keeping the double-precision recurrence
     // out of the cell waves keeps their register budget for the cell arrays.
     const int T = blockDim.x - 64, t = threadIdx.x;
-    float* const rec0 = lds + G * S;             // record of lane 0; lane l at rec0 + l * S
+    const int NL = T + 2 * G;                    // lanes with the guards
+    float* const Pg = lds + G * K;               // P[j]
+    float* const Ug = lds + (NL + G) * K;        // Uy[j]
+    float* const Xg = lds + 2 * NL * K + G;      // Ux of lane l's last cell at Xg[l]
 
     const int tube = order[blockIdx.x];
     const TubeDesc& D = tubes[tube];
     const int H = D.H, W = D.W, Hi = H - 2, Wi = W - 2;
     const int Wp = Wi + 1;
-    // j + Wp = (l + q) K + (k + m): slot k + m of lane l + q, or slot
-    // k + m - K of lane l + q + 1 when k + m >= K (Wp = q K + m); j - Wp
-    // symmetrically.  In record floats both are a uniform base plus k.
-    const int qW = Wp / K, mW = Wp - (Wp / K) * K;
-    const int nbOff = qW * S + mW;
     // graph replays read the first step from device memory
     const int t0 = d_t0 ? *d_t0 : t0_arg;
     const int tend = min(t0 + nsteps, D.num_steps);
     const long long co = D.cell_off;
 
-    // zero guard records (LDS starts undefined)
-    for (int i = t; i < 2 * G * S; i += blockDim.x) lds[i < G * S ? i : i + T * S] = 0.f;
+    // zero guards (LDS starts undefined): G K floats on both sides of the P
+    // and Uy regions, G on both sides of the run-end region
+    for (int i = t; i < 2 * (2 * G * K + G); i += blockDim.x) {
+        const int r = i / (2 * G * K);             // 0: P, 1: Uy, 2: run ends
+        const int g = i - r * 2 * G * K;
+        if (r < 2) lds[r * NL * K + (g < G * K ? g : g + T * K)] = 0.f;
+        else lds[2 * NL * K + (g < G ? g : g + T)] = 0.f;
+    }
 
     if (t >= T) {
         // ------------------------- control wave -------------------------
@@ -259,8 +266,8 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         const bool twomass = D.source_type == 2;
         const int jL = (D.listener / W - 1) * Wp + (D.listener % W - 1);
         const int jE = (D.excitation / W - 1) * Wp + (D.excitation % W - 1);
-        const float* pL = rec0 + (jL / K) * S + jL % K;
-        const float* pE = rec0 + (jE / K) * S + jE % K;
+        const float* pL = Pg + jL;
+        const float* pE = Pg + jE;
         double pexc = 0.0;
         ptab[lane] = f.ctab[lane];
         ftab[lane] = make_float2(f.ctab[kFastClasses + lane], f.ctab[2 * kFastClasses + lane]);
@@ -303,7 +310,6 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
     uint32_t cw[(K + 3) / 4];
     const int jb = t * K;
     {
-        float* const rec = rec0 + t * S;
         int r = jb / Wp, c = jb - (jb / Wp) * Wp;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -314,19 +320,19 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 p = b.P[q]; uy = f.Uyf[q]; ux = f.Uxf[q];
                 kp = f.kP[q]; gx = f.gX[q]; gy = f.gY[q]; cd = f.cls8[q];
             }
-            rec[k] = p;
-            rec[K + k] = uy;
+            Pg[jb + k] = p;
+            Ug[jb + k] = uy;
             Ux[k] = ux; kP[k] = kp; gX[k] = gx; gY[k] = gy;
             if (k & 3) cw[k >> 2] |= cd << (8 * (k & 3)); else cw[k >> 2] = cd;
             if (++c == Wp) { c = 0; ++r; }
         }
-        rec[2 * K] = Ux[K - 1];
+        Xg[t] = Ux[K - 1];
     }
     __syncthreads();
 
     for (int ts = t0; ts < tend; ++ts) {
         // Launder the lane index once per phase so the compiler recomputes
-        // the record address instead of keeping per-cell copies live across
+        // the run's addresses instead of keeping per-cell copies live across
         // the time loop.
         int tl = t;
         asm volatile("" : "+v"(tl));
@@ -343,10 +349,9 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         uint32_t vo = 4u * (uint32_t)tl;
         // ---------------- pressure phase ----------------
         {
-            float* const rec = rec0 + tl * S;
-            // Uy of the cell below: slot k + m of lane l + q; run end of lane l - 1
-            fast_pressure<K, REC>(rec, rec + K, rec + nbOff + K, ptab, Ux, kP, cw, rec[2 * K - S], mW, dr, vo, T4,
-                                  KT4);
+            const int j = tl * K;
+            // Uy of the cells below (j + Wp); Ux of the previous lane's last cell
+            fast_pressure<K, REC>(Pg + j, Ug + j, Ug + j + Wp, ptab, Ux, kP, cw, Xg[tl - 1], dr, vo, T4, KT4);
         }
         __syncthreads();
         const float ev = (float)sexe[ts & 1];
@@ -355,9 +360,9 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
             asm volatile("" : "+v"(tl));
 #pragma unroll
             for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
-            float* const rec = rec0 + tl * S;
-            // cell above: slot k - m of lane l - q
-            fast_faces<K>(rec, rec - nbOff, rec + K, ftab, Ux, gX, gY, cw, ev, mW);
+            const int j = tl * K;
+            // P of the cells above (j - Wp)
+            fast_faces<K>(Pg + j, Pg + j - Wp, Ug + j, Xg + tl, ftab, Ux, gX, gY, cw, ev);
         }
         __syncthreads();
     }
@@ -367,14 +372,13 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         // stay live across the time loop
         int jw = jb;
         asm volatile("" : "+v"(jw));
-        const float* const rec = rec0 + (jw / K) * S;
         int r = jw / Wp, c = jw - (jw / Wp) * Wp;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (r < Hi && c < Wi) {
                 const long long q = co + (long long)(r + 1) * W + (c + 1);
-                const float uy = rec[K + k];
-                b.P[q] = rec[k];
+                const float uy = Ug[jw + k];
+                b.P[q] = Pg[jw + k];
                 f.Uxf[q] = Ux[k];
                 f.Uyf[q] = uy;
                 b.Vx[q] = Ux[k] / b.dX[q];
@@ -399,7 +403,7 @@ template <int K>
 __device__ __forceinline__ void rev_pressure(float* __restrict__ KP, const float* __restrict__ LY,
                                              const float* __restrict__ LYB, const float* __restrict__ ptab,
                                              const float2* __restrict__ ftab, const float (&LX)[K], const float (&kP)[K],
-                                             const uint32_t (&cw)[(K + 3) / 4], float lx0, int mW, int kL, float gl,
+                                             const uint32_t (&cw)[(K + 3) / 4], float lx0, int kL, float gl,
                                              char* ad, uint32_t vo, uint32_t T4) {
 #pragma clang fp contract(fast)
     float left = lx0;   // r Lx of the cell on the left
@@ -408,7 +412,7 @@ __device__ __forceinline__ void rev_pressure(float* __restrict__ KP, const float
         const uint32_t cb = class_of<K>(cw, k);
         const float2 c2 = ftab[cb];   // ax, ay with the s, r flags in their signs
         const float kold = KP[k], ly = LY[k], lx = LX[k];
-        const float lyb = (k + mW >= K) ? LYB[k + K + 1] : LYB[k];
+        const float lyb = LYB[k];
         float br = (flag_off(c2.x) ? 0.f : lx + ly) - left - lyb;
         if (k == kL) br += gl;
         const float kn = ptab[cb] * kold + kP[k] * br;
@@ -421,21 +425,22 @@ __device__ __forceinline__ void rev_pressure(float* __restrict__ KP, const float
 
 template <int K>
 __device__ __forceinline__ void rev_faces(const float* __restrict__ KP, const float* __restrict__ KA,
-                                          float* __restrict__ LY, const float2* __restrict__ ftab, float (&LX)[K],
+                                          float* __restrict__ LY, float* __restrict__ LXE,
+                                          const float2* __restrict__ ftab, float (&LX)[K],
                                           const float (&gX)[K], const float (&gY)[K],
-                                          const uint32_t (&cw)[(K + 3) / 4], int mW) {
+                                          const uint32_t (&cw)[(K + 3) / 4]) {
 #pragma clang fp contract(fast)
     float kn = KP[0];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const float kc = kn;
-        kn = (k < K - 1) ? KP[k + 1] : KP[2 * K + 1];
-        const float ka = (k < mW) ? KA[k - K - 1] : KA[k];
+        kn = KP[k + 1];   // KP[K]: the next lane's first cell
+        const float ka = KA[k];
         const float2 c2 = ftab[class_of<K>(cw, k)];
         LX[k] = adj_face(fabsf(c2.x), LX[k], gX[k], kn, kc);
         LY[k] = adj_face(fabsf(c2.y), LY[k], gY[k], ka, kc);
     }
-    LY[K] = flag_off(ftab[class_of<K>(cw, K - 1)].y) ? 0.f : LX[K - 1];
+    *LXE = flag_off(ftab[class_of<K>(cw, K - 1)].y) ? 0.f : LX[K - 1];
 }
 
 // ---------------------------------------------------------------------------
@@ -450,8 +455,9 @@ __device__ __forceinline__ void rev_faces(const float* __restrict__ KP, const fl
 //   Kp' = aP Kp + kP (g_n [listener] + s (Lx + Ly) - r_left Lx_left - Ly_below)
 //   Lx' = ax Lx + gX (Kp'_right - Kp')
 //   Ly' = ay Ly + gY (Kp'_top - Kp')
-// i.e. the forward stencil with Kp in the place of P.  LDS records hold
-// [ Kp[0..K) | Ly[0..K) | r Lx of the run's last cell ]; Lx lives in VGPRs.
+// i.e. the forward stencil with Kp in the place of P.  The LDS regions hold
+// Kp[j], Ly[j] and r Lx of each run's last cell, laid out as the forward's
+// P, Uy and run-end Ux; Lx lives in VGPRs.
 // Every step stores Lx, Ly (as they enter the step) and Kp' into a.adj.
 // ---------------------------------------------------------------------------
 template <int K>
@@ -459,20 +465,20 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
                                                        const int* __restrict__ order, FastBufs f, FastAdj a,
                                                        int s0, int ns, int G) {
 #pragma clang fp contract(fast)
-    constexpr int S = 2 * K + 1;
     extern __shared__ float lds[];
     __shared__ float2 ftab[kFastClasses];
     __shared__ float ptab[kFastClasses];
     const int T = blockDim.x, t = threadIdx.x;
     const int KT = K * T;
-    float* const rec0 = lds + G * S;
+    const int NL = T + 2 * G;                    // lanes with the guards
+    float* const Kg = lds + G * K;               // Kp[j]
+    float* const Lg = lds + (NL + G) * K;        // Ly[j]
+    float* const Xg = lds + 2 * NL * K + G;      // r Lx of lane l's last cell at Xg[l]
 
     const int tube = order[blockIdx.x];
     const TubeDesc& D = tubes[tube];
     const int H = D.H, W = D.W, Hi = H - 2, Wi = W - 2;
     const int Wp = Wi + 1;
-    const int qW = Wp / K, mW = Wp - (Wp / K) * K;
-    const int nbOff = qW * S + mW;
     const int tend = min(s0 + ns, D.num_steps);
     const long long co = D.cell_off;
     // the listener's lane and slot
@@ -480,7 +486,12 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
     const int lL = jL / K, kL = jL - (jL / K) * K;
     const double* const gp = a.gprobe + D.src_off;
 
-    for (int i = t; i < 2 * G * S; i += T) lds[i < G * S ? i : i + T * S] = 0.f;
+    for (int i = t; i < 2 * (2 * G * K + G); i += T) {
+        const int r = i / (2 * G * K);             // 0: Kp, 1: Ly, 2: run ends
+        const int g = i - r * 2 * G * K;
+        if (r < 2) lds[r * NL * K + (g < G * K ? g : g + T * K)] = 0.f;
+        else lds[2 * NL * K + (g < G ? g : g + T)] = 0.f;
+    }
     if (t < 64) {
         ptab[t] = f.ctab[t];
         ftab[t] = make_float2(f.ctab[kFastClasses + t], f.ctab[2 * kFastClasses + t]);
@@ -489,7 +500,6 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
     uint32_t cw[(K + 3) / 4];
     float* const st = a.state + (long long)blockIdx.x * 3 * KT + t;
     {
-        float* const rec = rec0 + t * S;
         const int jb = t * K;
         int r = jb / Wp, c = jb - (jb / Wp) * Wp;
 #pragma unroll
@@ -502,15 +512,16 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
             }
             kP[k] = kp; gX[k] = gx; gY[k] = gy;
             if (k & 3) cw[k >> 2] |= cd << (8 * (k & 3)); else cw[k >> 2] = cd;
-            rec[k] = st[k * T];
-            rec[K + k] = st[KT + k * T];
+            const float ly = st[KT + k * T];
+            Kg[jb + k] = st[k * T];
+            Lg[jb + k] = ly;
             LX[k] = st[2 * KT + k * T];
             // Lx, Ly as this block's reverse begins, for the sums
             a.start[(long long)blockIdx.x * 2 * KT + k * T + t] = LX[k];
-            a.start[(long long)blockIdx.x * 2 * KT + KT + k * T + t] = rec[K + k];
+            a.start[(long long)blockIdx.x * 2 * KT + KT + k * T + t] = ly;
             if (++c == Wp) { c = 0; ++r; }
         }
-        rec[2 * K] = f.ctab[4 * kFastClasses + class_of<K>(cw, K - 1)] * LX[K - 1];   // r Lx of the run end
+        Xg[t] = f.ctab[4 * kFastClasses + class_of<K>(cw, K - 1)] * LX[K - 1];   // r Lx of the run end
     }
     __syncthreads();
 
@@ -528,10 +539,9 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
         const float gl = tl == lL ? (float)gp[n] : 0.f;
         // ---------------- pressure-like phase ----------------
         {
-            float* const rec = rec0 + tl * S;
-            // Ly of the cell below; r Lx of the previous lane's last cell
-            rev_pressure<K>(rec, rec + K, rec + nbOff + K, ptab, ftab, LX, kP, cw, rec[2 * K - S], mW, kL, gl, ad, vo,
-                            T4);
+            const int j = tl * K;
+            // Ly of the cells below (j + Wp); r Lx of the previous lane's last cell
+            rev_pressure<K>(Kg + j, Lg + j, Lg + j + Wp, ptab, ftab, LX, kP, cw, Xg[tl - 1], kL, gl, ad, vo, T4);
         }
         __syncthreads();
         // ---------------- face phase ----------------
@@ -539,20 +549,21 @@ __global__ __launch_bounds__(1024) void k_fast_reverse(const TubeDesc* __restric
             asm volatile("" : "+v"(tl));
 #pragma unroll
             for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
-            float* const rec = rec0 + tl * S;
-            rev_faces<K>(rec, rec - nbOff, rec + K, ftab, LX, gX, gY, cw, mW);   // cell above: rec - nbOff
+            const int j = tl * K;
+            // Kp of the cells above (j - Wp)
+            rev_faces<K>(Kg + j, Kg + j - Wp, Lg + j, Xg + tl, ftab, LX, gX, gY, cw);
         }
         __syncthreads();
     }
     {
         int tw = t;
         asm volatile("" : "+v"(tw));
-        const float* const rec = rec0 + tw * S;
+        const int jw = tw * K;
         float* const sw = a.state + (long long)blockIdx.x * 3 * KT + tw;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            sw[k * T] = rec[k];
-            sw[KT + k * T] = rec[K + k];
+            sw[k * T] = Kg[jw + k];
+            sw[KT + k * T] = Lg[jw + k];
             sw[2 * KT + k * T] = LX[k];
         }
     }
@@ -726,12 +737,12 @@ hipError_t launch_rev_k(const TubeDesc* d_tubes, const int* d_order, int n, Fast
 }  // namespace
 
 // K = kFastSplitK: the forward and recording step kernels of that K are
-// built in their own translation unit (vs_fast_k19.hip) under another
-// instruction scheduler.  Under the default one this K alone gets a
-// loop-carried register rotation (about 100 moves per step) and spills;
-// under the other, K = 15..18 and 23..24 would.  Its reverse kernel stays
-// here.
-constexpr int kFastSplitK = 19;
+// built in their own translation unit (vs_fast_k17.hip) under another
+// instruction scheduler.  Under the default one the forward kernel of this K
+// alone spills; the other spills at K = 23.  Its reverse kernel stays here.
+// The kernels exist for odd K up to kFastMaxK (fast_cells_per_lane never
+// picks an even K).
+constexpr int kFastSplitK = 17;
 hipError_t split_configure_step(bool rec);
 hipError_t split_launch_step(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
                              int t0, int nsteps, int threads, int guard, const int* d_t0, hipStream_t st);
@@ -744,9 +755,12 @@ size_t fast_lds_bytes(int K, int threads, int guard) {
     return (size_t)(T + 2 * guard) * (2 * K + 1) * sizeof(float);
 }
 
+// Odd K keeps the lane stride of the P and Uy regions odd, so the 64 lanes
+// of a wave hit distinct LDS banks (an even stride shares banks: K = 24 ran
+// 3.3 times slower than K = 23).
 int fast_cells_per_lane(int max_padded_cells, int threads) {
     const int T = threads - 64;
-    return (max_padded_cells + T - 1) / T;
+    return ((max_padded_cells + T - 1) / T) | 1;
 }
 
 int fast_guard_lanes(int K, int max_padded_width) { return (max_padded_width + K - 1) / K + 1; }
@@ -768,9 +782,8 @@ hipError_t launch_fast_setup(const TubeDesc* d_tubes, const TubeDesc* h_tubes, i
 hipError_t configure_fast(int K) {
     switch (K) {
 #define VS_K(k) case k: return configure_k<k>();
-        VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
-        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(20)
-        VS_K(21) VS_K(22) VS_K(23) VS_K(24)
+        VS_K(1) VS_K(3) VS_K(5) VS_K(7) VS_K(9) VS_K(11) VS_K(13) VS_K(15) VS_K(19) VS_K(21)
+        VS_K(23)
 #undef VS_K
     case kFastSplitK: {
         hipError_t e = split_configure_step(false);
@@ -787,9 +800,8 @@ hipError_t launch_fast_step(int K, const TubeDesc* d_tubes, const int* d_order, 
                             hipStream_t st) {
     switch (K) {
 #define VS_K(k) case k: return launch_k<k>(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard, d_t0, st);
-        VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
-        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(20)
-        VS_K(21) VS_K(22) VS_K(23) VS_K(24)
+        VS_K(1) VS_K(3) VS_K(5) VS_K(7) VS_K(9) VS_K(11) VS_K(13) VS_K(15) VS_K(19) VS_K(21)
+        VS_K(23)
 #undef VS_K
     case kFastSplitK:
         return split_launch_step(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard, d_t0, st);
@@ -802,9 +814,8 @@ hipError_t launch_fast_record(int K, const TubeDesc* d_tubes, const int* d_order
     if (threads - 64 != a.lanes) return hipErrorInvalidValue;
     switch (K) {
 #define VS_K(k) case k: return launch_rec_k<k>(d_tubes, d_order, n, b, f, a, t0, nsteps, threads, guard, st);
-        VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
-        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(20)
-        VS_K(21) VS_K(22) VS_K(23) VS_K(24)
+        VS_K(1) VS_K(3) VS_K(5) VS_K(7) VS_K(9) VS_K(11) VS_K(13) VS_K(15) VS_K(19) VS_K(21)
+        VS_K(23)
 #undef VS_K
     case kFastSplitK:
         return split_launch_record(d_tubes, d_order, n, b, f, a, t0, nsteps, threads, guard, st);
@@ -816,9 +827,8 @@ hipError_t launch_fast_reverse(int K, const TubeDesc* d_tubes, const int* d_orde
                                FastAdj a, int s0, int ns, int guard, hipStream_t st) {
     switch (K) {
 #define VS_K(k) case k: return launch_rev_k<k>(d_tubes, d_order, n, f, a, s0, ns, guard, st);
-        VS_K(1) VS_K(2) VS_K(3) VS_K(4) VS_K(5) VS_K(6) VS_K(7) VS_K(8) VS_K(9) VS_K(10)
-        VS_K(11) VS_K(12) VS_K(13) VS_K(14) VS_K(15) VS_K(16) VS_K(17) VS_K(18) VS_K(19) VS_K(20)
-        VS_K(21) VS_K(22) VS_K(23) VS_K(24)
+        VS_K(1) VS_K(3) VS_K(5) VS_K(7) VS_K(9) VS_K(11) VS_K(13) VS_K(15) VS_K(17) VS_K(19)
+        VS_K(21) VS_K(23)
 #undef VS_K
     default: return hipErrorInvalidValue;
     }
@@ -835,7 +845,7 @@ hipError_t launch_fast_adj_sums(int K, const TubeDesc* d_tubes, const int* d_ord
 
 #else   // VS_FAST_SPLIT_K: this unit provides the split K only
 
-static_assert(VS_FAST_SPLIT_K == kFastSplitK, "vs_fast_k19.hip builds kFastSplitK");
+static_assert(VS_FAST_SPLIT_K == kFastSplitK, "vs_fast_k17.hip builds kFastSplitK");
 
 hipError_t split_configure_step(bool rec) {
     return rec ? configure_step<VS_FAST_SPLIT_K, true>() : configure_step<VS_FAST_SPLIT_K, false>();
