@@ -128,3 +128,69 @@ def test_fast_odd_cells_per_lane(hip_device):
             assert np.array_equal(a, b)
     p64, _ = orc.OracleTube(spec, 64).run()
     assert rel_l2(runs[1024][0], p64) <= 1e-4
+
+
+def _full_batch_gaussian():
+    # the batch workload at full size (256 ragged tubes, 19 845 steps = 30 ms)
+    # with every tube on the Gaussian source, so the probes are linear in its
+    # amplitude
+    base = tubes.batch_tubes(256, seed=0, num_steps=19845)
+    g = base[0]
+    return [sp.replace(source_type=tubes.SOURCE_GAUSSIAN, gauss_amp=g.gauss_amp, gauss_t0=g.gauss_t0,
+                       gauss_width=g.gauss_width) for sp in base]
+
+
+def _batch_probes(specs, precision, kernel, amp):
+    run = [sp.replace(gauss_amp=sp.gauss_amp * amp) for sp in specs]
+    with Solver(run, precision=precision, threads=1024, kernel=kernel, steps_per_launch=256) as s:
+        s.set_option(_abi.OPT_USE_GRAPH, 1)
+        s.run()
+        assert s.steps_done == 19845
+        return [s.probe(i) for i in range(len(run))]
+
+
+def test_full_batch_fp32_determinism_linearity_parity(hip_device):
+    # The FP32 on-chip kernel on the full batch, 256-step launches replayed
+    # as graphs.  A second identical run is bit-identical (no races).  The
+    # probes are linear in the source amplitude only up to FP32 rounding:
+    # where the wave front passes through subnormal magnitudes, products
+    # round differently for amplitudes a and 2a, after which the two runs
+    # carry independent FP32 rounding noise (the same 1e-4..1e-3 as FP32
+    # against FP64; the FP32 oracle behaves the same way).  Two tubes are
+    # checked against the FP64 oracle over the whole run.
+    specs = _full_batch_gaussian()
+    print("[full batch FP32] amplitude x1, twice, and x2", flush=True)
+    a = _batch_probes(specs, 32, _abi.K_FAST, 1.0)
+    a2 = _batch_probes(specs, 32, _abi.K_FAST, 1.0)
+    b = _batch_probes(specs, 32, _abi.K_FAST, 2.0)
+    worst = 0.0
+    for x, y, z in zip(a, a2, b):
+        assert np.array_equal(x, y)
+        assert np.isfinite(x).all() and np.abs(x).max() > 0
+        worst = max(worst, float(np.max(np.abs(z - 2 * x)) / np.max(np.abs(x))))
+    print(f"[full batch FP32] max |p(2a) - 2 p(a)| / max |p(a)| = {worst:.3e}", flush=True)
+    assert worst <= 5e-3
+    for i in (0, 255):
+        p64, _ = orc.OracleTube(specs[i], 64).run()
+        e = rel_l2(a[i], p64)
+        print(f"[full batch FP32] tube {i}: FP32 on chip vs FP64 oracle rel L2 = {e:.2e}", flush=True)
+        assert e <= 1e-3
+
+
+def test_full_batch_fp64_exact_linearity_and_parity(hip_device):
+    # The FP64 exact kernel on the same full batch: no value of these runs
+    # comes near the FP64 subnormal range, so doubling the amplitude doubles
+    # every probe sample exactly, and two tubes equal the FP64 oracle's
+    # FDTDEngineCPU restatement over the whole run (1e-10 relative L2 bar)
+    specs = _full_batch_gaussian()
+    print("[full batch FP64] amplitude x1 and x2", flush=True)
+    a = _batch_probes(specs, 64, _abi.K_EXACT, 1.0)
+    b = _batch_probes(specs, 64, _abi.K_EXACT, 2.0)
+    for x, z in zip(a, b):
+        assert np.abs(x).max() > 0
+        assert np.array_equal(z, 2 * x)
+    for i in (0, 255):
+        p64, _ = orc.OracleTube(specs[i], 64).run()
+        e = rel_l2(a[i], p64)
+        print(f"[full batch FP64] tube {i}: exact kernel vs FP64 oracle rel L2 = {e:.2e}", flush=True)
+        assert e <= 1e-10
