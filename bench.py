@@ -18,6 +18,11 @@ sys.path.insert(0, os.path.join(ROOT, "vocalsim-2p5d-fdtd_amd"))
 BYTES_FP32 = 37   # P, Vx, Vy read+write, dP, dX, dY read (4 B each), 1 B class
 BYTES_FP64 = 73
 HBM_PEAK_GBS = 8000.0
+# LDS of the on-chip FP32 step per cell slot and step, from its compiled code:
+# reads P, Uy, Uy below, aP, P above, P right (4 B each), (ax, ay) (8 B), the
+# old Uy (4 B); writes P', Uy' (4 B each)
+LDS_BYTES_PER_SLOT = 44
+LDS_PEAK_GBS = 256 * 128 * 2.4   # 256 CUs x 128 B per clock x 2.4 GHz
 BLOCK = 256
 GRAPH_BLOCKS = 4   # 256-step launches per captured step graph (the library's default)
 # HBM bytes per 256-step launch of k_fast_step<19> on the 256-tube batch at 1024
@@ -155,6 +160,7 @@ def main():
         log("[bench] autotune: " + ", ".join(f"{c['threads']}/{c['steps_per_launch']}: {c['us_per_step']:.2f} us"
                                             for c in rep.get("candidates", [])) + f" -> threads {threads}")
     cells = s.cells_per_step
+    dims = s.dims
     log(f"[bench] interior cells per step: {cells}, threads {threads}")
     # 256-step blocks replayed as hipGraphs of GRAPH_BLOCKS launches (the
     # remainder, if any, as ordinary launches); the warm-up captures the graph
@@ -179,6 +185,13 @@ def main():
     s.close()
     alg_bytes = cells * BLOCK * BYTES_FP32
     achieved = alg_bytes / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
+    # LDS: every work-group runs K slots on each of its threads - 64 cell lanes
+    # (K odd, from the batch's largest padded interior), pads included
+    max_padded = max((d["frameH"] - 2) * (d["frameW"] - 1) for d in dims)
+    lanes = threads - 64
+    k_slots = (-(-max_padded // lanes)) | 1
+    lds_bytes = args.tubes * k_slots * lanes * LDS_BYTES_PER_SLOT * BLOCK
+    lds_achieved = lds_bytes / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
 
     # ---------------- plain FP64 single tube ----------------
     spec64 = tubes.vowel_tube(0, num_steps=33075)
@@ -227,7 +240,7 @@ def main():
         adjoint = adjoint_leg(args, np, torch, vocalsim, tubes, _abi)
 
     # ---------------- CPU baseline (oracle, one core) ----------------
-    cpu = cpu_structured = None
+    cpu = cpu_structured = cpu_threads = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
@@ -246,6 +259,21 @@ def main():
                "sample": f"FP64 oracle (flat arrays, -O2, 1 thread), 44-section vowel tube "
                          f"{t.H}x{t.W}: {n_cpu} steps in {runs} runs of up to {n_full} steps, {dt:.1f} s"}
         log(f"[bench] CPU oracle: {cpu['value']:.3e} cell-updates/s over {n_cpu} steps")
+        # the same FP64 engine with each phase's rows over the host's cores
+        # (bit-identical), on the CPU share of the box
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        tt = orc.OracleTube(spec64, 64)
+        n_t, dt_t, runs_t = 0, 0.0, 0
+        while dt_t < args.cpu_seconds / 2 and runs_t < 20:
+            tt.reset()
+            _, secs = tt.run_threads(n_full, nthr)
+            dt_t += secs
+            n_t += n_full
+            runs_t += 1
+        cpu_threads = {"value": c64 * n_t / dt_t, "unit": "grid-cell-updates/s", "cores": nthr, "kind": "port",
+                       "sample": f"FP64 oracle (flat arrays, -O2), each phase's rows over {nthr} OpenMP threads, "
+                                 f"same tube: {n_t} steps in {runs_t} runs, {dt_t:.1f} s of step loop"}
+        log(f"[bench] CPU oracle, {nthr} threads: {cpu_threads['value']:.3e} cell-updates/s over {n_t} steps")
         # the reference engine's own layout and precision: vector<vector<cell>>
         # state, a deep copy per step and the inert wall sweep, FP32, timed
         # over the step loop only (FDTDEngineCPU.cpp:254-292)
@@ -291,9 +319,15 @@ def main():
                      "traffic": PMC_TRAFFIC_BYTES if (kernel == _abi.K_FAST and threads == PMC_THREADS
                                                       and args.tubes == 256) else None,
                      "kernel": "k_fast_step<K> (batched FP32)", "mean_launch_ms": mean_launch_ms,
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "lds": {"achieved": lds_achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                             "frac": lds_achieved / LDS_PEAK_GBS, "bytes_per_launch": lds_bytes,
+                             "slots_per_lane": k_slots,
+                             "basis": f"{LDS_BYTES_PER_SLOT} B per cell slot and step over "
+                                      f"{args.tubes} x {lanes} lanes x K slots (pads included)"}},
         "cpu_baseline": cpu,
         "cpu_reference_structured": cpu_structured,
+        "cpu_oracle_threads": cpu_threads,
         "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)

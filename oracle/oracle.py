@@ -72,6 +72,7 @@ def lib():
         L.vso_get_state.argtypes = [ctypes.c_void_p, _D, _D, _D]
         L.vso_set_state.argtypes = [ctypes.c_void_p, _D, _D, _D]
         L.vso_run_structured.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D]
+        L.vso_run_threads.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _D]
         L.vso_float_num_steps.restype = ctypes.c_int
         L.vso_float_num_steps.argtypes = [ctypes.c_float, ctypes.c_int]
         L.vso_twomass_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -167,6 +168,15 @@ class OracleTube:
         Vx = np.ascontiguousarray(Vx, dtype=np.float64).ravel()
         Vy = np.ascontiguousarray(Vy, dtype=np.float64).ravel()
         lib().vso_set_state(self.h, _dptr(P), _dptr(Vx), _dptr(Vy))
+
+    def run_threads(self, n_steps: int, threads: int, first_step: int = 0):
+        """run() with each phase's rows over `threads` OpenMP threads (same
+        results); returns (probe, seconds of the step loop)."""
+        probe = np.zeros(first_step + n_steps)
+        secs = ctypes.c_double(0.0)
+        if lib().vso_run_threads(self.h, first_step, n_steps, _dptr(probe), threads, ctypes.byref(secs)) != 0:
+            raise OracleError("vso_run_threads: bad arguments")
+        return probe[first_step:], secs.value
 
     def run_structured(self, n_steps: int):
         probe = np.zeros(n_steps)

@@ -672,6 +672,7 @@ struct TubeBase {
     virtual void get_state(double*, double*, double*) const = 0;
     virtual void set_state(const double*, const double*, const double*) = 0;
     virtual double run_structured(int n, double* probe) = 0;
+    virtual double run_threads(int first, int n, double* probe, int nthreads) = 0;
 };
 
 template <typename Real>
@@ -725,6 +726,7 @@ struct Tube : TubeBase {
         Pn = P; Vxn = Vx; Vyn = Vy;
     }
     double run_structured(int n, double* probe) override;
+    double run_threads(int first, int n, double* probe, int nthreads) override;
 };
 
 template <typename Real>
@@ -1075,6 +1077,77 @@ void Tube<Real>::run(int first, int n, double* probe, double* src) {
     }
 }
 
+/* run() with each phase's rows split over nthreads OpenMP threads, for CPU
+ * timing on several host cores.  Every cell of a phase reads only the arrays
+ * the previous phase wrote, so the results equal run() bit for bit.  Returns
+ * the wall seconds of the step loop. */
+template <typename Real>
+double Tube<Real>::run_threads(int first, int n, double* probe, int nthreads) {
+    const int H = g.frameH, W = g.frameW, Wi = W - 2;
+    const Real K1 = ph.K1;
+    const size_t L = (size_t)g.listenerY * W + g.listenerX;
+    const size_t E = (size_t)g.excY * W + g.excX;
+    Real exeV = 0;
+    auto t0 = std::chrono::high_resolution_clock::now();
+#pragma omp parallel num_threads(nthreads)
+    for (int s = 0; s < n; ++s) {
+        const int t = first + s;
+#pragma omp single
+        {
+            if (source_type == 2) {
+                tm.p1 = P[E];
+                tm.run(srate_mul);
+                exeV = tm.ug_next;
+            } else {
+                exeV = (t < (int)source.size()) ? source[t] : Real(0);
+            }
+        }
+#pragma omp for schedule(static)
+        for (int r = 1; r < H - 1; ++r)
+            for (int c = 1; c < W - 1; ++c) {
+                const size_t i = (size_t)r * W + c;
+                const size_t k = (size_t)(r - 1) * Wi + (c - 1);
+                Real cxvx = Vx[i] * dX[i] - Vx[i - 1] * dX[i - 1];
+                Real cyvy = Vy[i] * dY[i] - Vy[i + W] * dY[i + W];
+                Real num = (P[i] * dP[i]) - (K1 * (cxvx + cyvy));
+                Real den = engine_variant == 0 ? ((1 + coef[8][k]) * dP[i]) : (coef[8][k] * dP[i] + dP[i]);
+                Real v = num / den;
+                if (dirichlet && type[i] == cell_noPressure) v = 0;
+                Pn[i] = v;
+            }
+#pragma omp single nowait
+        if (probe) probe[t] = (double)Pn[L];
+#pragma omp for schedule(static)
+        for (int r = 1; r < H - 1; ++r)
+            for (int c = 1; c < W - 1; ++c) {
+                const size_t i = (size_t)r * W + c;
+                const size_t k = (size_t)(r - 1) * Wi + (c - 1);
+                Real cxp = Pn[i + 1] - Pn[i];
+                Real cyp = Pn[i - W] - Pn[i];
+                Real vx = (coef[2][k] * Vx[i]) - (coef[6][k] * cxp);
+                Real vy = (coef[3][k] * Vy[i]) - (coef[7][k] * cyp);
+                vx = vx + (exeV * coef[11][k] * coef[4][k]);
+                vy = vy + (exeV * coef[12][k] * coef[5][k]);
+                Real bx = (coef[14][k] * Pn[i] * coef[0][k]) - (coef[13][k] * Pn[i + 1] * coef[1][k]);
+                Real by = (coef[16][k] * Pn[i] * coef[0][k]) - (coef[15][k] * Pn[i - W] * coef[1][k]);
+                bx = bx * coef[9][k] * zinv[i];
+                by = by * coef[10][k] * zinv[i];
+                vx += (coef[4][k] * bx);
+                vy += (coef[5][k] * by);
+                vx /= (coef[2][k] + coef[4][k]);
+                vy /= (coef[3][k] + coef[5][k]);
+                Vxn[i] = vx; Vyn[i] = vy;
+            }
+#pragma omp for schedule(static)
+        for (int r = 1; r < H - 1; ++r)
+            for (int c = 1; c < W - 1; ++c) {
+                const size_t i = (size_t)r * W + c;
+                P[i] = Pn[i]; Vx[i] = Vxn[i]; Vy[i] = Vyn[i];
+            }
+    }
+    return std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+}
+
 /* Reference-structured engine for CPU timing: the data structures and sweeps
  * of FDTDEngineCPU.cpp:222-433 (vector<vector<{P,Vx,Vy,type}>>, separate
  * CxVx/CyVy/CxP/CyP/Vb scratch grids, 17 coefficient matrices, the
@@ -1238,6 +1311,13 @@ int vso_run_structured(void* h, int n_steps, double* probe, double* seconds) {
     if (!h || n_steps < 0) return -1;
     double s = static_cast<TubeBase*>(h)->run_structured(n_steps, probe);
     if (seconds) *seconds = s;
+    return 0;
+}
+
+int vso_run_threads(void* h, int first_step, int n_steps, double* probe, int nthreads, double* seconds) {
+    if (!h || n_steps < 0 || first_step < 0 || nthreads < 1) return -1;
+    double sec = static_cast<TubeBase*>(h)->run_threads(first_step, n_steps, probe, nthreads);
+    if (seconds) *seconds = sec;
     return 0;
 }
 

@@ -93,6 +93,11 @@ int   vso_set_state(void* h, const double* P, const double* Vx, const double* Vy
  * step loop to *seconds. */
 int   vso_run_structured(void* h, int n_steps, double* probe, double* seconds);
 
+/* vso_run with each phase's rows split over nthreads OpenMP threads (CPU
+ * timing on several cores; bit-identical to vso_run).  Writes wall seconds of
+ * the step loop to *seconds. */
+int   vso_run_threads(void* h, int first_step, int n_steps, double* probe, int nthreads, double* seconds);
+
 /* Reference float32 time-step count for a duration (SolverParams.cu:59-66). */
 int   vso_float_num_steps(float audio_dur, int srate_mul);
 

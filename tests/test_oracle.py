@@ -309,6 +309,20 @@ def test_two_mass_matches_reference_build():
     assert np.array_equal(ug, ref)
 
 
+def test_threaded_engine_equals_serial():
+    # the multi-core timing engine splits each phase's rows over threads; every
+    # cell reads only the previous phase's arrays, so it equals the serial
+    # oracle bit for bit (two-mass feedback included), also across split runs
+    for spec in (tubes.vowel_tube(0, num_steps=800), tubes.vowel_tube(3, num_steps=800, source=tubes.SOURCE_GAUSSIAN)):
+        ref, _ = orc.OracleTube(spec, 64).run()
+        got, secs = orc.OracleTube(spec, 64).run_threads(800, 4)
+        assert np.array_equal(got, ref) and secs > 0
+        t = orc.OracleTube(spec, 64)
+        a, _ = t.run_threads(500, 3)
+        b, _ = t.run_threads(300, 5, first_step=500)
+        assert np.array_equal(np.concatenate([a, b]), ref)
+
+
 # ---------------------------------------------------------------------- step
 def numpy_step(P, Vx, Vy, ct, X, Y, Dp, z, K1, dt, rho, ds, exeV, dirichlet=True):
     """Independent vectorised restatement of one step on the whole interior."""
