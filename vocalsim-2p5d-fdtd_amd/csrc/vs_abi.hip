@@ -6,6 +6,8 @@
 #include "../../include/vocalsim_abi.h"
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1916,7 +1918,9 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
         (rc = fetch(lower, b.lower, h->total_walls)) || (rc = fetch(minor, b.minor, h->total_secs)) ||
         (rc = fetch(dscale, b.dscale, h->total_secs)))
         return rc;
-    for (int k = 0; k < h->n; ++k) {
+    // tubes are independent: each reads the shared host copies and writes
+    // only its own sections, so they are chained on up to 16 host threads
+    auto chain_tube = [&](int k) {
         const TubeDesc& d = h->desc[k];
         const int H = d.H, W = d.W, HW = H * W;
         const long long base = d.cell_off;
@@ -1995,7 +1999,25 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
             g_scale[so + s] = depth3d ? gsc[s] : 0.0;
             g_mu[so + s] = gz[s] * dz_dmu(h->sec_mu[so + s], rho_c);
         }
+    };
+    const int nthr = std::max(1, std::min({h->n, 16, (int)std::thread::hardware_concurrency()}));
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    auto worker = [&] {
+        try {
+            for (int k; (k = next.fetch_add(1)) < h->n && !failed.load();) chain_tube(k);
+        } catch (...) {
+            failed.store(true);
+        }
+    };
+    if (nthr == 1) {
+        worker();
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthr; ++t) pool.emplace_back(worker);
+        for (std::thread& t : pool) t.join();
     }
+    if (failed.load()) return fail(VS_E_STATE, "gradient chain: host memory allocation failed");
     return VS_OK;
 }
 
