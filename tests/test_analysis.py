@@ -62,3 +62,23 @@ def test_reference_picker_on_a_single_resonance():
     df = fs / (y.size - 1)
     assert abs(out[1, 0] - 1000.0) <= df
     assert out[0, 0] == pytest.approx(80.0, rel=0.35)
+
+
+def test_get_formants_bw_matches_reference_golden_vectors():
+    # outputs of the reference's own get_formants_bw (results/formants_bandwidths.py:5-80),
+    # written by tests/golden/formants/make_formants_golden.py: damped-resonator
+    # signals at 44.1 and 661.5 kHz, an odd length, 1 to 10 formants, float64
+    # and float32 input; the restatement reproduces them exactly
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "formants", "formants_ref.npz"))
+    n = 0
+    for key in g.files:
+        name, kind, *rest = key.split("__")
+        if kind not in ("f64", "f32"):
+            continue
+        x = g[f"{name}__x"]
+        fs = float(g[f"{name}__fs"])
+        got = analysis.get_formants_bw(x.astype(np.float64) if kind == "f64" else x, fs, int(rest[0]))
+        assert np.array_equal(got, g[key]), key
+        n += 1
+    assert n == 10

@@ -35,7 +35,7 @@ def get_formants_bw(audio_source, fs: float, num_formants: int) -> np.ndarray:
     if isinstance(audio_source, str):
         x = np.loadtxt(audio_source)
     else:
-        x = np.asarray(audio_source, dtype=np.float64)
+        x = np.asarray(audio_source)   # the caller's dtype (a float32 signal gets a float32 FFT)
     N = x.size
     P_db = 10 * np.log10(np.abs(np.fft.fft(x)) ** 2)
     df = fs / (N - 1)
