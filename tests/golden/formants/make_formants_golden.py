@@ -22,8 +22,11 @@ Run from the repository root: python tests/golden/formants/make_formants_golden.
 """
 import importlib.util
 import os
+import sys
 
 import numpy as np
+
+sys.dont_write_bytecode = True   # importing the reference must not write a __pycache__ beside it
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = "/root/reference/results/formants_bandwidths.py"
