@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle_vs.so")
+# VSO_ORACLE_LIB names another build of the same source (the mutation tests
+# load copies built with a planted mistake)
+LIB_PATH = os.environ.get("VSO_ORACLE_LIB") or os.path.join(HERE, "liboracle_vs.so")
 
 _F = ctypes.POINTER(ctypes.c_float)
 _D = ctypes.POINTER(ctypes.c_double)

@@ -20,6 +20,17 @@
  */
 #include "vs_oracle.h"
 
+/* Planted mistakes for the mutation tests (tests/test_oracle_walls.py builds
+ * a copy of this file with -DVSO_MUTATION=n into a temporary directory and
+ * checks that the wall pins reject it).  0, the default, plants nothing.
+ *   1: the wall-admittance velocity Vb enters with the wrong sign;
+ *   2: Vb drops its inward term (xor1 P_right N_in, xor3 P_top N_in);
+ *   3: z_inv is taken from whichever cell of the face is a wall, instead of
+ *      the current cell only (FDTDEngineCPU.cpp:397-398). */
+#ifndef VSO_MUTATION
+#define VSO_MUTATION 0
+#endif
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -1060,8 +1071,20 @@ void Tube<Real>::run(int first, int n, double* probe, double* src) {
                 vy = vy + (exeV * coef[12][k] * coef[5][k]);
                 Real bx = (coef[14][k] * Pn[i] * coef[0][k]) - (coef[13][k] * Pn[i + 1] * coef[1][k]);
                 Real by = (coef[16][k] * Pn[i] * coef[0][k]) - (coef[15][k] * Pn[i - W] * coef[1][k]);
+#if VSO_MUTATION == 2
+                bx = (coef[14][k] * Pn[i] * coef[0][k]);
+                by = (coef[16][k] * Pn[i] * coef[0][k]);
+#endif
+#if VSO_MUTATION == 3
+                bx = bx * coef[9][k] * std::max(zinv[i], zinv[i + 1]);
+                by = by * coef[10][k] * std::max(zinv[i], zinv[i - W]);
+#else
                 bx = bx * coef[9][k] * zinv[i];
                 by = by * coef[10][k] * zinv[i];
+#endif
+#if VSO_MUTATION == 1
+                bx = -bx; by = -by;
+#endif
                 vx += (coef[4][k] * bx);
                 vy += (coef[5][k] * by);
                 vx /= (coef[2][k] + coef[4][k]);
