@@ -7,6 +7,7 @@ One bench "step" is one 256-time-step block advanced over the whole batch
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -18,11 +19,19 @@ sys.path.insert(0, os.path.join(ROOT, "vocalsim-2p5d-fdtd_amd"))
 BYTES_FP32 = 37   # P, Vx, Vy read+write, dP, dX, dY read (4 B each), 1 B class
 BYTES_FP64 = 73
 HBM_PEAK_GBS = 8000.0
-# LDS of the on-chip FP32 step per cell slot and step, from its compiled code:
+# LDS of the on-chip FP32 step per cell and step, from its compiled code:
 # reads P, Uy, Uy below, aP, P above, P right (4 B each), (ax, ay) (8 B), the
-# old Uy (4 B); writes P', Uy' (4 B each)
-LDS_BYTES_PER_SLOT = 44
-LDS_PEAK_GBS = 256 * 128 * 2.4   # 256 CUs x 128 B per clock x 2.4 GHz
+# old Uy (4 B); writes P', Uy' (4 B each).  Rated on real cells only; the
+# peak is measured live (vs_measure_lds_bandwidth).
+LDS_BYTES_PER_CELL = 44
+# algorithmic flops per cell-update (pressure 15, Vx 18, Vy 18; counted from
+# FDTDEngineCPU.cpp:320-405) and MI355X's published dense FP32 vector peak
+FLOPS_PER_CELL = 51
+FP32_PEAK_TFLOPS = 157.3
+# HBM bytes the on-chip kernel must move per real cell and launch: loads P,
+# Ux, Uy, kP, gX, gY (4 B each) and the class byte; stores P, Ux, Uy, Vx, Vy
+# (4 B each) and reads dX, dY for the velocity export
+COMPULSORY_BYTES_PER_CELL = 6 * 4 + 1 + 5 * 4 + 2 * 4
 BLOCK = 256
 GRAPH_BLOCKS = 4   # 256-step launches per captured step graph (the library's default)
 # HBM bytes per 256-step launch of k_fast_step<19> on the 256-tube batch at 1024
@@ -56,6 +65,25 @@ def parse():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="counter calibration only: 5 launches of a plain FP32 copy kernel over this many MiB")
     return ap.parse_args()
+
+
+def host_cpu():
+    """Model name and core counts of the host the CPU baselines run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 0
+    return {"model": model, "nproc": usable, "logical_cpus": os.cpu_count() or 0,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def copy_calibration(mib):
@@ -184,14 +212,22 @@ def main():
     _, nl, mean_launch_ms = s.timing()
     s.close()
     alg_bytes = cells * BLOCK * BYTES_FP32
-    achieved = alg_bytes / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
-    # LDS: every work-group runs K slots on each of its threads - 64 cell lanes
-    # (K odd, from the batch's largest padded interior), pads included
+    hbm_alg = alg_bytes / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
+    # the bound the kernel works against: LDS bytes of real cells over the
+    # launch, against the device's LDS read bandwidth measured here
+    lds_bytes = cells * BLOCK * LDS_BYTES_PER_CELL
+    lds_achieved = lds_bytes / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
+    lds_peaks = {}
+    for w in (4, 8, 16):
+        bw = ctypes.c_double(0.0)
+        _abi.check(vocalsim.load().vs_measure_lds_bandwidth(w, ctypes.byref(bw)))
+        lds_peaks[f"b{8 * w}"] = bw.value / 1e9
+    lds_peak = max(lds_peaks.values())
+    log(f"[bench] LDS read bandwidth (GB/s): " + ", ".join(f"{k} {v:.0f}" for k, v in lds_peaks.items()))
+    flops = cells * BLOCK * FLOPS_PER_CELL / (mean_launch_ms * 1e-3) / 1e12 if mean_launch_ms > 0 else 0.0
     max_padded = max((d["frameH"] - 2) * (d["frameW"] - 1) for d in dims)
     lanes = threads - 64
     k_slots = (-(-max_padded // lanes)) | 1
-    lds_bytes = args.tubes * k_slots * lanes * LDS_BYTES_PER_SLOT * BLOCK
-    lds_achieved = lds_bytes / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
 
     # ---------------- plain FP64 single tube ----------------
     spec64 = tubes.vowel_tube(0, num_steps=33075)
@@ -256,6 +292,7 @@ def main():
             n_cpu += n
             runs += 1
         cpu = {"value": c64 * n_cpu / dt, "unit": "grid-cell-updates/s", "cores": 1, "kind": "port",
+               "host": host_cpu(),
                "sample": f"FP64 oracle (flat arrays, -O2, 1 thread), 44-section vowel tube "
                          f"{t.H}x{t.W}: {n_cpu} steps in {runs} runs of up to {n_full} steps, {dt:.1f} s"}
         log(f"[bench] CPU oracle: {cpu['value']:.3e} cell-updates/s over {n_cpu} steps")
@@ -300,6 +337,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps,
+        "step_unit": f"one {BLOCK}-time-step block over the whole batch",
+        "ms_per_time_step": wall * 1e3 / (args.steps * BLOCK),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -314,17 +353,23 @@ def main():
                        "workload": "44-section vowel tube, two-mass 800 Pa, FP64, one tube"},
         "ratio_batched_fp32_over_plain_fp64": value / fp64_rate,
         "precision_fp32_vs_fp64": precision,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
+        "roofline": {"bound": "lds", "achieved": lds_achieved, "peak": lds_peak, "unit": "GB/s",
+                     "frac": lds_achieved / lds_peak if lds_peak > 0 else None,
                      "traffic": PMC_TRAFFIC_BYTES if (kernel == _abi.K_FAST and threads == PMC_THREADS
                                                       and args.tubes == 256) else None,
                      "kernel": "k_fast_step<K> (batched FP32)", "mean_launch_ms": mean_launch_ms,
-                     "algorithmic_bytes_per_launch": alg_bytes,
-                     "lds": {"achieved": lds_achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
-                             "frac": lds_achieved / LDS_PEAK_GBS, "bytes_per_launch": lds_bytes,
-                             "slots_per_lane": k_slots,
-                             "basis": f"{LDS_BYTES_PER_SLOT} B per cell slot and step over "
-                                      f"{args.tubes} x {lanes} lanes x K slots (pads included)"}},
+                     "basis": f"{LDS_BYTES_PER_CELL} B of LDS traffic per real cell and step x {cells} cells x "
+                              f"{BLOCK} steps per launch; peak = LDS read bandwidth measured on this device "
+                              f"(vs_measure_lds_bandwidth, widest read)",
+                     "lds_peak_by_width_gbs": lds_peaks, "slots_per_lane": k_slots,
+                     "valu": {"achieved_tflops": flops, "peak_tflops": FP32_PEAK_TFLOPS,
+                              "frac": flops / FP32_PEAK_TFLOPS,
+                              "basis": f"{FLOPS_PER_CELL} algorithmic flops per cell-update"},
+                     "hbm": {"algorithmic_gbs_if_streamed": hbm_alg, "peak_gbs": HBM_PEAK_GBS,
+                             "compulsory_bytes_per_launch": cells * COMPULSORY_BYTES_PER_CELL,
+                             "measured_bytes_per_launch": PMC_TRAFFIC_BYTES,
+                             "note": "the state stays on chip for the whole launch: HBM carries only the "
+                                     "launch's load and write-back"}},
         "cpu_baseline": cpu,
         "cpu_reference_structured": cpu_structured,
         "cpu_oracle_threads": cpu_threads,

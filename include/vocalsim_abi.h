@@ -233,6 +233,13 @@ int vs_copy_to_host(void* dst, const void* src, size_t bytes);
  * sum (frameH-2)(frameW-2)  (FDTDEngineCPU.cpp:316-317). */
 int vs_cells_per_step(const vs_handle* h, long long* cells);
 
+/* LDS read bandwidth of the current device in bytes per second (no reference
+ * counterpart: the peak against which the on-chip step kernel's LDS traffic
+ * is rated).  A streaming kernel, 8 work-groups of 1024 threads per compute
+ * unit, each lane reading lane-consecutive vectors of width_bytes (4, 8 or
+ * 16) from a 32 KiB region; best of three timed launches after a warm-up. */
+int vs_measure_lds_bandwidth(int width_bytes, double* bytes_per_s);
+
 /* Audio export.  Replaces results/generate_audio.py:9-42: the probe of a
  * tube (tube >= 0) or of every tube concatenated in tube order (tube = -1),
  * over the steps advanced so far, normalised by its max |p|, clipped to

@@ -9,6 +9,11 @@ for p in (os.path.join(ROOT, "vocalsim-2p5d-fdtd_amd"), os.path.join(ROOT, "orac
         sys.path.insert(0, p)
 
 
+# the reference's area-function files (data/tube-geometry/*.txt), kept as
+# fixtures beside the tests
+AREA_FILES = os.path.join(ROOT, "tests", "golden", "tube-geometry")
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 

@@ -1,0 +1,259 @@
+"""GPU parity at the workloads' full horizons, through the C ABI.
+
+* Uniform 17.5 cm tube, FP64, 13 230 steps (20 ms), Dirichlet and open
+  mouth: probe, source trace and final state within 1e-10 relative L2 of the
+  FP64 oracle (the FDTDEngineCPU restatement, FDTDEngineCPU.cpp:257-288).
+* 44-section vowel tube with the two-mass source at 800 Pa, FP64, 33 075 steps
+  (50 ms): the run passes through fold closures and reopenings
+  (TwoMassModel.cpp:90-231, the closed-fold branch included); probe and
+  injected flow within 1e-10 of the oracle.  The FP32 on-chip kernel on the
+  same tube: probe within 1e-3 relative L2 of the FP64 oracle and F1-F4
+  within 1 Hz.
+* The 256-tube ragged batch at its full size (19 845 steps, graph-replayed
+  256-step blocks): FP32 on chip, the two-mass tubes 128 and 255 against the
+  FP64 oracle (relative L2 <= 1e-3), F1-F4 of four tubes within 1 Hz of the
+  FP64 oracle's; FP64 exact kernel, tubes 128 and 255 within 1e-10.
+* The gradient workload: 64 FP32 tubes x 13 230 steps, 512-step checkpoints,
+  log-spectrum loss; the first 4 tubes' gradients against the FP64 adjoint
+  (relative L2 per kind) and the FP64 adjoint against central finite
+  differences (h = 1e-4 relative, bar 1e-4).
+* Sine source (source_type 0, GlottalExcitations.cu:16-26): FP64 within 1e-10
+  of the oracle, FP32 exact kernel within 1e-6 of the FP32 oracle, FP32
+  on-chip kernel within 1e-4 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from vocalsim import Solver, _abi, analysis, tubes
+
+pytestmark = pytest.mark.gpu
+
+FS = 661500.0
+REL_FP64 = 1e-10
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+# ------------------------------------------------------------- uniform tube
+@pytest.mark.parametrize("mouth_dirichlet", [1, 0])
+def test_uniform_tube_full_20ms_fp64(hip_device, mouth_dirichlet):
+    spec = tubes.uniform_tube(num_steps=13230).replace(mouth_dirichlet=mouth_dirichlet)
+    print(f"[full horizon] uniform tube FP64 13230 steps dirichlet={mouth_dirichlet}", flush=True)
+    ref = orc.OracleTube(spec, 64)
+    p_ref, s_ref = ref.run()
+    with Solver(spec, precision=64) as s:
+        s.run()
+        assert s.steps_done == 13230
+        p, src = s.probe(), s.source_trace()
+        P, Vx, Vy = s.field("pressure"), s.field("vx"), s.field("vy")
+    rP, rVx, rVy = ref.state()
+    print(f"  probe rel L2 {rel_l2(p, p_ref):.3e} max|diff| {np.max(np.abs(p - p_ref)):.3e}; "
+          f"peak at step {int(np.argmax(np.abs(p_ref)))}", flush=True)
+    assert np.abs(p_ref).max() > 0
+    assert rel_l2(p, p_ref) <= REL_FP64 and rel_l2(src, s_ref) <= REL_FP64
+    assert rel_l2(P, rP) <= REL_FP64 and rel_l2(Vx, rVx) <= REL_FP64 and rel_l2(Vy, rVy) <= REL_FP64
+
+
+# ------------------------------------------------------- vowel tube, two-mass
+@pytest.fixture(scope="module")
+def vowel_two_mass_oracle():
+    spec = tubes.vowel_tube(0, num_steps=33075)
+    p, s = orc.OracleTube(spec, 64).run()
+    return spec, p, s
+
+
+def _closures(flow):
+    z = np.nonzero(flow <= 0.0)[0]
+    if z.size == 0:
+        return []
+    starts = [int(z[0])] + [int(v) for v in z[1:][np.diff(z) > 1]]
+    return starts
+
+
+def test_vowel_two_mass_full_50ms_fp64(hip_device, vowel_two_mass_oracle):
+    spec, p_ref, s_ref = vowel_two_mass_oracle
+    closes = _closures(s_ref)
+    print(f"[full horizon] vowel two-mass FP64 33075 steps; fold closures start at {closes}", flush=True)
+    # the folds close (zero flow) and open again, at least twice
+    assert len(closes) >= 2 and closes[0] < 10000
+    assert s_ref[closes[-1]:].max() > 0
+    with Solver(spec, precision=64) as s:
+        s.run()
+        assert s.steps_done == 33075
+        p, src = s.probe(), s.source_trace()
+    print(f"  probe rel L2 {rel_l2(p, p_ref):.3e} max|diff| {np.max(np.abs(p - p_ref)):.3e}; "
+          f"flow max|diff| {np.max(np.abs(src - s_ref)):.3e}", flush=True)
+    assert rel_l2(p, p_ref) <= REL_FP64
+    assert rel_l2(src, s_ref) <= REL_FP64
+    assert _closures(src) == closes
+
+
+def test_vowel_two_mass_full_50ms_fp32_on_chip(hip_device, vowel_two_mass_oracle):
+    spec, p_ref, s_ref = vowel_two_mass_oracle
+    print("[full horizon] vowel two-mass FP32 on chip, 33075 steps", flush=True)
+    with Solver(spec, precision=32, threads=1024, kernel=_abi.K_FAST) as s:
+        s.run()
+        p, src = s.probe(), s.source_trace()
+    e = rel_l2(p, p_ref)
+    F64, _ = analysis.formants(p_ref, s_ref, FS)
+    F32, _ = analysis.formants(p, src, FS)
+    print(f"  probe rel L2 vs FP64 oracle {e:.3e}; F1-4 {np.round(F64, 2)} dF {np.round(F32 - F64, 4)}", flush=True)
+    assert e <= 1e-3
+    assert np.all(np.isfinite(F64[:4])) and np.max(np.abs(F32 - F64)) <= 1.0
+
+
+# ------------------------------------------------------ the batch at full size
+@pytest.fixture(scope="module")
+def full_batch(hip_device):
+    specs = tubes.batch_tubes(256, seed=0, num_steps=19845)
+    print("[full batch] 256 ragged tubes x 19845 steps, half two-mass: FP32 on chip (graphs) and FP64", flush=True)
+    out = {}
+    with Solver(specs, precision=32, threads=1024, kernel=_abi.K_FAST, steps_per_launch=256) as s:
+        s.set_option(_abi.OPT_USE_GRAPH, 1)
+        s.run()
+        assert s.steps_done == 19845
+        out[32] = [(s.probe(i), s.source_trace(i)) for i in range(256)]
+    with Solver(specs, precision=64, threads=1024, steps_per_launch=256) as s:
+        s.run()
+        assert s.steps_done == 19845
+        out[64] = [(s.probe(i), s.source_trace(i)) for i in (0, 127, 128, 255)]
+    return specs, out
+
+
+@pytest.fixture(scope="module")
+def batch_oracle(full_batch):
+    specs, _ = full_batch
+    return {i: orc.OracleTube(specs[i], 64).run() for i in (0, 127, 128, 255)}
+
+
+def test_full_batch_two_mass_half_fp32_vs_fp64_oracle(full_batch, batch_oracle):
+    specs, out = full_batch
+    for i in (128, 255):
+        assert specs[i].source_type == tubes.SOURCE_TWO_MASS
+        p, src = out[32][i]
+        p_ref, s_ref = batch_oracle[i]
+        e = rel_l2(p, p_ref)
+        print(f"  tube {i} (two-mass) FP32 on chip vs FP64 oracle: probe rel L2 {e:.3e}, "
+              f"flow rel L2 {rel_l2(src, s_ref):.3e}; oracle closures {_closures(s_ref)[:4]}", flush=True)
+        assert np.isfinite(p).all() and e <= 1e-3
+
+
+def test_full_batch_formants_within_1hz(full_batch, batch_oracle):
+    _, out = full_batch
+    for i in (0, 127, 128, 255):
+        p, src = out[32][i]
+        p_ref, s_ref = batch_oracle[i]
+        F64, _ = analysis.formants(p_ref, s_ref, FS)
+        F32, _ = analysis.formants(p, src, FS)
+        print(f"  tube {i}: F1-4 {np.round(F64, 2)} dF {np.round(F32 - F64, 4)}", flush=True)
+        assert np.all(np.isfinite(F64)) and np.max(np.abs(F32 - F64)) <= 1.0
+
+
+def test_full_batch_fp64_two_mass_tubes_match_oracle(full_batch, batch_oracle):
+    _, out = full_batch
+    for k, i in enumerate((0, 127, 128, 255)):
+        p, src = out[64][k]
+        p_ref, s_ref = batch_oracle[i]
+        print(f"  tube {i} FP64 exact vs oracle: probe rel L2 {rel_l2(p, p_ref):.3e}", flush=True)
+        assert rel_l2(p, p_ref) <= REL_FP64 and rel_l2(src, s_ref) <= REL_FP64
+
+
+# ------------------------------------------------------- gradient workload
+def test_gradient_workload_full_horizon(hip_device):
+    import torch
+    from vocalsim.adjoint import dft_basis, log_magnitude, tube_probe
+
+    N, B = 13230, 64
+    H_REL = 1e-4
+
+    def spec_of(seed):
+        sp = tubes.vowel_tube(seed, num_steps=N, source=tubes.SOURCE_GAUSSIAN)
+        return sp.replace(depth_scale=np.ones(sp.num_sections, np.float32),
+                          mu3d=np.full(sp.num_sections, 0.005, np.float32))
+
+    specs = [spec_of(k) for k in range(B)]
+    basis = dft_basis(N, FS, np.linspace(0.0, 5000.0, 1024))
+    with Solver([spec_of(64 + k) for k in range(B)], precision=64, threads=1024) as s:
+        s.run()
+        target = log_magnitude(torch.tensor(np.stack([s.probe(i) for i in range(B)]), device="cuda"), basis)
+
+    def per_tube_loss(probe, tgt):
+        # sum over tubes of each tube's mean over bins: a tube's gradient
+        # depends on that tube alone
+        return ((log_magnitude(probe, basis) - tgt) ** 2).mean(dim=1).sum()
+
+    def grads(sp, tgt, precision):
+        A = torch.tensor(np.stack([s.areas_cm2 for s in sp]), dtype=torch.float64, device="cuda").requires_grad_(True)
+        D = torch.tensor(np.stack([s.depth_scale for s in sp]), dtype=torch.float64, device="cuda").requires_grad_(True)
+        M = torch.tensor(np.stack([s.mu3d for s in sp]), dtype=torch.float64, device="cuda").requires_grad_(True)
+        per_tube_loss(tube_probe(A, D, M, sp, precision, 512), tgt).backward()
+        return {"area": A.grad.cpu().numpy(), "scale": D.grad.cpu().numpy(), "mu": M.grad.cpu().numpy()}
+
+    print("[gradient workload] 64 FP32 tubes x 13230 steps, 512-step checkpoints", flush=True)
+    g32 = grads(specs, target, 32)
+    for k in g32:
+        assert np.isfinite(g32[k]).all()
+    print("[gradient workload] FP64 adjoint of tubes 0-3", flush=True)
+    g64 = grads(specs[:4], target[:4], 64)
+    for k in ("area", "scale", "mu"):
+        r = rel_l2(g32[k][:4], g64[k])
+        print(f"  FP32 (64-tube batch) vs FP64 adjoint, {k}: rel L2 {r:.3e}", flush=True)
+        assert r <= 2e-3, k
+
+    def L(sp_list):
+        with Solver(sp_list, precision=64, threads=1024) as s:
+            s.run()
+            p = torch.tensor(np.stack([s.probe(i) for i in range(len(sp_list))]), device="cuda")
+        return float(per_tube_loss(p, target[:4]))
+
+    field = {"area": "areas_cm2", "scale": "depth_scale", "mu": "mu3d"}
+    for kind, tube, sec in (("area", 1, 10), ("scale", 2, 30), ("mu", 3, 5)):
+        vals = []
+        for sign in (+1, -1):
+            arr = np.array(getattr(specs[tube], field[kind]), dtype=np.float32)
+            arr[sec] = np.float32(float(arr[sec]) * (1.0 + sign * H_REL))
+            vals.append((specs[:tube] + [specs[tube].replace(**{field[kind]: arr})] + specs[tube + 1:4], float(arr[sec])))
+        fd = (L(vals[0][0]) - L(vals[1][0])) / (vals[0][1] - vals[1][1])
+        g = float(g64[kind][tube, sec])
+        print(f"  FD check {kind}[{tube},{sec}]: FP64 adjoint {g:.10e} FD {fd:.10e} "
+              f"rel {abs(g - fd) / abs(fd):.2e}; FP32 {g32[kind][tube, sec]:.6e}", flush=True)
+        assert abs(g - fd) <= 1e-4 * abs(fd), (kind, g, fd)
+
+
+# ----------------------------------------------------------- sine source
+SINE_CASES = {
+    "uniform": tubes.uniform_tube(num_steps=3000).replace(source_type=tubes.SOURCE_SINE),
+    "vowel2": tubes.vowel_tube(2, num_steps=3000, source=tubes.SOURCE_SINE),
+}
+
+
+@pytest.mark.parametrize("name", list(SINE_CASES))
+def test_sine_source_parity(hip_device, name):
+    spec = SINE_CASES[name]
+    print(f"[sine source] {name}", flush=True)
+    p64, s64 = orc.OracleTube(spec, 64).run()
+    p32, _ = orc.OracleTube(spec, 32).run()
+    with Solver(spec, precision=64) as s:
+        s.run()
+        assert np.array_equal(s.field("source"), orc.OracleTube(spec, 64).field("source"))
+        e = rel_l2(s.probe(), p64)
+        print(f"  FP64 probe rel L2 {e:.3e}", flush=True)
+        assert e <= REL_FP64 and rel_l2(s.source_trace(), s64) <= REL_FP64
+    with Solver(spec, precision=32) as s:
+        s.run()
+        assert np.array_equal(s.field("source").astype(np.float64), orc.OracleTube(spec, 32).field("source"))
+        e = rel_l2(s.probe(), p32)
+        print(f"  FP32 exact kernel vs FP32 oracle rel L2 {e:.3e}", flush=True)
+        assert e <= 1e-6
+    with Solver(spec, precision=32, threads=1024, kernel=_abi.K_FAST) as s:
+        s.run()
+        e = rel_l2(s.probe(), p64)
+        print(f"  FP32 on chip vs FP64 oracle rel L2 {e:.3e}", flush=True)
+        assert e <= 1e-4
+    assert np.abs(p64).max() > 0

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
+from conftest import AREA_FILES
 from vocalsim import Solver, tubes
 
 pytestmark = pytest.mark.gpu
@@ -37,9 +38,9 @@ def _cases():
         "vowel3_gauss": tubes.vowel_tube(3, num_steps=600, source=tubes.SOURCE_GAUSSIAN),
         "batch_tube7": tubes.batch_tubes(8, seed=1, num_steps=600)[7],
         "batch_tube2": tubes.batch_tubes(8, seed=1, num_steps=600)[2],
-        "vowel_a_file_ellipse": tubes.read_area_file(tubes.reference_data_dir() + "/vocaltract_a.txt").replace(
+        "vowel_a_file_ellipse": tubes.read_area_file(AREA_FILES + "/vocaltract_a.txt").replace(
             shape=tubes.SHAPE_ELLIPTICAL, num_steps=600),
-        "cylinder_file_circle": tubes.read_area_file(tubes.reference_data_dir() + "/cylindrical-tube.txt").replace(
+        "cylinder_file_circle": tubes.read_area_file(AREA_FILES + "/cylindrical-tube.txt").replace(
             num_steps=600),
         "vowel5_plane_symmetric": tubes.vowel_tube(5, num_steps=600, source=tubes.SOURCE_GAUSSIAN,
                                                    n_sections=12).replace(shape=tubes.SHAPE_PLANE_SYMMETRIC,
@@ -47,7 +48,7 @@ def _cases():
         "open_space_pml": tubes.open_space(num_steps=500),
         "open_space_2d_nopml_twomass": tubes.open_space(num_steps=500, pml=0, solver_type=0,
                                                         source_type=tubes.SOURCE_TWO_MASS),
-        "vowel_u_file_plane_symmetric": tubes.read_area_file(tubes.reference_data_dir() + "/vocaltract_u.txt").replace(
+        "vowel_u_file_plane_symmetric": tubes.read_area_file(AREA_FILES + "/vocaltract_u.txt").replace(
             shape=tubes.SHAPE_PLANE_SYMMETRIC, num_steps=400),
     }
     return cases

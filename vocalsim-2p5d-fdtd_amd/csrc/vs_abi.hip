@@ -137,7 +137,7 @@ int prelude_open_space(const vs_tube_spec& s, bool fp64, Prelude& p) {
     return VS_OK;
 }
 
-int prelude(const vs_tube_spec& s, bool fp64, Prelude& p) {
+int prelude(const vs_tube_spec& s, bool fp64, Prelude& p, hipStream_t st) {
     if (s.srate_mul <= 0) return fail(VS_E_INVALID, "srate_mul must be positive");
     if (s.fdtd_solver_type != 0 && s.fdtd_solver_type != 1) return fail(VS_E_INVALID, "fdtd_solver_type must be 0 or 1");
     if (s.source_type < 0 || s.source_type > 2) return fail(VS_E_INVALID, "Invalid source type");
@@ -177,7 +177,7 @@ int prelude(const vs_tube_spec& s, bool fp64, Prelude& p) {
     }
     if (s.cross_sectional_shape == 3) {
         std::string err;
-        const int rc = plane_sections(s.area_cm2, n, p.plane, err);
+        const int rc = plane_sections(s.area_cm2, n, p.plane, err, st);
         if (rc != VS_OK) return fail(rc, err);
         for (int i = 0; i < n; ++i) width[i] = p.plane[i].diameter;
     }
@@ -279,6 +279,7 @@ struct vs_handle {
     std::vector<vs_dims> dims;
     int max_steps = 0;
     int steps_done = 0;
+    int graph_t0 = 0;          // host source of the graph replays' first-step copy
     long long cells_per_step = 0;
     // options
     int steps_per_launch = 256;
@@ -438,7 +439,7 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
     const bool fp64 = sizeof(Real) == 8;
     std::vector<Prelude> pre(h->n);
     for (int k = 0; k < h->n; ++k) {
-        int rc = prelude(specs[k], fp64, pre[k]);
+        int rc = prelude(specs[k], fp64, pre[k], h->stream);
         if (rc != VS_OK) return fail(rc, "tube " + std::to_string(k) + ": " + g_last_error);
     }
     h->desc.resize(h->n);
@@ -836,17 +837,19 @@ int run_steps(vs_handle* h, int n_steps) {
                 const int rc = build_graph<Real>(h, b);
                 if (rc != VS_OK) return rc;
             }
-            VS_HIP(hipMemcpyAsync(h->d_t0, &h->steps_done, sizeof(int), hipMemcpyHostToDevice, h->stream));
+            // the first step of the replays, staged in a host field that nothing
+            // touches until the copy has run (the synchronize below)
+            h->graph_t0 = start;
+            VS_HIP(hipMemcpyAsync(h->d_t0, &h->graph_t0, sizeof(int), hipMemcpyHostToDevice, h->stream));
             VS_HIP(hipEventRecord(h->ev0, h->stream));
             for (int k = 0; k < reps; ++k) VS_HIP(hipGraphLaunch(h->graph_exec, h->stream));
             VS_HIP(hipEventRecord(h->ev1, h->stream));
             h->ev_recorded = true;
             if (h->kernel != VS_K_FAST) h->flux_valid = false;
-            h->steps_done = start + reps * per;
             h->last_launches = reps * h->graph_blocks;
             graph_ran = true;
-            // keep the counter in host memory alive until the copy has run
             VS_HIP(hipStreamSynchronize(h->stream));
+            h->steps_done = start + reps * per;
         }
         if (end - h->steps_done <= 0) return VS_OK;
         n_steps = end - h->steps_done;
@@ -1421,6 +1424,30 @@ int vs_copy_to_host(void* dst, const void* src, size_t bytes) {
 int vs_cells_per_step(const vs_handle* h, long long* cells) {
     if (check_handle(h) || !cells) return fail(VS_E_INVALID, "vs_cells_per_step: bad argument");
     *cells = h->cells_per_step;
+    return VS_OK;
+}
+
+int vs_measure_lds_bandwidth(int width_bytes, double* bytes_per_s) {
+    if (!bytes_per_s || (width_bytes != 4 && width_bytes != 8 && width_bytes != 16))
+        return fail(VS_E_INVALID, "vs_measure_lds_bandwidth: width must be 4, 8 or 16 bytes");
+    int dev = 0, cus = 0;
+    VS_HIP(hipGetDevice(&dev));
+    VS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int groups = 8 * std::max(cus, 1), iters = 512, words = width_bytes / 4;
+    float* d_out = nullptr;
+    VS_HIP(hipMalloc(&d_out, sizeof(float) * (size_t)groups * 1024));
+    hipStream_t st = nullptr;
+    hipError_t e = hipStreamCreate(&st);
+    float best = 0.f;
+    for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {   // first launch warms up
+        float ms = 0.f;
+        e = measure_lds_read(words, groups, iters, d_out, st, &ms);
+        if (e == hipSuccess && rep > 0 && (best == 0.f || ms < best)) best = ms;
+    }
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("lds measurement: ") + hipGetErrorString(e));
+    *bytes_per_s = (double)groups * 1024.0 * iters * 8.0 * width_bytes / (best * 1e-3);
     return VS_OK;
 }
 

@@ -269,7 +269,7 @@ struct PlaneSection {
     float diameter = 0;
     int grow_steps = 0;   // 0.01 mm growth steps taken to reach the area
 };
-int plane_sections(const float* area_cm2, int n, std::vector<PlaneSection>& out, std::string& err);
+int plane_sections(const float* area_cm2, int n, std::vector<PlaneSection>& out, std::string& err, hipStream_t st);
 
 // Per-cell coefficients of the FP32 flux-form kernel (vs_fast.hip).
 constexpr int kFastClasses = 64;   // cell classes of one batch (class bytes 0..63)
@@ -342,6 +342,7 @@ AudioFilter butter2_lowpass(double fc, double fs);
 hipError_t launch_audio(const double* d_probe, const AudioJob* d_jobs, int njobs, float* d_out, hipStream_t st);
 
 // Kernel launchers (vs_kernels.hip, vs_fast.hip, vs_adjoint.hip).
+hipError_t measure_lds_read(int width_words, int groups, int iters, float* d_out, hipStream_t st, float* ms);
 hipError_t launch_dtft(const double* d_x, long long stride, int nsig, int n, double fs, int hann,
                        const double* d_f, int nf, double* d_re, double* d_im, hipStream_t st);
 // On-chip FP32 kernel sizing: cells per lane K (odd, at most kFastMaxK) for the

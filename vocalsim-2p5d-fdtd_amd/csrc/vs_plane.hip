@@ -132,29 +132,46 @@ void joined_half(float a1, float b1, float a2, float b2, float yc1, float yc2, i
 }
 
 // For every point of every section, the farthest point of the same section
-// (first index attaining the strict maximum) and that distance.
-__global__ void k_max_chord(const float* __restrict__ py, const float* __restrict__ pz, int npt, float* dist,
-                            int* far_idx) {
+// (first index attaining the strict maximum) and that distance.  One lane per
+// point; the section's points stream through LDS in tiles of kChordTile, which
+// every lane of the work-group then reads as a broadcast.  The candidates are
+// visited in index order, so the first maximum is kept as in a plain loop.
+constexpr int kChordTile = 256;
+__global__ __launch_bounds__(kChordTile) void k_max_chord(const float* __restrict__ py, const float* __restrict__ pz,
+                                                          int npt, float* __restrict__ dist, int* __restrict__ far_idx) {
+    __shared__ float ty[kChordTile], tz[kChordTile];
     const int sec = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npt) return;
+    const int i = blockIdx.x * kChordTile + threadIdx.x;
     const float* y = py + (size_t)sec * npt;
     const float* z = pz + (size_t)sec * npt;
-    const float yi = y[i], zi = z[i];
+    const bool live = i < npt;
+    const float yi = live ? y[i] : 0.0f, zi = live ? z[i] : 0.0f;
     float best = 0.0f;
     int arg = 0;
-    for (int k = 0; k < npt; ++k) {
-        const float dy = yi - y[k], dz = zi - z[k];
-        const float d = __builtin_sqrtf(dy * dy + dz * dz);
-        if (d > best) { best = d; arg = k; }
+    for (int k0 = 0; k0 < npt; k0 += kChordTile) {
+        const int k = k0 + threadIdx.x;
+        __syncthreads();   // the previous tile has been read
+        if (k < npt) {
+            ty[threadIdx.x] = y[k];
+            tz[threadIdx.x] = z[k];
+        }
+        __syncthreads();
+        const int m = min(kChordTile, npt - k0);
+        for (int j = 0; j < m; ++j) {
+            const float dy = yi - ty[j], dz = zi - tz[j];
+            const float d = __builtin_sqrtf(dy * dy + dz * dz);
+            if (d > best) { best = d; arg = k0 + j; }
+        }
     }
-    dist[(size_t)sec * npt + i] = best;
-    far_idx[(size_t)sec * npt + i] = arg;
+    if (live) {
+        dist[(size_t)sec * npt + i] = best;
+        far_idx[(size_t)sec * npt + i] = arg;
+    }
 }
 
 }  // namespace
 
-int plane_sections(const float* area_cm2, int n, std::vector<PlaneSection>& out, std::string& err) {
+int plane_sections(const float* area_cm2, int n, std::vector<PlaneSection>& out, std::string& err, hipStream_t st) {
     constexpr int kCoords = 1999;               // TubeGeometry.cu:373
     constexpr int nh = (kCoords + 1) / 2;       // points per half shape
     out.assign(n, PlaneSection{});
@@ -201,27 +218,31 @@ int plane_sections(const float* area_cm2, int n, std::vector<PlaneSection>& out,
         all_y.insert(all_y.end(), P.y.begin(), P.y.end());
         all_z.insert(all_z.end(), P.z.begin(), P.z.end());
     }
-    // maximum chord on the device: O(points^2) per section
+    // maximum chord on the device: O(points^2) per section, on the caller's
+    // stream with stream-ordered buffers (no device-wide synchronisation)
     const int npt = kPlanePoints;
-    float *dy = nullptr, *dz = nullptr, *dd = nullptr;
-    int* di = nullptr;
     const size_t np = (size_t)n * npt;
-    hipError_t e = hipMalloc(&dy, np * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&dz, np * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&dd, np * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&di, np * sizeof(int));
+    float* buf = nullptr;   // y, z, distance (float) and far index (int), np each
     std::vector<float> hd(np);
     std::vector<int> hi(np);
-    if (e == hipSuccess) e = hipMemcpy(dy, all_y.data(), np * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dz, all_z.data(), np * sizeof(float), hipMemcpyHostToDevice);
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), 4 * np * sizeof(float), st);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_max_chord, dim3((npt + 255) / 256, n), dim3(256), 0, 0, dy, dz, npt, dd, di);
-        e = hipGetLastError();
+        float *dy = buf, *dz = buf + np, *dd = buf + 2 * np;
+        int* di = reinterpret_cast<int*>(buf + 3 * np);
+        e = hipMemcpyAsync(dy, all_y.data(), np * sizeof(float), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dz, all_z.data(), np * sizeof(float), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_max_chord, dim3((npt + kChordTile - 1) / kChordTile, n), dim3(kChordTile), 0, st,
+                               dy, dz, npt, dd, di);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(hd.data(), dd, np * sizeof(float), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(hi.data(), di, np * sizeof(int), hipMemcpyDeviceToHost, st);
+        const hipError_t ef = hipFreeAsync(buf, st);
+        if (e == hipSuccess) e = ef;
+        const hipError_t es = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = es;
     }
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(hd.data(), dd, np * sizeof(float), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(hi.data(), di, np * sizeof(int), hipMemcpyDeviceToHost);
-    (void)hipFree(dy); (void)hipFree(dz); (void)hipFree(dd); (void)hipFree(di);
     if (e != hipSuccess) { err = std::string("max chord: ") + hipGetErrorString(e); return VS_E_DEVICE; }
     for (int s = 0; s < n; ++s) {
         PlaneSection& P = out[s];

@@ -36,7 +36,7 @@ EXPORTS = [
     "vs_get_source_trace", "vs_set_option", "vs_get_option", "vs_field_ptr", "vs_stream",
     "vs_last_run_timing", "vs_cells_per_step", "vs_dtft", "vs_probe_dtft", "vs_autotune",
     "vs_adjoint_forward", "vs_adjoint_backward", "vs_probe_audio", "vs_butter2_lowpass", "vs_write_wav",
-    "vs_copy_to_host",
+    "vs_copy_to_host", "vs_measure_lds_bandwidth",
 ]
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -115,6 +115,7 @@ def load():
     L.vs_butter2_lowpass.argtypes = [ctypes.c_double, ctypes.c_double, _D, _D]
     L.vs_write_wav.argtypes = [ctypes.c_char_p, _F, c_size, c_int]
     L.vs_copy_to_host.argtypes = [c_void, c_void, c_size]
+    L.vs_measure_lds_bandwidth.argtypes = [c_int, _D]
     for name in EXPORTS:
         if name not in ("vs_last_error", "vs_abi_version"):
             getattr(L, name).restype = c_int

@@ -19,7 +19,6 @@ from __future__ import annotations
 
 import dataclasses
 import math
-import os
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -209,9 +208,3 @@ def batch_tubes(n: int = 256, seed: int = 0, num_steps: int = 19845,
             spec.tm_q = TWO_MASS_Q_120HZ
         out.append(spec)
     return out
-
-
-def reference_data_dir() -> str:
-    """Directory of the area-function fixtures shipped with the tests."""
-    here = os.path.dirname(os.path.abspath(__file__))
-    return os.path.normpath(os.path.join(here, "..", "..", "tests", "golden", "tube-geometry"))
