@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--tubes", type=int, default=256)
     ap.add_argument("--threads", type=int, default=0, help="0: autotuned / default")
     ap.add_argument("--kernel", type=int, default=-1, help="-1: best available")
-    ap.add_argument("--fp64-blocks", type=int, default=8)
+    ap.add_argument("--fp64-blocks", type=int, default=8, help="0: skip the plain FP64 leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
@@ -231,17 +231,19 @@ def main():
 
     # ---------------- plain FP64 single tube ----------------
     spec64 = tubes.vowel_tube(0, num_steps=33075)
-    s64 = vocalsim.Solver(spec64, precision=64, threads=1024, steps_per_launch=BLOCK)
-    c64 = s64.cells_per_step
-    s64.run(BLOCK)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s64.run(args.fp64_blocks * BLOCK)
-    torch.cuda.synchronize()
-    w64 = time.perf_counter() - t0
-    s64.close()
-    fp64_rate = c64 * args.fp64_blocks * BLOCK / w64
-    log(f"[bench] plain FP64 single tube ({c64} cells): {fp64_rate:.3e} cell-updates/s")
+    fp64_rate, c64 = None, None
+    if args.fp64_blocks > 0:
+        s64 = vocalsim.Solver(spec64, precision=64, threads=1024, steps_per_launch=BLOCK)
+        c64 = s64.cells_per_step
+        s64.run(BLOCK)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s64.run(args.fp64_blocks * BLOCK)
+        torch.cuda.synchronize()
+        w64 = time.perf_counter() - t0
+        s64.close()
+        fp64_rate = c64 * args.fp64_blocks * BLOCK / w64
+        log(f"[bench] plain FP64 single tube ({c64} cells): {fp64_rate:.3e} cell-updates/s")
 
     # ---------------- FP32 precision: F1-F4 vs FP64 (17 tubes, 50 ms) ----------------
     precision = None
@@ -281,6 +283,7 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
         t = orc.OracleTube(spec64, 64)
+        c64 = c64 or (t.H - 2) * (t.W - 2)
         n_full = t.num_steps
         n_cpu, dt, runs = 0, 0.0, 0
         while dt < args.cpu_seconds and runs < 20:
@@ -351,7 +354,7 @@ def main():
                    "graph_blocks": GRAPH_BLOCKS, "kernel": kernel, "threads": threads},
         "fp64_plain": {"value": fp64_rate, "cells_per_step": c64,
                        "workload": "44-section vowel tube, two-mass 800 Pa, FP64, one tube"},
-        "ratio_batched_fp32_over_plain_fp64": value / fp64_rate,
+        "ratio_batched_fp32_over_plain_fp64": value / fp64_rate if fp64_rate else None,
         "precision_fp32_vs_fp64": precision,
         "roofline": {"bound": "lds", "achieved": lds_achieved, "peak": lds_peak, "unit": "GB/s",
                      "frac": lds_achieved / lds_peak if lds_peak > 0 else None,
