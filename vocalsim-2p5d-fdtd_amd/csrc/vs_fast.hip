@@ -164,6 +164,25 @@ __device__ inline uint32_t next_off(uint32_t vo, uint32_t step) {
 // Sign-bit flags of the (ax, ay) class entries.
 __device__ __forceinline__ bool flag_off(float e) { return __float_as_int(e) < 0; }
 
+// Walks the padded interior j = r * Wp + c (c < Wp; c == Wp - 1 is the pad
+// column) over the work-group: thread i visits j = i, i + B, ... below n,
+// with its row and column advanced incrementally.  Consecutive threads visit
+// consecutive cells of a row, so the global accesses f makes at
+// (r + 1) * W + c + 1 are coalesced.
+template <typename F>
+__device__ __forceinline__ void for_padded(int n, int Wp, F&& f) {
+    const int B = blockDim.x;
+    const int Bq = B / Wp, Br = B - Bq * Wp;
+    int j = threadIdx.x;
+    int r = j / Wp, c = j - r * Wp;
+    for (; j < n; j += B) {
+        f(j, r, c);
+        c += Br;
+        r += Bq;
+        if (c >= Wp) { c -= Wp; ++r; }
+    }
+}
+
 // The two phases of one step over a lane's run of K cells.  The pointers are
 // __restrict__: in the pressure-like phase only the lane's own P-region is
 // written (Uy of this lane and of the lane below are only read), in the face
@@ -260,6 +279,71 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         else lds[2 * NL * K + (g < G ? g : g + T)] = 0.f;
     }
 
+    // ------------- state and coefficients into the work-group -------------
+    // Each field crosses global memory once, coalesced: the work-group copies
+    // it in padded-index order into an LDS region (the Uy region serves as
+    // the staging area for the per-lane fields), then every cell lane picks
+    // up its run j = t K + k.  Slots past the tube's last row (r >= Hi) and
+    // pad-column slots are zero.
+    const int KT = K * T;
+    const int jb = t * K;
+    const bool cell_lane = t < T;
+    float Ux[K], kP[K], gX[K], gY[K];
+    uint32_t cw[(K + 3) / 4];
+    auto stage = [&](const float* __restrict__ src) {
+        for_padded(KT, Wp, [&](int j, int r, int c) {
+            Ug[j] = (r < Hi && c < Wi) ? src[co + (long long)(r + 1) * W + (c + 1)] : 0.f;
+        });
+        __syncthreads();
+    };
+    stage(f.Uxf);
+    if (cell_lane) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) Ux[k] = Ug[jb + k];
+    }
+    __syncthreads();
+    stage(f.kP);
+    if (cell_lane) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) kP[k] = Ug[jb + k];
+    }
+    __syncthreads();
+    stage(f.gX);
+    if (cell_lane) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) gX[k] = Ug[jb + k];
+    }
+    __syncthreads();
+    stage(f.gY);
+    if (cell_lane) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) gY[k] = Ug[jb + k];
+    }
+    __syncthreads();
+    for_padded(KT, Wp, [&](int j, int r, int c) {
+        const uint32_t cd = (r < Hi && c < Wi) ? (uint32_t)f.cls8[co + (long long)(r + 1) * W + (c + 1)] : 0u;
+        Ug[j] = __uint_as_float(cd);
+    });
+    __syncthreads();
+    if (cell_lane) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t cd = __float_as_uint(Ug[jb + k]);
+            if (k & 3) cw[k >> 2] |= cd << (8 * (k & 3)); else cw[k >> 2] = cd;
+        }
+    }
+    __syncthreads();
+    for_padded(KT, Wp, [&](int j, int r, int c) {
+        float p = 0.f, uy = 0.f;
+        if (r < Hi && c < Wi) {
+            const long long q = co + (long long)(r + 1) * W + (c + 1);
+            p = b.P[q];
+            uy = f.Uyf[q];
+        }
+        Pg[j] = p;
+        Ug[j] = uy;
+    });
+
     if (t >= T) {
         // ------------------------- control wave -------------------------
         const int lane = t - T;
@@ -301,33 +385,9 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
             b.pexc[tube] = pexc;
             if (twomass) b.tm[tube] = tm;
         }
-        return;
-    }
-
+    } else {
     // --------------------------- cell waves ---------------------------
-    const int KT = K * T;
-    float Ux[K], kP[K], gX[K], gY[K];
-    uint32_t cw[(K + 3) / 4];
-    const int jb = t * K;
-    {
-        int r = jb / Wp, c = jb - (jb / Wp) * Wp;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            float p = 0.f, uy = 0.f, ux = 0.f, kp = 0.f, gx = 0.f, gy = 0.f;
-            uint32_t cd = 0u;
-            if (r < Hi && c < Wi) {
-                const long long q = co + (long long)(r + 1) * W + (c + 1);
-                p = b.P[q]; uy = f.Uyf[q]; ux = f.Uxf[q];
-                kp = f.kP[q]; gx = f.gX[q]; gy = f.gY[q]; cd = f.cls8[q];
-            }
-            Pg[jb + k] = p;
-            Ug[jb + k] = uy;
-            Ux[k] = ux; kP[k] = kp; gX[k] = gx; gY[k] = gy;
-            if (k & 3) cw[k >> 2] |= cd << (8 * (k & 3)); else cw[k >> 2] = cd;
-            if (++c == Wp) { c = 0; ++r; }
-        }
-        Xg[t] = Ux[K - 1];
-    }
+    Xg[t] = Ux[K - 1];
     __syncthreads();
 
     for (int ts = t0; ts < tend; ++ts) {
@@ -366,27 +426,34 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         }
         __syncthreads();
     }
+    }   // cell waves
     // ---------------- write back ----------------
-    {
-        // fresh lane coordinates: the addresses of the initial loads must not
-        // stay live across the time loop
-        int jw = jb;
-        asm volatile("" : "+v"(jw));
-        int r = jw / Wp, c = jw - (jw / Wp) * Wp;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (r < Hi && c < Wi) {
-                const long long q = co + (long long)(r + 1) * W + (c + 1);
-                const float uy = Ug[jw + k];
-                b.P[q] = Pg[jw + k];
-                f.Uxf[q] = Ux[k];
-                f.Uyf[q] = uy;
-                b.Vx[q] = Ux[k] / b.dX[q];
-                b.Vy[q] = uy / b.dY[q];
-            }
-            if (++c == Wp) { c = 0; ++r; }
+    // P and Uy leave from their LDS regions, coalesced; then the lanes' Ux
+    // go through the P region the same way.  V = U / depth as before.
+    __syncthreads();
+    for_padded(Hi * Wp, Wp, [&](int j, int r, int c) {
+        if (c < Wi) {
+            const long long q = co + (long long)(r + 1) * W + (c + 1);
+            const float uy = Ug[j];
+            b.P[q] = Pg[j];
+            f.Uyf[q] = uy;
+            b.Vy[q] = uy / b.dY[q];
         }
+    });
+    __syncthreads();
+    if (cell_lane) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) Pg[jb + k] = Ux[k];
     }
+    __syncthreads();
+    for_padded(Hi * Wp, Wp, [&](int j, int r, int c) {
+        if (c < Wi) {
+            const long long q = co + (long long)(r + 1) * W + (c + 1);
+            const float ux = Pg[j];
+            f.Uxf[q] = ux;
+            b.Vx[q] = ux / b.dX[q];
+        }
+    });
 }
 
 // Face update of the scaled adjoint, a L + g (K_neighbour - K_cell), with its

@@ -5,7 +5,8 @@
 // Every work-group (1024 threads) fills a 32 KiB LDS region, then each lane
 // reads 8 independent W-word vectors per iteration at lane-consecutive
 // addresses (conflict-free: lane l of a read touches words [l W, l W + W)),
-// stepping through the region; the words are summed so no read is dead.
+// all 8 at distinct addresses, stepping through the region; the words are
+// summed so no read is dead.
 // The grid holds 8 work-groups per CU so every CU streams for the whole
 // launch.  Bytes = groups * 1024 * iters * 8 * 4 W.
 
@@ -28,11 +29,14 @@ __global__ __launch_bounds__(kLdsThreads) void k_lds_read(int iters, float* __re
     V acc[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc[u] = V(0.f);
+    // the 8 reads of an iteration sit NV / 8 vectors apart, so no two of them
+    // (nor of consecutive iterations) read the same address and none can be
+    // merged by the compiler
     int base = threadIdx.x;
     for (int it = 0; it < iters; ++it) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc[u] += vb[(base + u * kLdsThreads) & (NV - 1)];
-        base = (base + 8 * kLdsThreads + 64) & (NV - 1);
+        for (int u = 0; u < 8; ++u) acc[u] += vb[(base + u * (NV / 8)) & (NV - 1)];
+        base = (base + 64) & (NV - 1);
     }
     float s = 0.f;
 #pragma unroll
