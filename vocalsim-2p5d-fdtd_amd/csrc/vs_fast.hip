@@ -281,43 +281,43 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
 
     // ------------- state and coefficients into the work-group -------------
     // Each field crosses global memory once, coalesced: the work-group copies
-    // it in padded-index order into an LDS region (the Uy region serves as
-    // the staging area for the per-lane fields), then every cell lane picks
-    // up its run j = t K + k.  Slots past the tube's last row (r >= Hi) and
-    // pad-column slots are zero.
+    // two fields at a time in padded-index order into the P and Uy regions
+    // of LDS, then every cell lane picks up its run j = t K + k.  Slots past
+    // the tube's last row (r >= Hi) and pad-column slots are zero.
     const int KT = K * T;
     const int jb = t * K;
     const bool cell_lane = t < T;
     float Ux[K], kP[K], gX[K], gY[K];
     uint32_t cw[(K + 3) / 4];
-    auto stage = [&](const float* __restrict__ src) {
+    auto stage2 = [&](const float* __restrict__ a, const float* __restrict__ c2) {
         for_padded(KT, Wp, [&](int j, int r, int c) {
-            Ug[j] = (r < Hi && c < Wi) ? src[co + (long long)(r + 1) * W + (c + 1)] : 0.f;
+            float va = 0.f, vb = 0.f;
+            if (r < Hi && c < Wi) {
+                const long long q = co + (long long)(r + 1) * W + (c + 1);
+                va = a[q];
+                vb = c2[q];
+            }
+            Pg[j] = va;
+            Ug[j] = vb;
         });
         __syncthreads();
     };
-    stage(f.Uxf);
+    stage2(f.Uxf, f.kP);
     if (cell_lane) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) Ux[k] = Ug[jb + k];
+        for (int k = 0; k < K; ++k) {
+            Ux[k] = Pg[jb + k];
+            kP[k] = Ug[jb + k];
+        }
     }
     __syncthreads();
-    stage(f.kP);
+    stage2(f.gX, f.gY);
     if (cell_lane) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) kP[k] = Ug[jb + k];
-    }
-    __syncthreads();
-    stage(f.gX);
-    if (cell_lane) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) gX[k] = Ug[jb + k];
-    }
-    __syncthreads();
-    stage(f.gY);
-    if (cell_lane) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) gY[k] = Ug[jb + k];
+        for (int k = 0; k < K; ++k) {
+            gX[k] = Pg[jb + k];
+            gY[k] = Ug[jb + k];
+        }
     }
     __syncthreads();
     for_padded(KT, Wp, [&](int j, int r, int c) {
@@ -366,7 +366,18 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
             }
         }
         __syncthreads();
+        // The two-mass step of ts + 1 is split: the new fold positions and the
+        // flow resistance and inertance (twomass_solve, twomass_flow) do not
+        // read the pressure, so they run while the cells do the pressure phase
+        // of ts; only twomass_finish, fed by the excitation cell's new
+        // pressure, is left for the velocity phase.  The operations and their
+        // order are those of twomass_step.
+        TwoMassNext nx;
         for (int ts = t0; ts < tend; ++ts) {
+            if (lane == 0 && twomass && ts + 1 < tend) {
+                twomass_solve(tm, nx);
+                twomass_flow(tm, nx);
+            }
             __syncthreads();   // pressure phase done
             if (lane == 0) {
                 b.probe[D.src_off + ts] = (double)*pL;
@@ -374,7 +385,7 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 b.strace[D.src_off + ts] = sexe[ts & 1];
                 if (ts + 1 < tend) {
                     double e;
-                    if (twomass) { tm.p1 = pexc; e = twomass_step(tm); }
+                    if (twomass) e = twomass_finish(tm, nx, pexc);
                     else e = (double)b.source[D.src_off + ts + 1];
                     sexe[(ts + 1) & 1] = (double)(float)e;
                 }
