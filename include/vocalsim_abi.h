@@ -220,6 +220,17 @@ int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len);
  * results/plt_transfer_function.py:21-23 with an evaluation at any frequency,
  * which the sub-bin formant estimates need. */
 int vs_dtft(const double* x, int n, double fs, int hann, const double* freqs, int nf, double* re, double* im);
+/* vs_dtft without any device allocation (no counterpart in the reference:
+ * for callers that cannot allocate device memory, e.g. code under graph
+ * capture or in a serving loop that forbids hipMalloc).  The device copies of
+ * x, freqs, re and im live in a static device array of the library, one per
+ * GPU and held until the process ends (2.375 MiB): n <= 262144 samples and
+ * nf <= 16384 frequencies, VS_E_RANGE beyond.  Runs on the current device's
+ * null stream and returns when the results are in re and im.  Calls on one
+ * GPU share that array: the library serialises its own calls per device, and
+ * calls from other processes on the same GPU race. */
+int vs_dtft_noalloc(const double* x, int n, double fs, int hann, const double* freqs, int nf, double* re,
+                    double* im);
 /* Same transform of a tube's device-resident probe waveform (kind 0) or
  * injected source velocity (kind 1) over the steps advanced so far. */
 int vs_probe_dtft(vs_handle* h, int tube, int kind, int hann, const double* freqs, int nf, double* re, double* im);

@@ -1303,6 +1303,18 @@ int vs_dtft(const double* x, int n, double fs, int hann, const double* freqs, in
     return rc;
 }
 
+int vs_dtft_noalloc(const double* x, int n, double fs, int hann, const double* freqs, int nf, double* re,
+                    double* im) {
+    if (!x || !freqs || !re || !im || n <= 0 || nf <= 0 || !(fs > 0))
+        return fail(VS_E_INVALID, "vs_dtft_noalloc: bad argument");
+    if (n > kNoAllocMaxSamples || nf > kNoAllocMaxFreqs)
+        return fail(VS_E_RANGE, "vs_dtft_noalloc: at most " + std::to_string(kNoAllocMaxSamples) + " samples and " +
+                                    std::to_string(kNoAllocMaxFreqs) + " frequencies");
+    const hipError_t e = dtft_noalloc(x, n, fs, hann, freqs, nf, re, im);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("vs_dtft_noalloc: ") + hipGetErrorString(e));
+    return VS_OK;
+}
+
 int vs_probe_dtft(vs_handle* h, int tube, int kind, int hann, const double* freqs, int nf, double* re, double* im) {
     if (check_handle(h) || !freqs || !re || !im || nf <= 0) return fail(VS_E_INVALID, "vs_probe_dtft: bad argument");
     if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");

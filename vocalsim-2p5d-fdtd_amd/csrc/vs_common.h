@@ -342,6 +342,12 @@ AudioFilter butter2_lowpass(double fc, double fs);
 hipError_t launch_audio(const double* d_probe, const AudioJob* d_jobs, int njobs, float* d_out, hipStream_t st);
 
 // Kernel launchers (vs_kernels.hip, vs_fast.hip, vs_adjoint.hip).
+// Allocation-free FP64 transform (vs_dtft_noalloc): limits of its static
+// device scratch array.
+constexpr int kNoAllocMaxSamples = 1 << 18;
+constexpr int kNoAllocMaxFreqs = 1 << 14;
+hipError_t dtft_noalloc(const double* x, int n, double fs, int hann, const double* freqs, int nf, double* re,
+                        double* im);
 hipError_t measure_lds_read(int width_words, int groups, int iters, float* d_out, hipStream_t st, float* ms);
 hipError_t launch_dtft(const double* d_x, long long stride, int nsig, int n, double fs, int hann,
                        const double* d_f, int nf, double* d_re, double* d_im, hipStream_t st);

@@ -36,7 +36,7 @@ EXPORTS = [
     "vs_get_source_trace", "vs_set_option", "vs_get_option", "vs_field_ptr", "vs_stream",
     "vs_last_run_timing", "vs_cells_per_step", "vs_dtft", "vs_probe_dtft", "vs_autotune",
     "vs_adjoint_forward", "vs_adjoint_backward", "vs_probe_audio", "vs_butter2_lowpass", "vs_write_wav",
-    "vs_copy_to_host", "vs_measure_lds_bandwidth",
+    "vs_copy_to_host", "vs_measure_lds_bandwidth", "vs_dtft_noalloc",
 ]
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -116,6 +116,7 @@ def load():
     L.vs_write_wav.argtypes = [ctypes.c_char_p, _F, c_size, c_int]
     L.vs_copy_to_host.argtypes = [c_void, c_void, c_size]
     L.vs_measure_lds_bandwidth.argtypes = [c_int, _D]
+    L.vs_dtft_noalloc.argtypes = [_D, c_int, ctypes.c_double, c_int, _D, c_int, _D, _D]
     for name in EXPORTS:
         if name not in ("vs_last_error", "vs_abi_version"):
             getattr(L, name).restype = c_int
