@@ -144,10 +144,19 @@ enum vs_option {
                                     (with VS_OPT_EXACT_GROUPS set: as many as that count
                                     allows, else 1); 1..32 (lowered where the LDS tile or
                                     the source placement needs it) */
-    VS_OPT_BARRIER_SPIN = 10     /* polls a grid barrier of that cooperative launch makes
+    VS_OPT_BARRIER_SPIN = 10,    /* polls a grid barrier of that cooperative launch makes
                                     before it gives up: the launch then finishes without
                                     synchronising and vs_sync reports VS_E_DEVICE.  0, the
                                     default: about a quarter of a second; >= 1: that many */
+    VS_OPT_FAST_WIDE = 11,       /* FP32 on-chip kernel layout: 1 (default) the wide-access form
+                                    (rows padded to 4 cells, 4 (mod 8) cells per lane, 16-byte LDS
+                                    accesses) whenever the batch fits it at the work-group size,
+                                    else the odd-run form; 0: always the odd-run form.  Both
+                                    give the same values (signed zeros aside). */
+    VS_OPT_FAST_LAYOUT_K = 12    /* read-only (vs_get_option): cells per lane of the on-chip
+                                    kernel the current work-group size runs, 4 (mod 8) for the
+                                    wide-access form, odd for the odd-run form, 0 when neither
+                                    fits the batch */
 };
 
 enum vs_kernel {

@@ -194,3 +194,53 @@ def test_full_batch_fp64_exact_linearity_and_parity(hip_device):
         e = rel_l2(a[i], p64)
         print(f"[full batch FP64] tube {i}: exact kernel vs FP64 oracle rel L2 = {e:.2e}", flush=True)
         assert e <= 1e-10
+
+
+WIDE_CASES = {
+    "batch16_mixed": tubes.batch_tubes(16, seed=4, num_steps=700),
+    "open_mouth": [tubes.uniform_tube(num_steps=700).replace(mouth_dirichlet=0)],
+    "no_pml_vowel": [tubes.vowel_tube(1, num_steps=700, source=tubes.SOURCE_GAUSSIAN).replace(pml=0)],
+    "vowel_twomass": [tubes.vowel_tube(0, num_steps=900)],
+    "sine": [tubes.vowel_tube(2, num_steps=700, source=tubes.SOURCE_SINE)],
+}
+
+
+@pytest.mark.parametrize("threads", [1024, 512])
+@pytest.mark.parametrize("name", list(WIDE_CASES))
+def test_wide_layout_equals_odd_layout(hip_device, name, threads):
+    # the wide-access form (rows padded to 4 cells, 4 (mod 8) cells per lane,
+    # 16-byte LDS accesses, select-free face update) gives the values of the
+    # odd-run form: probes, source traces and final states compare equal
+    specs = WIDE_CASES[name]
+    runs = {}
+    for wide in (1, 0):
+        with Solver(specs, precision=32, threads=threads, kernel=_abi.K_FAST, steps_per_launch=256) as s:
+            s.set_option(_abi.OPT_FAST_WIDE, wide)
+            K = s.get_option(_abi.OPT_FAST_LAYOUT_K)
+            if wide:
+                if K == 0 or K % 8 != 4:
+                    pytest.skip(f"the wide form does not hold this batch at {threads} threads (K = {K})")
+            else:
+                assert K % 2 == 1
+            print(f"[wide vs odd] case={name} threads={threads} wide={wide} K={K}", flush=True)
+            s.run()
+            runs[wide] = [(s.probe(i), s.source_trace(i), s.field("pressure", i), s.field("vx", i), s.field("vy", i))
+                          for i in range(len(specs))]
+    for a, b in zip(runs[1], runs[0]):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    p64, _ = orc.OracleTube(specs[0], 64).run()
+    assert rel_l2(runs[1][0][0], p64) <= 1e-3
+
+
+def test_wide_layout_graph_replays(hip_device):
+    specs = tubes.batch_tubes(8, seed=6, num_steps=1100)
+    out = []
+    for graph in (0, 1):
+        with Solver(specs, precision=32, threads=1024, kernel=_abi.K_FAST, steps_per_launch=128) as s:
+            s.set_option(_abi.OPT_USE_GRAPH, graph)
+            assert s.get_option(_abi.OPT_FAST_LAYOUT_K) % 8 == 4
+            s.run()
+            out.append([s.probe(i) for i in range(len(specs))])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)

@@ -309,6 +309,9 @@ struct vs_handle {
     int max_interior = 0;      // largest (H-2)*(W-2) of the batch
     int max_padded = 0;        // largest (H-2)*(W-1): interior with one pad column per row
     int max_wp = 0;            // widest padded row W-1
+    int max_padded4 = 0;       // the same with rows padded to a multiple of 4 (wide on-chip kernel)
+    int max_wp4 = 0;
+    int fast_wide = 1;         // VS_OPT_FAST_WIDE
     int lds_limit = 0;
     // materialised export buffers (coefficients, types, z_inv) per tube
     std::vector<void*> coeff_dev, types_dev, zinv_dev;
@@ -382,8 +385,8 @@ struct vs_handle {
 namespace {
 
 long long graph_key(const vs_handle* h) {
-    return ((long long)h->kernel << 40) ^ ((long long)h->threads << 24) ^ ((long long)h->steps_per_launch << 8) ^
-           (long long)h->graph_blocks;
+    return ((long long)h->fast_wide << 48) ^ ((long long)h->kernel << 40) ^ ((long long)h->threads << 24) ^
+           ((long long)h->steps_per_launch << 8) ^ (long long)h->graph_blocks;
 }
 
 template <typename Real>
@@ -470,6 +473,8 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
         h->max_interior = std::max(h->max_interior, (p.H - 2) * (p.W - 2));
         h->max_padded = std::max(h->max_padded, (p.H - 2) * (p.W - 1));
         h->max_wp = std::max(h->max_wp, p.W - 1);
+        h->max_padded4 = std::max(h->max_padded4, (p.H - 2) * fast_wide_wp(p.W - 2));
+        h->max_wp4 = std::max(h->max_wp4, fast_wide_wp(p.W - 2));
         vs_dims& dm = h->dims[k];
         dm.frameH = p.H; dm.frameW = p.W; dm.domainH = p.domainH; dm.domainW = p.domainW;
         dm.listenerX = p.listenerX; dm.listenerY = p.listenerY;
@@ -624,6 +629,22 @@ int build_fast_classes(vs_handle* h) {
     return VS_OK;
 }
 
+// Whether the batch fits the wide-access on-chip kernel at `threads` threads
+// (VS_OPT_FAST_WIDE on, K <= kFastWideMaxK, LDS within the device limit);
+// the guard is in floats.
+bool fast_wide_fits(const vs_handle* h, int threads, int* K_out, int* GW_out) {
+    if (!h->fast_wide || h->precision != VS_FP32 || threads < 128) return false;
+    const int K = fast_wide_cells_per_lane(h->max_padded4, threads);
+    if (K == 0) return false;
+    const int GW = h->max_wp4;
+    if ((long long)(fast_wide_lds_bytes(K, threads, GW) + fast_wide_lds_static_bytes()) > h->lds_limit) return false;
+    for (const TubeDesc& d : h->desc)
+        if (d.srate_mul != h->desc[0].srate_mul) return false;
+    if (K_out) *K_out = K;
+    if (GW_out) *GW_out = GW;
+    return true;
+}
+
 // Whether the batch fits the on-chip kernels at `threads` threads per tube
 // (cells per lane K <= kFastMaxK, LDS within the device limit, one srate_mul).
 bool fast_fits(const vs_handle* h, int threads, int* K_out, int* G_out) {
@@ -643,10 +664,12 @@ bool fast_fits(const vs_handle* h, int threads, int* K_out, int* G_out) {
 int prepare_fast(vs_handle* h, int threads) {
     if (h->precision != VS_FP32) return fail(VS_E_INVALID, "the flux-form kernel is FP32 only");
     if (threads < 128) return fail(VS_E_INVALID, "the on-chip kernel needs at least 128 threads");
+    int Kw = 0;
+    const bool wide = fast_wide_fits(h, threads, &Kw, nullptr);
     const int K = fast_cells_per_lane(h->max_padded, threads);
-    if (K > kFastMaxK) return fail(VS_E_INVALID, "tube too large for the on-chip kernel at this work-group size");
+    if (!wide && K > kFastMaxK) return fail(VS_E_INVALID, "tube too large for the on-chip kernel at this work-group size");
     const int G = fast_guard_lanes(K, h->max_wp);
-    if ((long long)(fast_lds_bytes(K, threads, G) + fast_lds_static_bytes()) > h->lds_limit)
+    if (!wide && (long long)(fast_lds_bytes(K, threads, G) + fast_lds_static_bytes()) > h->lds_limit)
         return fail(VS_E_INVALID, "on-chip kernel needs " + std::to_string(fast_lds_bytes(K, threads, G)) +
                     " bytes of LDS, device allows " + std::to_string(h->lds_limit));
     if (!h->fast_ready) {
@@ -667,7 +690,7 @@ int prepare_fast(vs_handle* h, int threads) {
         h->fast_ready = true;
     }
     {
-        hipError_t e = configure_fast(K);
+        hipError_t e = wide ? configure_fast_wide(Kw) : configure_fast(K);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("configure: ") + hipGetErrorString(e));
     }
     if (!h->flux_valid) {
@@ -779,6 +802,9 @@ hipError_t launch_any<double>(vs_handle* h, Bufs<double>& b, int t0, int ns, con
 template <>
 hipError_t launch_any<float>(vs_handle* h, Bufs<float>& b, int t0, int ns, const int* d_t0) {
     if (h->kernel == VS_K_FAST) {
+        int Kw = 0, GW = 0;
+        if (fast_wide_fits(h, h->threads, &Kw, &GW))
+            return launch_fast_wide(Kw, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, GW, d_t0, h->stream);
         const int K = fast_cells_per_lane(h->max_padded, h->threads);
         const int G = fast_guard_lanes(K, h->max_wp);
         return launch_fast_step(K, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, G, d_t0, h->stream);
@@ -1223,6 +1249,10 @@ int vs_set_option(vs_handle* h, int option, int value) {
     case VS_OPT_BARRIER_SPIN:
         if (value < 0) return fail(VS_E_INVALID, "barrier spin must be >= 0");
         h->barrier_spin = value; return VS_OK;
+    case VS_OPT_FAST_LAYOUT_K: return fail(VS_E_INVALID, "VS_OPT_FAST_LAYOUT_K is read-only");
+    case VS_OPT_FAST_WIDE:
+        if (value != 0 && value != 1) return fail(VS_E_INVALID, "fast wide must be 0 or 1");
+        h->fast_wide = value; return VS_OK;
     case VS_OPT_ADJOINT_BLOCK:
         if (value < 0) return fail(VS_E_INVALID, "adjoint block must be >= 0");
         if (value != h->adj_block) {   // slabs are re-sized by the next forward pass
@@ -1255,6 +1285,14 @@ int vs_get_option(const vs_handle* h, int option, int* value) {
     case VS_OPT_EXACT_GROUPS: *value = h->exact_groups; return VS_OK;
     case VS_OPT_EXACT_STEPS: *value = h->exact_steps; return VS_OK;
     case VS_OPT_BARRIER_SPIN: *value = h->barrier_spin; return VS_OK;
+    case VS_OPT_FAST_WIDE: *value = h->fast_wide; return VS_OK;
+    case VS_OPT_FAST_LAYOUT_K: {
+        int Kw = 0, K = 0;
+        if (fast_wide_fits(h, h->threads, &Kw, nullptr)) *value = Kw;
+        else if (fast_fits(h, h->threads, &K, nullptr)) *value = K;
+        else *value = 0;
+        return VS_OK;
+    }
     default: return fail(VS_E_INVALID, "unknown option");
     }
 }
@@ -1331,11 +1369,11 @@ int vs_probe_dtft(vs_handle* h, int tube, int kind, int hann, const double* freq
 
 namespace {
 struct TuneKey {
-    int n, max_interior, precision, kernel;
+    int n, max_interior, precision, kernel, wide;
     long long cells;
     bool operator==(const TuneKey& o) const {
         return n == o.n && max_interior == o.max_interior && precision == o.precision && kernel == o.kernel &&
-               cells == o.cells;
+               wide == o.wide && cells == o.cells;
     }
 };
 std::vector<std::pair<TuneKey, std::pair<int, int>>> g_tune_cache;
@@ -1346,7 +1384,7 @@ int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) 
     if (check_handle(h)) return VS_E_INVALID;
     if (h->steps_done != 0) return fail(VS_E_STATE, "vs_autotune needs a handle with no steps advanced");
     if (timed_steps <= 0) timed_steps = 64;
-    const TuneKey key{h->n, h->max_interior, h->precision, h->kernel, h->cells_per_step};
+    const TuneKey key{h->n, h->max_interior, h->precision, h->kernel, h->fast_wide, h->cells_per_step};
     std::string rep = "{\"candidates\": [";
     std::unique_lock<std::mutex> cache_lock(g_tune_mutex);
     for (auto& kv : g_tune_cache)
@@ -1364,8 +1402,9 @@ int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) 
     if (h->kernel == VS_K_FAST) {
         for (int t = 1024; t >= 128; t -= 64) {
             const int K = fast_cells_per_lane(h->max_padded, t);
-            if (K <= kFastMaxK && (long long)(fast_lds_bytes(K, t, fast_guard_lanes(K, h->max_wp)) +
-                                       fast_lds_static_bytes()) <= h->lds_limit)
+            if (fast_wide_fits(h, t, nullptr, nullptr) ||
+                (K <= kFastMaxK && (long long)(fast_lds_bytes(K, t, fast_guard_lanes(K, h->max_wp)) +
+                                       fast_lds_static_bytes()) <= h->lds_limit))
                 threads.push_back(t);
         }
     } else {

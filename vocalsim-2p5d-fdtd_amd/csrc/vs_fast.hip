@@ -467,6 +467,290 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
     });
 }
 
+#ifndef VS_FAST_SPLIT_K
+// ---------------------------------------------------------------------------
+// Wide-access variant of the step above (forward only), for K = 4 (mod 8).
+// Rows are padded to Wp = Wi + 1 rounded up to a multiple of 4, so every
+// lane run (t K words) and every row offset is 16-byte aligned: the run's own
+// P and Uy, the Uy of the cells below and the P of the cells above move as
+// ds_read_b128 / ds_write_b128, and with K / 4 odd the 16 lanes that one
+// b128 access serves per pass cover all 64 banks.  Guards of GW >= Wp zero
+// floats on both sides of the two regions keep every neighbour read of a
+// real or pad slot inside the region.  The class table holds (ax, ay, s, nr)
+// with s the cell's own-pressure weight and nr = 1 - r the source weight of
+// its x-face, so the face update needs no selects:
+//   fx = fma(nr, ev, fma(s, P, -fma(-nr, P_right, P_right))),  fy = fma(s, P, -P_top)
+// which for s, nr in {0, 1} rounds exactly as the selects of fast_faces, and
+// the step is written with the same fused operations as the contracted code
+// of k_fast_step, so both kernels give the same values (signed zeros aside).
+// ---------------------------------------------------------------------------
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <int K>
+__device__ __forceinline__ void wide_pressure(float* __restrict__ P, const float* __restrict__ UY,
+                                              const float* __restrict__ UYB, const float* __restrict__ ptab,
+                                              const float (&Ux)[K], const float (&kP)[K],
+                                              const uint32_t (&cw)[(K + 3) / 4], float uxl0) {
+#pragma clang fp contract(off)
+    f4v* const P4 = reinterpret_cast<f4v*>(P);
+    const f4v* const U4 = reinterpret_cast<const f4v*>(UY);
+    const f4v* const B4 = reinterpret_cast<const f4v*>(UYB);
+#pragma unroll
+    for (int m = 0; m < K / 4; ++m) {
+        const f4v p = P4[m], uy = U4[m], uyb = B4[m];
+        f4v pn;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = 4 * m + i;
+            const float left = k > 0 ? Ux[k - 1] : uxl0;
+            const float dv = (Ux[k] - left) + (uy[i] - uyb[i]);
+            const float aP = ptab[class_of<K>(cw, k)];
+            pn[i] = __builtin_fmaf(p[i], aP, -(dv * kP[k]));
+        }
+        P4[m] = pn;
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void wide_faces(const float* __restrict__ P, const float* __restrict__ PA,
+                                           float* __restrict__ UY, float* __restrict__ UXE,
+                                           const f4v* __restrict__ ftab4, float (&Ux)[K],
+                                           const float (&gX)[K], const float (&gY)[K],
+                                           const uint32_t (&cw)[(K + 3) / 4], float ev) {
+#pragma clang fp contract(off)
+    const f4v* const P4 = reinterpret_cast<const f4v*>(P);
+    const f4v* const A4 = reinterpret_cast<const f4v*>(PA);
+    f4v* const U4 = reinterpret_cast<f4v*>(UY);
+    f4v p = P4[0];
+#pragma unroll
+    for (int m = 0; m < K / 4; ++m) {
+        // P of the next four cells; past the run, the next lane's first cell
+        f4v pnext;
+        if (m + 1 < K / 4) pnext = P4[m + 1];
+        else pnext[0] = P[K];
+        const f4v pa = A4[m], uy = U4[m];
+        f4v un;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = 4 * m + i;
+            const float pc = p[i];
+            const float pr = i < 3 ? p[i + 1] : pnext[0];
+            const f4v c = ftab4[class_of<K>(cw, k)];   // ax, ay, s, nr
+            const float rpr = __builtin_fmaf(-c[3], pr, pr);
+            const float fx = __builtin_fmaf(c[3], ev, __builtin_fmaf(c[2], pc, -rpr));
+            const float fy = __builtin_fmaf(c[2], pc, -pa[i]);
+            Ux[k] = __builtin_fmaf(Ux[k], c[0], gX[k] * fx);
+            un[i] = __builtin_fmaf(c[1], uy[i], gY[k] * fy);
+        }
+        U4[m] = un;
+        p = pnext;
+    }
+    *UXE = Ux[K - 1];
+}
+
+template <int K>
+__global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__ tubes,
+                                                    const int* __restrict__ order, Bufs<float> b,
+                                                    FastBufs f, int t0_arg, int nsteps,
+                                                    const int* __restrict__ d_t0, int GW) {
+    static_assert(K % 8 == 4, "the wide kernel needs K = 4 (mod 8)");
+    extern __shared__ f4v lds4[];
+    float* const lds = reinterpret_cast<float*>(lds4);
+    __shared__ f4v ftab4[kFastClasses];          // (ax, ay, s, nr) by class
+    __shared__ float ptab[kFastClasses];         // aP by class
+    __shared__ double sexe[2];                   // source value of steps t, t + 1
+    __shared__ TwoMassState tm;
+    const int T = blockDim.x - 64, t = threadIdx.x;
+    const int KT = K * T;
+    float* const Pg = lds + GW;                  // P[j], j in [-GW, KT + GW)
+    float* const Ug = lds + KT + 3 * GW;         // Uy[j]
+    float* const Xg = lds + 2 * KT + 4 * GW + 4; // Ux of lane l's last cell at Xg[l], Xg[-1] = 0
+
+    const int tube = order[blockIdx.x];
+    const TubeDesc& D = tubes[tube];
+    const int H = D.H, W = D.W, Hi = H - 2, Wi = W - 2;
+    const int Wp = fast_wide_wp(Wi);
+    const int t0 = d_t0 ? *d_t0 : t0_arg;
+    const int tend = min(t0 + nsteps, D.num_steps);
+    const long long co = D.cell_off;
+
+    // zero guards: GW floats before and after both regions, and Xg[-1]
+    for (int i = t; i < 4 * GW + 1; i += blockDim.x) {
+        if (i < 4 * GW) {
+            const int r = i / (2 * GW), g = i - r * 2 * GW;
+            float* const base = r == 0 ? Pg : Ug;
+            base[g < GW ? g - GW : KT + (g - GW)] = 0.f;
+        } else {
+            Xg[-1] = 0.f;
+        }
+    }
+
+    const int jb = t * K;
+    const bool cell_lane = t < T;
+    float Ux[K], kP[K], gX[K], gY[K];
+    uint32_t cw[(K + 3) / 4];
+    auto stage2 = [&](const float* __restrict__ a, const float* __restrict__ c2) {
+        for_padded(KT, Wp, [&](int j, int r, int c) {
+            float va = 0.f, vb = 0.f;
+            if (r < Hi && c < Wi) {
+                const long long q = co + (long long)(r + 1) * W + (c + 1);
+                va = a[q];
+                vb = c2[q];
+            }
+            Pg[j] = va;
+            Ug[j] = vb;
+        });
+        __syncthreads();
+    };
+    stage2(f.Uxf, f.kP);
+    if (cell_lane) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            Ux[k] = Pg[jb + k];
+            kP[k] = Ug[jb + k];
+        }
+    }
+    __syncthreads();
+    stage2(f.gX, f.gY);
+    if (cell_lane) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            gX[k] = Pg[jb + k];
+            gY[k] = Ug[jb + k];
+        }
+    }
+    __syncthreads();
+    for_padded(KT, Wp, [&](int j, int r, int c) {
+        const uint32_t cd = (r < Hi && c < Wi) ? (uint32_t)f.cls8[co + (long long)(r + 1) * W + (c + 1)] : 0u;
+        Ug[j] = __uint_as_float(cd);
+    });
+    __syncthreads();
+    if (cell_lane) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t cd = __float_as_uint(Ug[jb + k]);
+            if (k & 3) cw[k >> 2] |= cd << (8 * (k & 3)); else cw[k >> 2] = cd;
+        }
+    }
+    __syncthreads();
+    for_padded(KT, Wp, [&](int j, int r, int c) {
+        float p = 0.f, uy = 0.f;
+        if (r < Hi && c < Wi) {
+            const long long q = co + (long long)(r + 1) * W + (c + 1);
+            p = b.P[q];
+            uy = f.Uyf[q];
+        }
+        Pg[j] = p;
+        Ug[j] = uy;
+    });
+
+    if (t >= T) {
+        // ------------------------- control wave -------------------------
+        const int lane = t - T;
+        const bool twomass = D.source_type == 2;
+        const int jL = (D.listener / W - 1) * Wp + (D.listener % W - 1);
+        const int jE = (D.excitation / W - 1) * Wp + (D.excitation % W - 1);
+        const float* pL = Pg + jL;
+        const float* pE = Pg + jE;
+        double pexc = 0.0;
+        {
+            const float ax = f.ctab[kFastClasses + lane], ay = f.ctab[2 * kFastClasses + lane];
+            f4v c;
+            c[0] = fabsf(ax);
+            c[1] = fabsf(ay);
+            c[2] = flag_off(ax) ? 0.f : 1.f;   // s
+            c[3] = flag_off(ay) ? 1.f : 0.f;   // 1 - r
+            ftab4[lane] = c;
+            ptab[lane] = f.ctab[lane];
+        }
+        if (lane == 0) {
+            pexc = b.pexc[tube];
+            if (twomass) tm = b.tm[tube];
+            if (t0 < tend) {
+                double e;
+                if (twomass) { tm.p1 = pexc; e = twomass_step(tm); }
+                else e = (double)b.source[D.src_off + t0];
+                sexe[t0 & 1] = (double)(float)e;
+            }
+        }
+        __syncthreads();
+        TwoMassNext nx;
+        for (int ts = t0; ts < tend; ++ts) {
+            if (lane == 0 && twomass && ts + 1 < tend) {
+                twomass_solve(tm, nx);
+                twomass_flow(tm, nx);
+            }
+            __syncthreads();   // pressure phase done
+            if (lane == 0) {
+                b.probe[D.src_off + ts] = (double)*pL;
+                pexc = (double)*pE;
+                b.strace[D.src_off + ts] = sexe[ts & 1];
+                if (ts + 1 < tend) {
+                    double e;
+                    if (twomass) e = twomass_finish(tm, nx, pexc);
+                    else e = (double)b.source[D.src_off + ts + 1];
+                    sexe[(ts + 1) & 1] = (double)(float)e;
+                }
+            }
+            __syncthreads();   // velocity phase done
+        }
+        if (lane == 0) {
+            b.pexc[tube] = pexc;
+            if (twomass) b.tm[tube] = tm;
+        }
+    } else {
+        // --------------------------- cell waves ---------------------------
+        Xg[t] = Ux[K - 1];
+        __syncthreads();
+        for (int ts = t0; ts < tend; ++ts) {
+            int tl = t;
+            asm volatile("" : "+v"(tl));
+#pragma unroll
+            for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
+            {
+                const int j = tl * K;
+                wide_pressure<K>(Pg + j, Ug + j, Ug + j + Wp, ptab, Ux, kP, cw, Xg[tl - 1]);
+            }
+            __syncthreads();
+            const float ev = (float)sexe[ts & 1];
+            {
+                asm volatile("" : "+v"(tl));
+#pragma unroll
+                for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
+                const int j = tl * K;
+                wide_faces<K>(Pg + j, Pg + j - Wp, Ug + j, Xg + tl, ftab4, Ux, gX, gY, cw, ev);
+            }
+            __syncthreads();
+        }
+    }
+    // ---------------- write back ----------------
+    __syncthreads();
+    for_padded(Hi * Wp, Wp, [&](int j, int r, int c) {
+        if (c < Wi) {
+            const long long q = co + (long long)(r + 1) * W + (c + 1);
+            const float uy = Ug[j];
+            b.P[q] = Pg[j];
+            f.Uyf[q] = uy;
+            b.Vy[q] = uy / b.dY[q];
+        }
+    });
+    __syncthreads();
+    if (cell_lane) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) Pg[jb + k] = Ux[k];
+    }
+    __syncthreads();
+    for_padded(Hi * Wp, Wp, [&](int j, int r, int c) {
+        if (c < Wi) {
+            const long long q = co + (long long)(r + 1) * W + (c + 1);
+            const float ux = Pg[j];
+            f.Uxf[q] = ux;
+            b.Vx[q] = ux / b.dX[q];
+        }
+    });
+}
+#endif   // VS_FAST_SPLIT_K
+
 // Face update of the scaled adjoint, a L + g (K_neighbour - K_cell), with its
 // rounding fixed (a product, a difference, one fused multiply-add) so the
 // sums kernel replays it bit for bit.
@@ -802,6 +1086,29 @@ hipError_t launch_rec_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs
     return hipGetLastError();
 }
 
+#ifndef VS_FAST_SPLIT_K
+template <int K>
+hipError_t configure_wide_k() {
+    static std::atomic<int> done{0};
+    if (done.load()) return hipSuccess;
+    const int dyn = 160 * 1024 - (int)fast_wide_lds_static_bytes();
+    const hipError_t e = hipFuncSetAttribute((const void*)k_fast_wide<K>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    if (e == hipSuccess) done.store(1);
+    return e;
+}
+
+template <int K>
+hipError_t launch_wide_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f, int t0,
+                         int nsteps, int threads, int GW, const int* d_t0, hipStream_t st) {
+    const size_t lds = fast_wide_lds_bytes(K, threads, GW);
+    hipError_t e = configure_wide_k<K>();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_fast_wide<K>), dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0, nsteps, d_t0,
+                       GW);
+    return hipGetLastError();
+}
+#endif
+
 template <int K>
 hipError_t launch_rev_k(const TubeDesc* d_tubes, const int* d_order, int n, FastBufs f, FastAdj a, int s0,
                         int ns, int guard, hipStream_t st) {
@@ -842,6 +1149,42 @@ int fast_cells_per_lane(int max_padded_cells, int threads) {
 }
 
 int fast_guard_lanes(int K, int max_padded_width) { return (max_padded_width + K - 1) / K + 1; }
+
+// Wide variant: K = 4 (mod 8), at most kFastWideMaxK (0: the batch needs more).
+int fast_wide_cells_per_lane(int max_padded_cells, int threads) {
+    const int T = threads - 64;
+    const int k = std::max(4, (max_padded_cells + T - 1) / T);
+    const int v = ((k - 4 + 7) / 8) * 8 + 4;
+    return v <= kFastWideMaxK ? v : 0;
+}
+
+size_t fast_wide_lds_bytes(int K, int threads, int guard_floats) {
+    const int T = threads - 64;
+    return (size_t)(2 * K * T + 4 * guard_floats + 4 + T) * sizeof(float);
+}
+
+size_t fast_wide_lds_static_bytes() {
+    return sizeof(TwoMassState) + kFastClasses * (4 * sizeof(float) + sizeof(float)) + 2 * sizeof(double) + 64;
+}
+
+hipError_t configure_fast_wide(int K) {
+    switch (K) {
+    case 4: return configure_wide_k<4>();
+    case 12: return configure_wide_k<12>();
+    case 20: return configure_wide_k<20>();
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fast_wide(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
+                            int t0, int nsteps, int threads, int guard_floats, const int* d_t0, hipStream_t st) {
+    switch (K) {
+    case 4: return launch_wide_k<4>(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard_floats, d_t0, st);
+    case 12: return launch_wide_k<12>(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard_floats, d_t0, st);
+    case 20: return launch_wide_k<20>(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard_floats, d_t0, st);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 size_t fast_lds_static_bytes() {
     return sizeof(TwoMassState) + kFastClasses * (sizeof(float2) + sizeof(float)) + 2 * sizeof(double) + 64;

@@ -26,7 +26,8 @@ COEFFS = ["N_out", "N_in", "minVxBeta", "minVyBeta", "maxVxSigmaPrimedt",
           "excitationWeightVx", "excitationWeightVy", "xor_val1", "xor_val2", "xor_val3", "xor_val4"]
 
 (OPT_STEPS_PER_LAUNCH, OPT_THREADS, OPT_USE_GRAPH, OPT_KERNEL, OPT_TIMING, OPT_ENGINE_VARIANT,
- OPT_ADJOINT_KERNEL, OPT_ADJOINT_BLOCK, OPT_EXACT_GROUPS, OPT_EXACT_STEPS, OPT_BARRIER_SPIN) = range(11)
+ OPT_ADJOINT_KERNEL, OPT_ADJOINT_BLOCK, OPT_EXACT_GROUPS, OPT_EXACT_STEPS, OPT_BARRIER_SPIN,
+ OPT_FAST_WIDE, OPT_FAST_LAYOUT_K) = range(13)
 F_AS_F32 = 0x1000
 K_EXACT, K_FAST = 0, 1
 
