@@ -19,11 +19,15 @@ sys.path.insert(0, os.path.join(ROOT, "vocalsim-2p5d-fdtd_amd"))
 BYTES_FP32 = 37   # P, Vx, Vy read+write, dP, dX, dY read (4 B each), 1 B class
 BYTES_FP64 = 73
 HBM_PEAK_GBS = 8000.0
-# LDS of the on-chip FP32 step per cell and step, from its compiled code:
-# reads P, Uy, Uy below, aP, P above, P right (4 B each), (ax, ay) (8 B), the
-# old Uy (4 B); writes P', Uy' (4 B each).  Rated on real cells only; the
-# peak is measured live (vs_measure_lds_bandwidth).
-LDS_BYTES_PER_CELL = 44
+# LDS bytes per cell and step of the on-chip FP32 step, from its compiled
+# code.  Odd-run form: reads P, Uy, Uy below, aP, P above, P right (4 B
+# each), (ax, ay) (8 B), the old Uy (4 B); writes P', Uy' (4 B each).  Wide
+# form (K = 4 mod 8): reads P, Uy, Uy below, P, P above, Uy (4 B each, as
+# 16-byte vectors), aP (4 B), (ax, ay, s, nr) (16 B), a run end per K cells;
+# writes P', Uy' (4 B each): 67 840 B per wave and step at K = 20, i.e. 53 B
+# per cell slot.  Rated on real cells only; the peak is measured live
+# (vs_measure_lds_bandwidth).
+LDS_BYTES_PER_CELL = {"odd": 44, "wide": 53}
 # algorithmic flops per cell-update (pressure 15, Vx 18, Vy 18; counted from
 # FDTDEngineCPU.cpp:320-405) and MI355X's published dense FP32 vector peak
 FLOPS_PER_CELL = 51
@@ -189,6 +193,9 @@ def main():
                                             for c in rep.get("candidates", [])) + f" -> threads {threads}")
     cells = s.cells_per_step
     dims = s.dims
+    layout_k = s.get_option(_abi.OPT_FAST_LAYOUT_K) if kernel == _abi.K_FAST else 0
+    layout = "wide" if layout_k % 8 == 4 else "odd"
+    log(f"[bench] on-chip layout: {layout} form, K = {layout_k}")
     log(f"[bench] interior cells per step: {cells}, threads {threads}")
     # 256-step blocks replayed as hipGraphs of GRAPH_BLOCKS launches (the
     # remainder, if any, as ordinary launches); the warm-up captures the graph
@@ -215,7 +222,7 @@ def main():
     hbm_alg = alg_bytes / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
     # the bound the kernel works against: LDS bytes of real cells over the
     # launch, against the device's LDS read bandwidth measured here
-    lds_bytes = cells * BLOCK * LDS_BYTES_PER_CELL
+    lds_bytes = cells * BLOCK * LDS_BYTES_PER_CELL[layout]
     lds_achieved = lds_bytes / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
     lds_peaks = {}
     for w in (4, 8, 16):
@@ -225,9 +232,6 @@ def main():
     lds_peak = max(lds_peaks.values())
     log(f"[bench] LDS read bandwidth (GB/s): " + ", ".join(f"{k} {v:.0f}" for k, v in lds_peaks.items()))
     flops = cells * BLOCK * FLOPS_PER_CELL / (mean_launch_ms * 1e-3) / 1e12 if mean_launch_ms > 0 else 0.0
-    max_padded = max((d["frameH"] - 2) * (d["frameW"] - 1) for d in dims)
-    lanes = threads - 64
-    k_slots = (-(-max_padded // lanes)) | 1
 
     # ---------------- plain FP64 single tube ----------------
     spec64 = tubes.vowel_tube(0, num_steps=33075)
@@ -360,11 +364,12 @@ def main():
                      "frac": lds_achieved / lds_peak if lds_peak > 0 else None,
                      "traffic": PMC_TRAFFIC_BYTES if (kernel == _abi.K_FAST and threads == PMC_THREADS
                                                       and args.tubes == 256) else None,
-                     "kernel": "k_fast_step<K> (batched FP32)", "mean_launch_ms": mean_launch_ms,
-                     "basis": f"{LDS_BYTES_PER_CELL} B of LDS traffic per real cell and step x {cells} cells x "
+                     "kernel": ("k_fast_wide<K>" if layout == "wide" else "k_fast_step<K>") + " (batched FP32)",
+                     "cells_per_lane": layout_k, "mean_launch_ms": mean_launch_ms,
+                     "basis": f"{LDS_BYTES_PER_CELL[layout]} B of LDS traffic per real cell and step x {cells} cells x "
                               f"{BLOCK} steps per launch; peak = LDS read bandwidth measured on this device "
                               f"(vs_measure_lds_bandwidth, widest read)",
-                     "lds_peak_by_width_gbs": lds_peaks, "slots_per_lane": k_slots,
+                     "lds_peak_by_width_gbs": lds_peaks,
                      "valu": {"achieved_tflops": flops, "peak_tflops": FP32_PEAK_TFLOPS,
                               "frac": flops / FP32_PEAK_TFLOPS,
                               "basis": f"{FLOPS_PER_CELL} algorithmic flops per cell-update"},

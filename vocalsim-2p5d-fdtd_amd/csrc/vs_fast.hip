@@ -497,15 +497,20 @@ __device__ __forceinline__ void wide_pressure(float* __restrict__ P, const float
     const f4v* const B4 = reinterpret_cast<const f4v*>(UYB);
 #pragma unroll
     for (int m = 0; m < K / 4; ++m) {
+        // a group's class-table reads go out with its state reads, so the
+        // group waits for LDS once
+        float aP[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) aP[i] = ptab[class_of<K>(cw, 4 * m + i)];
         const f4v p = P4[m], uy = U4[m], uyb = B4[m];
+        __builtin_amdgcn_sched_barrier(0);
         f4v pn;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int k = 4 * m + i;
             const float left = k > 0 ? Ux[k - 1] : uxl0;
             const float dv = (Ux[k] - left) + (uy[i] - uyb[i]);
-            const float aP = ptab[class_of<K>(cw, k)];
-            pn[i] = __builtin_fmaf(p[i], aP, -(dv * kP[k]));
+            pn[i] = __builtin_fmaf(p[i], aP[i], -(dv * kP[k]));
         }
         P4[m] = pn;
     }
@@ -524,18 +529,23 @@ __device__ __forceinline__ void wide_faces(const float* __restrict__ P, const fl
     f4v p = P4[0];
 #pragma unroll
     for (int m = 0; m < K / 4; ++m) {
-        // P of the next four cells; past the run, the next lane's first cell
+        // the group's class-table and state reads go out together (one wait);
+        // P of the next four cells, past the run the next lane's first cell
+        f4v c4[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c4[i] = ftab4[class_of<K>(cw, 4 * m + i)];   // ax, ay, s, nr
         f4v pnext;
         if (m + 1 < K / 4) pnext = P4[m + 1];
         else pnext[0] = P[K];
         const f4v pa = A4[m], uy = U4[m];
+        __builtin_amdgcn_sched_barrier(0);
         f4v un;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int k = 4 * m + i;
             const float pc = p[i];
             const float pr = i < 3 ? p[i + 1] : pnext[0];
-            const f4v c = ftab4[class_of<K>(cw, k)];   // ax, ay, s, nr
+            const f4v c = c4[i];
             const float rpr = __builtin_fmaf(-c[3], pr, pr);
             const float fx = __builtin_fmaf(c[3], ev, __builtin_fmaf(c[2], pc, -rpr));
             const float fy = __builtin_fmaf(c[2], pc, -pa[i]);
