@@ -38,12 +38,14 @@ FP32_PEAK_TFLOPS = 157.3
 COMPULSORY_BYTES_PER_CELL = 6 * 4 + 1 + 5 * 4 + 2 * 4
 BLOCK = 256
 GRAPH_BLOCKS = 4   # 256-step launches per captured step graph (the library's default)
-# HBM bytes per 256-step launch of k_fast_step<19> on the 256-tube batch at 1024
-# threads (the autotuned shape), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# passes, each scaled by the same counter over a plain copy kernel of known size
-# (FETCH x2.000, WRITE x1.000); profiles/k_fast_step_batch256_fp32_pmc.md.
-PMC_TRAFFIC_BYTES = 1.834e9
+# HBM bytes per 256-step launch of k_fast_wide<20> on the 256-tube batch at
+# 1024 threads (the autotuned shape), from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes, each scaled by the same counter over a plain copy kernel
+# of known size (FETCH x2.000, WRITE x1.000): 102.5 MB read, 62.7 MB written;
+# profiles/k_fast_wide_batch256_fp32_pmc.md.
+PMC_TRAFFIC_BYTES = 165.16e6
 PMC_THREADS = 1024
+PMC_LAYOUT_K = 20
 
 
 def log(msg):
@@ -145,6 +147,8 @@ def adjoint_leg(args, np, torch, vocalsim, tubes, _abi):
 
     one_pass()   # untimed: first-call costs (kernel loading, library GEMM selection)
     A, loss, t0, t1, t2 = one_pass()
+    from vocalsim.adjoint import last_backward_timing
+    rev_ms, chain_ms = last_backward_timing()
     out = {"workload": f"{B} seeded 44-section vowel tubes, Gaussian pulse, {n} steps; loss = mean squared "
                        f"ln|Y(f)| difference over 1024 bins 0-5 kHz against {B} other seeds",
            "precision": "f32 state, f64 gradient accumulation", "checkpoint_every": 512,
@@ -152,9 +156,10 @@ def adjoint_leg(args, np, torch, vocalsim, tubes, _abi):
            "cells_per_step": cells, "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
            "forward_cell_updates_per_s": cells * n / (t1 - t0),
            "gradient_cell_updates_per_s": cells * n / (t2 - t0),
+           "backward_split_ms": {"reverse_pass_on_device": rev_ms, "host_chain_to_sections": chain_ms},
            "loss": float(loss.detach()), "grad_norm_area": float(A.grad.norm())}
     log(f"[bench] gradient workload: forward {out['forward_ms']:.0f} ms, backward {out['backward_ms']:.0f} ms "
-        f"({B} tubes x {n} steps)")
+        f"(reverse pass {rev_ms:.0f} ms, host chain {chain_ms:.1f} ms; {B} tubes x {n} steps)")
     return out
 
 
@@ -363,7 +368,7 @@ def main():
         "roofline": {"bound": "lds", "achieved": lds_achieved, "peak": lds_peak, "unit": "GB/s",
                      "frac": lds_achieved / lds_peak if lds_peak > 0 else None,
                      "traffic": PMC_TRAFFIC_BYTES if (kernel == _abi.K_FAST and threads == PMC_THREADS
-                                                      and args.tubes == 256) else None,
+                                                      and layout_k == PMC_LAYOUT_K and args.tubes == 256) else None,
                      "kernel": ("k_fast_wide<K>" if layout == "wide" else "k_fast_step<K>") + " (batched FP32)",
                      "cells_per_lane": layout_k, "mean_launch_ms": mean_launch_ms,
                      "basis": f"{LDS_BYTES_PER_CELL[layout]} B of LDS traffic per real cell and step x {cells} cells x "
@@ -375,7 +380,7 @@ def main():
                               "basis": f"{FLOPS_PER_CELL} algorithmic flops per cell-update"},
                      "hbm": {"algorithmic_gbs_if_streamed": hbm_alg, "peak_gbs": HBM_PEAK_GBS,
                              "compulsory_bytes_per_launch": cells * COMPULSORY_BYTES_PER_CELL,
-                             "measured_bytes_per_launch": PMC_TRAFFIC_BYTES,
+                             "measured_bytes_per_launch": PMC_TRAFFIC_BYTES if layout_k == PMC_LAYOUT_K else None,
                              "note": "the state stays on chip for the whole launch: HBM carries only the "
                                      "launch's load and write-back"}},
         "cpu_baseline": cpu,

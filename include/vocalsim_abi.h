@@ -307,6 +307,10 @@ int vs_write_wav(const char* path, const float* samples, size_t n, int rate);
 int vs_adjoint_forward(vs_handle* h, int checkpoint_every, double* probe, size_t probe_len);
 int vs_adjoint_backward(vs_handle* h, const double* gprobe, size_t gprobe_len, double* g_area,
                         double* g_depth_scale, double* g_mu3d, size_t n_sections_total);
+/* Wall-clock split of the last vs_adjoint_backward: the reverse pass on the
+ * device including its copies back (reverse_ms) and the host chain from the
+ * per-cell log-derivatives to the per-section inputs (chain_ms). */
+int vs_adjoint_timing(const vs_handle* h, double* reverse_ms, double* chain_ms);
 
 #ifdef __cplusplus
 }

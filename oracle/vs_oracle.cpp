@@ -26,7 +26,9 @@
  *   1: the wall-admittance velocity Vb enters with the wrong sign;
  *   2: Vb drops its inward term (xor1 P_right N_in, xor3 P_top N_in);
  *   3: z_inv is taken from whichever cell of the face is a wall, instead of
- *      the current cell only (FDTDEngineCPU.cpp:397-398). */
+ *      the current cell only (FDTDEngineCPU.cpp:397-398);
+ *   4: the probe reads one element past the end of the pressure array (a
+ *      memory error the sanitizer build must report). */
 #ifndef VSO_MUTATION
 #define VSO_MUTATION 0
 #endif
@@ -1058,6 +1060,9 @@ void Tube<Real>::run(int first, int n, double* probe, double* src) {
                 Pn[i] = v;
             }
         if (probe) probe[t] = (double)Pn[L];
+#if VSO_MUTATION == 4
+        if (probe) probe[t] = (double)Pn[Pn.size()];
+#endif
         if (src) src[t] = (double)exeV;
         for (int r = 1; r < H - 1; ++r)
             for (int c = 1; c < W - 1; ++c) {

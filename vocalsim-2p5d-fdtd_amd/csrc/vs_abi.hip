@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 #include <mutex>
+#include <chrono>
 #include <array>
 
 namespace vs {
@@ -354,6 +355,7 @@ struct vs_handle {
     double* adj_pexc = nullptr;
     TwoMassState* adj_tm = nullptr;
     std::vector<void*> adj_fast_slabs;
+    double adj_reverse_ms = 0, adj_chain_ms = 0;   // last vs_adjoint_backward: device part, host chain
 
     ~vs_handle() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1940,6 +1942,7 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
     AdjBufs<Real>& a = adj_of<Real>(h);
     Bufs<Real>& b = bufs_of<Real>(h);
     const int S = h->adj_S;
+    const auto t_start = std::chrono::steady_clock::now();
     VS_HIP(hipMemcpyAsync(a.gprobe, gprobe, sizeof(double) * glen, hipMemcpyHostToDevice, h->stream));
     const size_t cr = sizeof(Real) * (size_t)h->total_cells, cd = sizeof(double) * (size_t)h->total_cells;
     VS_HIP(hipMemsetAsync(a.LP, 0, cr, h->stream));
@@ -1973,6 +1976,7 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
     if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("raw depth: ") + hipGetErrorString(e));
     if (e2 != hipSuccess) return fail(VS_E_DEVICE, std::string("reverse pass: ") + hipGetErrorString(e2));
     if (rc != VS_OK) return rc;
+    const auto t_device = std::chrono::steady_clock::now();
     // Lp, Lx, Ly: d J / d log of each cell's kP, x-face and y-face coefficient
     std::vector<double> Lp, Lx, Ly;
     std::vector<Real> dP, dX, dY, zcol, minor, dscale;
@@ -2096,12 +2100,22 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
         for (std::thread& t : pool) t.join();
     }
     if (failed.load()) return fail(VS_E_STATE, "gradient chain: host memory allocation failed");
+    const auto t_end = std::chrono::steady_clock::now();
+    h->adj_reverse_ms = std::chrono::duration<double, std::milli>(t_device - t_start).count();
+    h->adj_chain_ms = std::chrono::duration<double, std::milli>(t_end - t_device).count();
     return VS_OK;
 }
 
 }  // namespace
 
 extern "C" {
+
+int vs_adjoint_timing(const vs_handle* h, double* reverse_ms, double* chain_ms) {
+    if (check_handle(h)) return VS_E_INVALID;
+    if (reverse_ms) *reverse_ms = h->adj_reverse_ms;
+    if (chain_ms) *chain_ms = h->adj_chain_ms;
+    return VS_OK;
+}
 
 int vs_adjoint_forward(vs_handle* h, int checkpoint_every, double* probe, size_t probe_len) {
     if (int rc = check_handle(h)) return rc;

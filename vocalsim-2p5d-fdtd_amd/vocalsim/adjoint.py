@@ -27,6 +27,14 @@ from .solver import Solver
 from .tubes import TubeSpec
 
 
+_last_backward_timing = (0.0, 0.0)
+
+
+def last_backward_timing():
+    """(device reverse pass ms, host chain ms) of the last TubeProbe backward."""
+    return _last_backward_timing
+
+
 class TubeProbe(torch.autograd.Function):
     @staticmethod
     def forward(ctx, area, depth_scale, mu3d, specs: Sequence[TubeSpec], precision: int = 64,
@@ -59,6 +67,8 @@ class TubeProbe(torch.autograd.Function):
         device, dtype = ctx.meta
         g = gprobe.detach().to("cpu", torch.float64).contiguous().numpy()
         ga, gs, gm = ctx.solver.adjoint_backward([g[b] for b in range(g.shape[0])])
+        global _last_backward_timing
+        _last_backward_timing = ctx.solver.adjoint_timing()
         ctx.solver.close()
         ctx.solver = None
         to = lambda x: torch.from_numpy(np.stack(x)).to(device=device, dtype=dtype)

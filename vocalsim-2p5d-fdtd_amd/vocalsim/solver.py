@@ -142,6 +142,12 @@ class Solver:
         split = np.cumsum([d["num_sections"] for d in self.dims])[:-1]
         return np.split(ga, split), np.split(gs, split), np.split(gm, split)
 
+    def adjoint_timing(self):
+        """(device reverse pass ms, host chain ms) of the last adjoint_backward."""
+        r, c = ctypes.c_double(), ctypes.c_double()
+        check(load().vs_adjoint_timing(self.h, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
+
     def _split_steps(self, flat: np.ndarray) -> List[np.ndarray]:
         split = np.cumsum([d["num_steps"] for d in self.dims])[:-1]
         return np.split(flat, split)
