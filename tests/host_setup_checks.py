@@ -1,0 +1,328 @@
+"""Checks of the library's host-only set-up code (csrc/vs_host.cpp: the
+float32 prelude, the single-plane contours, the cell-class enumeration of
+both step forms), built into a small CPU library with
+tests/host_setup_harness.cpp and compared with the oracle.
+
+Used in-process on a plain build (tests/test_host_setup.py) and, through
+``main``, in a child process on a build with AddressSanitizer and
+UndefinedBehaviorSanitizer.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for _p in (os.path.join(ROOT, "vocalsim-2p5d-fdtd_amd"), os.path.join(ROOT, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import oracle as orc  # noqa: E402
+from vocalsim import tubes  # noqa: E402
+from vocalsim._abi import TubeSpecC, spec_to_c  # noqa: E402
+
+AREA_FILES = os.path.join(ROOT, "tests", "golden", "tube-geometry")
+HOST_SRC = os.path.join(ROOT, "vocalsim-2p5d-fdtd_amd", "csrc", "vs_host.cpp")
+HARNESS_SRC = os.path.join(ROOT, "tests", "host_setup_harness.cpp")
+DIMS = ["frameH", "frameW", "domainH", "domainW", "listenerX", "listenerY",
+        "excitationX", "excitationY", "num_steps", "tube_start_x", "tube_end_x",
+        "tube_len_cells", "midY", "num_sections", "mic_cells"]
+SANITIZE = ["-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+            "-fno-sanitize-recover=undefined"]
+
+_P = ctypes.POINTER
+_c = ctypes
+
+
+def build(out_path: str, sanitize: bool = False) -> str:
+    flags = SANITIZE if sanitize else ["-O2"]
+    subprocess.run(["g++", *flags, "-std=c++17", "-fPIC", "-ffp-contract=off", "-shared", "-o", out_path,
+                    HOST_SRC, HARNESS_SRC], check=True)
+    return out_path
+
+
+def load(path: str):
+    L = ctypes.CDLL(path)
+    L.hs_prelude.argtypes = [_P(TubeSpecC), _c.c_int, _P(_c.c_int), _P(_c.c_double), _c.c_int, _c.c_char_p, _c.c_int]
+    L.hs_plane_section.argtypes = [_c.c_float, _P(_c.c_float), _P(_c.c_float), _P(_c.c_double), _c.c_char_p, _c.c_int]
+    L.hs_exact_classes.argtypes = [_P(_c.c_uint8), _c.c_int, _c.c_int, _P(_c.c_uint8), _P(_c.c_int), _P(_c.c_int),
+                                   _c.c_char_p, _c.c_int]
+    L.hs_fast_classes.argtypes = [_P(_c.c_uint16), _c.c_long, _P(_c.c_float), _P(_c.c_uint8), _P(_c.c_float),
+                                  _P(_c.c_int), _c.c_char_p, _c.c_int]
+    L.hs_planted_overflow.argtypes = [_c.c_int]
+    return L
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(_P(t))
+
+
+def prelude(L, spec, fp64: bool):
+    """(rc, dims dict, per-section array [n, 5], message)."""
+    c, keep = spec_to_c(spec)
+    dims = (ctypes.c_int * 15)()
+    n = max(spec.num_sections, 1)
+    sec = np.zeros((n, 5))
+    err = ctypes.create_string_buffer(256)
+    rc = L.hs_prelude(ctypes.byref(c), int(fp64), dims, _ptr(sec, _c.c_double), n, err, 256)
+    del keep
+    return rc, dict(zip(DIMS, list(dims))), sec, err.value.decode()
+
+
+def plane_section(L, area: float):
+    npt = L.hs_plane_points()
+    y = np.zeros(npt, dtype=np.float32)
+    z = np.zeros(npt, dtype=np.float32)
+    info = np.zeros(5)
+    err = ctypes.create_string_buffer(256)
+    rc = L.hs_plane_section(area, _ptr(y, _c.c_float), _ptr(z, _c.c_float), _ptr(info, _c.c_double), err, 256)
+    assert rc == npt, (rc, err.value)
+    return y, z, dict(zip(["minY", "minZ", "maxZ", "diameter", "steps"], info.tolist()))
+
+
+def exact_classes(L, cls):
+    H, W = cls.shape
+    cls = np.ascontiguousarray(cls, dtype=np.uint8)
+    xcls = np.zeros_like(cls)
+    keys = np.zeros(L.hs_exact_max(), dtype=np.int32)
+    nk = ctypes.c_int(0)
+    err = ctypes.create_string_buffer(256)
+    rc = L.hs_exact_classes(_ptr(cls, _c.c_uint8), H, W, _ptr(xcls, _c.c_uint8), _ptr(keys, _c.c_int),
+                            ctypes.byref(nk), err, 256)
+    return rc, xcls, keys[:nk.value], err.value.decode()
+
+
+def fast_classes(L, code, lut):
+    code = np.ascontiguousarray(code, dtype=np.uint16)
+    lut = np.ascontiguousarray(lut, dtype=np.float32)
+    c8 = np.zeros(code.size, dtype=np.uint8)
+    nf = L.hs_fast_max()
+    tab = np.full(5 * nf, np.nan, dtype=np.float32)
+    ncls = ctypes.c_int(0)
+    err = ctypes.create_string_buffer(256)
+    rc = L.hs_fast_classes(_ptr(code, _c.c_uint16), code.size, _ptr(lut, _c.c_float), _ptr(c8, _c.c_uint8),
+                           _ptr(tab, _c.c_float), ctypes.byref(ncls), err, 256)
+    return rc, c8, tab.reshape(5, nf), ncls.value, err.value.decode()
+
+
+# ---------------------------------------------------------------------------
+# cases
+
+
+def cases(with_plane_area_file: bool = True):
+    out = {
+        "uniform": tubes.uniform_tube(num_steps=50),
+        "uniform_no_pml": tubes.uniform_tube(num_steps=50, pml=0),
+        "uniform_open_mouth": tubes.uniform_tube(num_steps=50).replace(mouth_dirichlet=0),
+        "uniform_2d": tubes.uniform_tube(num_steps=50).replace(solver_type=0),
+        "uniform_srate8": tubes.uniform_tube(num_steps=50).replace(srate_mul=8),
+        "uniform_float_steps": tubes.uniform_tube().replace(num_steps=0, audio_dur=0.01),
+        "open_space": tubes.open_space(num_steps=30),
+        "open_space_2d_no_pml": tubes.open_space(num_steps=30, pml=0, solver_type=0),
+        "vowel_circle": tubes.vowel_tube(1, num_steps=40).replace(shape=1, aspect=None),
+        "vowel_2d": tubes.vowel_tube(2, num_steps=40).replace(solver_type=0),
+        "plane": tubes.vowel_tube(5, num_steps=30, n_sections=6).replace(shape=3, aspect=None),
+        "plane_2d": tubes.vowel_tube(4, num_steps=30, n_sections=4).replace(shape=3, aspect=None, solver_type=0),
+    }
+    for seed in range(6):
+        out[f"vowel{seed}"] = tubes.vowel_tube(seed, num_steps=40)
+    for i, sp in enumerate(tubes.batch_tubes(12, seed=3, num_steps=40)):
+        out[f"batch{i}"] = sp
+    for name in sorted(os.listdir(AREA_FILES)):
+        sp = tubes.read_area_file(os.path.join(AREA_FILES, name)).replace(num_steps=40)
+        stem = name.split(".")[0]
+        out[f"file_{stem}"] = sp
+        out[f"file_{stem}_ellipse"] = sp.replace(shape=2)
+        if with_plane_area_file and stem == "vocaltract_a":
+            out[f"file_{stem}_plane"] = sp.replace(shape=3)
+    return out
+
+
+def bad_specs():
+    u = tubes.uniform_tube(num_steps=50)
+    return {
+        "negative_area": u.replace(areas_cm2=np.array([-1.0])),
+        "zero_length": u.replace(lengths_cm=np.array([0.0])),
+        "shape4": u.replace(shape=4),
+        "solver2": u.replace(solver_type=2),
+        "srate0": u.replace(srate_mul=0),
+        "source3": u.replace(source_type=3),
+        "open_mouth_no_pml": u.replace(mouth_dirichlet=0, pml=0),
+        "zero_aspect": tubes.vowel_tube(0, num_steps=40).replace(aspect=np.zeros(44)),
+        "negative_mu": u.replace(mu3d=np.array([-0.01])),
+        "zero_depth_scale": u.replace(depth_scale=np.array([0.0])),
+        "too_short": u.replace(lengths_cm=np.array([0.01])),
+    }
+
+
+# ---------------------------------------------------------------------------
+# checks
+
+
+def check_prelude(L, names=None, with_plane_area_file=True):
+    """Frame dimensions and per-section values of every case equal the
+    oracle's, at both precisions; the cumulative lengths are the float32
+    inclusive scan of the section lengths."""
+    n = 0
+    for name, spec in cases(with_plane_area_file).items():
+        if names is not None and name not in names:
+            continue
+        for prec in (32, 64):
+            rc, dims, sec, msg = prelude(L, spec, prec == 64)
+            assert rc == 0, f"{name}/{prec}: {msg}"
+            ref = orc.OracleTube(spec, prec)
+            assert dims == {k: ref.dims[k] for k in DIMS}, f"{name}/{prec}: frame dimensions differ"
+            if not spec.open_space:
+                assert np.array_equal(sec[:, 1], ref.field("diam_cells")), f"{name}/{prec}: diameters differ"
+                assert np.array_equal(sec[:, 3], ref.field("z_section")), f"{name}/{prec}: z_inv differs"
+                if spec.shape != 3:
+                    assert np.array_equal(sec[:, 2], ref.field("minor_axis")), f"{name}/{prec}: minor axes differ"
+                lens = spec.lengths_cm.astype(np.float32) * np.float32(1e-2)
+                assert np.array_equal(sec[:, 0], np.cumsum(lens, dtype=np.float32).astype(np.float64)), name
+                ds = np.ones(spec.num_sections, np.float32) if spec.depth_scale is None else spec.depth_scale
+                assert np.array_equal(sec[:, 4], ds.astype(np.float64)), name
+            n += 1
+    return n
+
+
+# The per-section depth scale is an input of this library (the reference has
+# none); the prelude requires it positive, while the oracle clamps the
+# resulting depth like any other.
+REFUSED_BY_PRELUDE_ONLY = {"zero_depth_scale"}
+
+
+def check_bad_specs(L):
+    """Bad specs are rejected with VS_E_INVALID and a message, and the oracle
+    rejects them too (except the depth-scale case above)."""
+    for name, spec in bad_specs().items():
+        for prec in (32, 64):
+            rc, _, _, msg = prelude(L, spec, prec == 64)
+            assert rc == -1 and msg, f"{name}/{prec}: rc {rc} ({msg})"
+            if name in REFUSED_BY_PRELUDE_ONLY:
+                continue
+            try:
+                orc.OracleTube(spec, prec)
+            except orc.OracleError:
+                pass
+            else:
+                raise AssertionError(f"{name}: the oracle accepts what the prelude rejects")
+
+
+def check_plane_contours(L, areas=(0.4, 2.84, 7.5)):
+    """Aligned single-plane contours equal the oracle's point for point."""
+    for a in areas:
+        y, z, info = plane_section(L, a)
+        ry, rz, rinfo = orc.plane_contour(a)
+        assert np.array_equal(y, ry) and np.array_equal(z, rz), f"area {a}: contour differs"
+        for k in ("minY", "minZ", "maxZ", "diameter", "steps"):
+            assert info[k] == rinfo[k], f"area {a}: {k} {info[k]} != {rinfo[k]}"
+
+
+def _cls_of(ref):
+    t = ref.field("cell_type").astype(np.int64)
+    seg = ref.field("boundary_seg") != 0
+    return (t | (seg.astype(np.int64) << 4)).astype(np.uint8)
+
+
+def _expected_keys(cls):
+    c = cls.astype(np.int64)
+    cur = c[1:-1, 1:-1]
+    key = (cur & 15) | np.where(cur & 16, 16, 0) | ((c[1:-1, 2:] & 15) << 5) | ((c[:-2, 1:-1] & 15) << 9)
+    flat = key.ravel()
+    _, first = np.unique(flat, return_index=True)
+    order = flat[np.sort(first)]
+    return key, order
+
+
+def check_exact_classes(L, names=("uniform", "vowel0", "vowel_2d", "plane", "open_space", "uniform_open_mouth",
+                                  "batch0", "file_vocaltract_i")):
+    """Classes of oracle-built frames: keys in first-seen row-major order,
+    every interior cell's index names its own key, border cells untouched; a
+    frame with more classes than the table holds is refused."""
+    cs = cases(False)
+    for name in names:
+        ref = orc.OracleTube(cs[name], 64)
+        cls = _cls_of(ref)
+        rc, xcls, keys, msg = exact_classes(L, cls)
+        assert rc == 0, f"{name}: {msg}"
+        key, order = _expected_keys(cls)
+        assert np.array_equal(keys, order), f"{name}: class keys differ"
+        assert np.array_equal(keys[xcls[1:-1, 1:-1]], key), f"{name}: class index of a cell differs"
+        border = np.ones(cls.shape, bool)
+        border[1:-1, 1:-1] = False
+        assert not xcls[border].any(), f"{name}: border cells written"
+    rng = np.random.default_rng(7)
+    noisy = rng.integers(0, 32, size=(40, 60)).astype(np.uint8)
+    rc, _, _, msg = exact_classes(L, noisy)
+    assert rc == -2 and "cell classes" in msg, (rc, msg)
+    # one interior row whose cell i has type i % 16, the segment bit of
+    # (i // 16) % 2 and a top neighbour of type i // 32: i distinct keys for
+    # i cells.  64 is the most the table holds; 65 is refused.
+    for cells, want in ((64, 0), (65, -2)):
+        i = np.arange(cells)
+        frame = np.zeros((3, cells + 2), np.uint8)
+        frame[1, 1:-1] = (i % 16) | (((i // 16) % 2) << 4)
+        frame[0, 1:-1] = i // 32
+        rc, _, keys, msg = exact_classes(L, frame)
+        assert rc == want, (cells, rc, msg)
+        if rc == 0:
+            assert len(keys) == 64
+
+
+def check_fast_classes(L):
+    """Class bytes in first-seen order over the 11 class bits of the face
+    codes (bits 3..13; the others ignored), class 0 for the zero-damping
+    code, and per class aP, ax, ay, s, r from the damping table with the
+    self-off and source-face bits in the signs of ax and ay."""
+    rng = np.random.default_rng(11)
+    lut = np.array([1.0, 0.0, 0.99, 0.98, 0.97, 0.96, 0.95, 0.94], np.float32)
+    zero = 1 + 8 * 1 + 64 * 1
+    picks = rng.choice(np.setdiff1d(np.arange(2048), [zero]), size=40, replace=False)
+    d = np.concatenate([[zero], rng.choice(np.concatenate([[zero], picks]), size=5000)])
+    code = (d << 3) | rng.integers(0, 8, d.size) | (rng.integers(0, 4, d.size) << 14)
+    rc, c8, tab, ncls, msg = fast_classes(L, code, lut)
+    assert rc == 0, msg
+    _, first = np.unique(d, return_index=True)
+    order = d[np.sort(first)]
+    assert order[0] == zero and ncls == len(order)
+    cls_of = {int(v): i for i, v in enumerate(order)}
+    assert np.array_equal(c8, np.array([cls_of[int(v)] for v in d], np.uint8))
+    for v, c in cls_of.items():
+        s0, r0 = bool(v & 0x200), bool(v & 0x400)
+        assert tab[0, c] == lut[(v >> 6) & 7]
+        assert tab[1, c] == (-1 if s0 else 1) * lut[v & 7] and np.signbit(tab[1, c]) == s0
+        assert tab[2, c] == (-1 if r0 else 1) * lut[(v >> 3) & 7] and np.signbit(tab[2, c]) == r0
+        assert tab[3, c] == (0.0 if s0 else 1.0) and tab[4, c] == (0.0 if r0 else 1.0)
+    assert not tab[:, ncls:].any(), "unused classes must be zero"
+    # 63 classes besides class 0 fit, one more does not
+    many = np.setdiff1d(np.arange(2048), [zero])[:63] << 3
+    assert fast_classes(L, many, lut)[0] == 0
+    rc, *_, msg = fast_classes(L, np.setdiff1d(np.arange(2048), [zero])[:64] << 3, lut)
+    assert rc == -1 and "cell classes" in msg, (rc, msg)
+
+
+def run_all(L, with_plane_area_file: bool = True) -> int:
+    n = check_prelude(L, with_plane_area_file=with_plane_area_file)
+    check_bad_specs(L)
+    check_plane_contours(L)
+    check_exact_classes(L)
+    check_fast_classes(L)
+    return n
+
+
+def main(argv):
+    L = load(argv[1])
+    if len(argv) > 2 and argv[2] == "overflow":
+        print("planted read", L.hs_planted_overflow(5), flush=True)
+        return 0
+    n = run_all(L, with_plane_area_file=False)
+    print(f"host set-up checks ok ({n} preludes)", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))

@@ -36,211 +36,6 @@ int fail(int code, const std::string& msg) {
             return fail(VS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));        \
     } while (0)
 
-// ---------------------------------------------------------------------------
-// Host prelude: O(sections) float32 integerisation of one tube
-// (TubeGeometry.cu:53-179, 241-286; FDTDSolver.cu:591-617).  Unit constants
-// are float32 as in include/SimulationUnits.h:11-25.
-// ---------------------------------------------------------------------------
-constexpr float kCm = 1e-2f;
-constexpr float kMm = 1e-3f;
-
-struct Prelude {
-    int H, W, domainH, domainW, L, ts, te, midY, mic, listenerX, listenerY, excX, excY;
-    int num_steps;
-    std::vector<float> cum;        // float32 inclusive scan of section lengths [m]
-    std::vector<int> diam;         // diameter in cells (odd)
-    std::vector<double> minor;     // ellipse minor axis [m] (Real-evaluated, stored in double)
-    std::vector<double> zsec;      // z_inv per section
-    std::vector<double> dscale;
-    std::vector<PlaneSection> plane;   // single-plane-symmetric contours (shape 3)
-    PhysTables ph;
-    float ds32;
-};
-
-// SimulationParams.cu:7-33 in float32, SolverParams.cu:5-57, DeviceSymbols.cu:19-20.
-void phys_tables(int srate_mul, bool fp64, PhysTables& t, float& ds32) {
-    const float c32 = 350.0f, rho32 = 1.14f;
-    const int srate = 44100 * srate_mul;
-    const float dt32 = 1.0f / (float)srate;
-    ds32 = dt32 * c32 * sqrtf(2.0f);
-    if (fp64) {
-        t.c = 350.0; t.rho = 1.14;
-        t.dt = 1.0 / (double)srate;
-        t.ds = t.dt * t.c * std::sqrt(2.0);
-        const double kappa = t.rho * t.c * t.c;
-        t.K1 = (kappa * t.dt) / t.ds;
-        t.K2 = kappa * t.dt;
-        t.rds = t.rho * t.ds;
-    } else {
-        const float kappa = rho32 * c32 * c32;
-        t.c = c32; t.rho = rho32; t.dt = dt32; t.ds = ds32;
-        t.K1 = (float)((kappa * dt32) / ds32);
-        t.K2 = (float)(kappa * dt32);
-        t.rds = (float)(rho32 * ds32);
-    }
-    for (int k = 0; k < 16; ++k) { t.beta[k] = 0; t.sigma[k] = 0; }
-    t.beta[kAir] = 1; t.beta[kNoPressure] = 1;
-    for (int k = 0; k < 6; ++k) t.beta[kPml0 + k] = 1;
-    t.sigma[kWall] = t.dt; t.sigma[kExcitation] = t.dt; t.sigma[kHead] = t.dt;
-    t.sigma[kDead] = 1000000;
-    for (int k = 0; k < 6; ++k) {
-        if (fp64) t.sigma[kPml0 + k] = ((double)k / (double)5) * 0.5;
-        else t.sigma[kPml0 + k] = ((float)k / (float)5) * 0.5f;
-    }
-    t.air_normal[0] = 0; t.air_normal[1] = 1; t.air_normal[2] = 1;
-    t.air_normal[3] = fp64 ? 0.707106 : (double)0.707106f;
-    t.wall_normal[0] = 1; t.wall_normal[1] = 1; t.wall_normal[2] = 1; t.wall_normal[3] = 0;
-    t.src_dir[0] = 0; t.src_dir[1] = 0; t.src_dir[2] = 1; t.src_dir[3] = 0;
-}
-
-// Wall admittance -> z_inv (FDTDSolver.cu:613-617).
-double z_inv_of(float mu, bool fp64, const PhysTables& t) {
-    if (fp64) {
-        const double m2 = (double)mu * ((2.0 * 0.5 * M_PI) / 1.84);
-        const double a = 1.0 / (0.5 + 0.25 * (m2 + (1 / m2)));
-        return 1 / (t.rho * t.c * ((1 + std::sqrt(1 - a)) / (1 - std::sqrt(1 - a))));
-    }
-    const float m2 = mu * ((2.0f * 0.5F * M_PI) / 1.84f);
-    const float a = 1.0f / (0.5f + 0.25f * (m2 + (1 / m2)));
-    const float rc = (float)t.rho * (float)t.c;
-    return 1 / (rc * ((1 + sqrtf(1 - a)) / (1 - sqrtf(1 - a))));
-}
-
-// Time stamps accumulated in float32 (SolverParams.cu:59-66).
-int float_step_count(float dur, float dt) {
-    int n = 0;
-    for (float t = 0; t <= dur - dt; t = t + dt) ++n;
-    return n;
-}
-
-// Open-space domain (FDTDSolver.cu:479-550): 43 x 341 air cells inside the
-// dead ring and PML, the excitation cell at the frame centre (integer halves,
-// as `ceil(frameH / 2) - 1` evaluates on ints) and the listener 5 cells below
-// and to the right of it.  No tube, no walls, constant depth.
-int prelude_open_space(const vs_tube_spec& s, bool fp64, Prelude& p) {
-    phys_tables(s.srate_mul, fp64, p.ph, p.ds32);
-    p.num_steps = s.num_steps > 0 ? s.num_steps : float_step_count(s.audio_dur, 1.0f / (float)(44100 * s.srate_mul));
-    if (p.num_steps <= 0) return fail(VS_E_INVALID, "number of steps must be positive");
-    const int pml = s.pml_flag ? 1 : 0;
-    p.domainH = 43;
-    p.domainW = 341;
-    p.H = p.domainH + 2 + 12 * pml;
-    p.W = p.domainW + 2 + 12 * pml;
-    p.excY = p.H / 2 - 1;
-    p.excX = p.W / 2 - 1;
-    p.listenerY = p.excY + 5;
-    p.listenerX = p.excX + 5;
-    p.L = 0;
-    p.ts = 1;
-    p.te = 0;
-    p.midY = p.excY;
-    p.mic = 0;
-    return VS_OK;
-}
-
-int prelude(const vs_tube_spec& s, bool fp64, Prelude& p, hipStream_t st) {
-    if (s.srate_mul <= 0) return fail(VS_E_INVALID, "srate_mul must be positive");
-    if (s.fdtd_solver_type != 0 && s.fdtd_solver_type != 1) return fail(VS_E_INVALID, "fdtd_solver_type must be 0 or 1");
-    if (s.source_type < 0 || s.source_type > 2) return fail(VS_E_INVALID, "Invalid source type");
-    if (s.open_space) return prelude_open_space(s, fp64, p);
-    if (s.num_sections <= 0 || !s.length_cm || !s.area_cm2) return fail(VS_E_INVALID, "empty area function");
-    if (s.fdtd_solver_type != 0 && s.fdtd_solver_type != 1) return fail(VS_E_INVALID, "fdtd_solver_type must be 0 or 1");
-    if (s.cross_sectional_shape < 1 || s.cross_sectional_shape > 3)
-        return fail(VS_E_INVALID, "cross_sectional_shape must be 1 (circular), 2 (elliptical) or 3 (single-plane symmetric)");
-    if (s.source_type < 0 || s.source_type > 2) return fail(VS_E_INVALID, "Invalid source type");
-    const int n = s.num_sections;
-    for (int i = 0; i < n; ++i)
-        if (!(s.area_cm2[i] > 0.0f) || !(s.length_cm[i] > 0.0f))
-            return fail(VS_E_INVALID, "section area and length must be positive");
-    phys_tables(s.srate_mul, fp64, p.ph, p.ds32);
-    const float ds = p.ds32;
-    p.num_steps = s.num_steps > 0 ? s.num_steps : float_step_count(s.audio_dur, 1.0f / (float)(44100 * s.srate_mul));
-    if (p.num_steps <= 0) return fail(VS_E_INVALID, "number of steps must be positive");
-    // sagittal widths (circle / q:1 ellipse) and minor axes
-    std::vector<float> width(n);
-    p.minor.resize(n);
-    for (int i = 0; i < n; ++i) {
-        const float a = s.area_cm2[i] * kCm * kCm;
-        if (s.cross_sectional_shape == 3) {
-            width[i] = 0.0f;   // maximum chord of the contour, below
-            p.minor[i] = 0.0;
-        } else if (s.cross_sectional_shape == 1) {
-            width[i] = 2.0 * sqrt(a / M_PI);
-            p.minor[i] = fp64 ? 2.0 * std::sqrt((double)s.area_cm2[i] * 1e-2 * 1e-2 / M_PI) : (double)width[i];
-        } else {
-            const float q = s.aspect ? s.aspect[i] : 3.0f;
-            if (!(q > 0.0f)) return fail(VS_E_INVALID, "ellipse aspect must be positive");
-            const float lx = sqrtf(a / (q * 1.0f * M_PI));
-            width[i] = lx * (q * 2);
-            if (fp64) p.minor[i] = std::sqrt((double)s.area_cm2[i] * 1e-2 * 1e-2 / ((double)q * 1.0 * M_PI)) * 2.0;
-            else p.minor[i] = lx * (1.0f * 2);
-        }
-    }
-    if (s.cross_sectional_shape == 3) {
-        std::string err;
-        const int rc = plane_sections(s.area_cm2, n, p.plane, err, st);
-        if (rc != VS_OK) return fail(rc, err);
-        for (int i = 0; i < n; ++i) width[i] = p.plane[i].diameter;
-    }
-    if (s.fdtd_solver_type == 0) {
-        // 2D area scaling (TubeGeometry.cu:241-286); pow(x, 2) as x * x
-        const float mx = *std::max_element(width.begin(), width.end());
-        size_t mi = 0;
-        for (size_t i = 0; i < width.size(); ++i) if (width[i] == mx) mi = i;
-        std::vector<float> sc(n, 0.0f), m(n, 1.0f);
-        sc[mi] = (mx * 0.5 * M_PI) / 1.84f;
-        for (int i = 0; i < n; ++i) {
-            if ((size_t)i < mi) { const float q = width[i] / width[i + 1]; m[i] = width[i + 1] != 0.0f ? q * q : 1.0f; }
-            if ((size_t)i > mi) { const float q = width[i] / width[i - 1]; m[i] = width[i - 1] != 0.0f ? q * q : 1.0f; }
-        }
-        for (int i = (int)mi - 1; i >= 0; --i) sc[i] = sc[i + 1] * m[i];
-        for (int i = (int)mi + 1; i < n; ++i) sc[i] = sc[i - 1] * m[i];
-        width = sc;
-    }
-    p.mic = (int)roundf((3 * kMm) / ds);
-    float total = 0.0f;
-    p.cum.resize(n);
-    for (int i = 0; i < n; ++i) { total = total + s.length_cm[i] * kCm; p.cum[i] = total; }
-    p.L = (int)roundf(total / ds);
-    if (p.L < 2) return fail(VS_E_INVALID, "tube shorter than two cells");
-    p.diam.resize(n);
-    for (int i = 0; i < n; ++i) {
-        int d = (int)roundf(width[i] / ds);
-        if (d == 0) d = 1;
-        if (d % 2 == 0) {
-            const float over = (float)d - width[i] / ds;
-            d = (over > 0 || s.cross_sectional_shape >= 3) ? d - 1 : d + 1;
-        }
-        p.diam[i] = d;
-    }
-    const int pml = s.pml_flag ? 1 : 0;
-    const int maxd = *std::max_element(p.diam.begin(), p.diam.end());
-    p.domainW = s.mouth_radiation_cond == 1 ? p.L + 2 : p.L + 1 + p.mic;
-    p.domainH = maxd + 2;
-    p.W = p.domainW + 2 + 12 * pml;
-    p.H = p.domainH + 2 + 12 * pml;
-    p.ts = 2 + 6 * pml;
-    p.te = p.ts + p.L - 1;
-    p.midY = (int)ceilf(p.H / 2.0f) - 1;
-    p.listenerY = p.midY;
-    p.listenerX = s.mouth_radiation_cond == 1 ? p.te - p.mic : p.te + 3 + p.mic;
-    if (p.listenerX > p.W - 2) return fail(VS_E_INVALID, "listener falls outside the frame (open mouth without PML)");
-    p.excX = p.ts - 1;
-    p.excY = p.midY;
-    const float mu_def = s.mu3d_default > 0 ? s.mu3d_default : (s.fdtd_solver_type == 1 ? 0.025f : 0.005f);
-    p.zsec.resize(n);
-    p.dscale.resize(n);
-    for (int i = 0; i < n; ++i) {
-        const float mu = s.mu3d ? s.mu3d[i] : mu_def;
-        if (!(mu > 0.0f)) return fail(VS_E_INVALID, "wall admittance must be positive");
-        p.zsec[i] = z_inv_of(mu, fp64, p.ph);
-        const float sc = s.depth_scale ? s.depth_scale[i] : 1.0f;
-        if (!(sc > 0.0f)) return fail(VS_E_INVALID, "depth scale must be positive");
-        p.dscale[i] = sc;
-    }
-    return VS_OK;
-}
-
 // Excitation table (GlottalExcitations.cu:9-52) in Real.
 template <typename Real>
 void source_table(const vs_tube_spec& s, const PhysTables& ph, int num_steps, std::vector<Real>& out) {
@@ -399,36 +194,19 @@ template <>
 Bufs<double>& bufs_of<double>(vs_handle* h) { return h->bd; }
 
 // Per tube, the distinct (cur, right, top, segment-wall) classes of its
-// interior cells, a class byte per cell, and the class coefficients computed
-// on the device in the step's own arithmetic.
+// interior cells (enumerate_exact_classes) and a class byte per cell; the
+// class coefficients are computed on the device in the step's own arithmetic.
 template <typename Real>
 int build_exact_classes(vs_handle* h, Bufs<Real>& b) {
     std::vector<uint8_t> cls((size_t)h->total_cells), xc((size_t)h->total_cells, 0);
     VS_HIP(hipMemcpy(cls.data(), b.cls, cls.size(), hipMemcpyDeviceToHost));
     std::vector<int> keys((size_t)h->n * kExactClasses, 0), nk(h->n, 0);
-    std::vector<int> id(1 << 13, -1);
     for (int k = 0; k < h->n; ++k) {
         const TubeDesc& d = h->desc[k];
-        const uint8_t* c = cls.data() + d.cell_off;
-        uint8_t* x = xc.data() + d.cell_off;
-        std::vector<int> used;
-        for (int r = 1; r < d.H - 1; ++r)
-            for (int col = 1; col < d.W - 1; ++col) {
-                const int q = r * d.W + col;
-                const int key = (c[q] & kTypeMask) | ((c[q] & kSegWallBit) ? 16 : 0) | ((c[q + 1] & kTypeMask) << 5) |
-                                ((c[q - d.W] & kTypeMask) << 9);
-                if (id[key] < 0) {
-                    if (nk[k] == kExactClasses)
-                        return fail(VS_E_GEOMETRY, "tube " + std::to_string(k) + ": more than " +
-                                                       std::to_string(kExactClasses) + " cell classes");
-                    id[key] = nk[k];
-                    keys[(size_t)k * kExactClasses + nk[k]] = key;
-                    ++nk[k];
-                    used.push_back(key);
-                }
-                x[q] = (uint8_t)id[key];
-            }
-        for (int key : used) id[key] = -1;
+        std::string err;
+        const int rc = enumerate_exact_classes(cls.data() + d.cell_off, d.H, d.W, xc.data() + d.cell_off,
+                                               keys.data() + (size_t)k * kExactClasses, nk[k], err);
+        if (rc != VS_OK) return fail(rc, "tube " + std::to_string(k) + ": " + err);
     }
     VS_HIP(hipMemcpy(b.xcls, xc.data(), xc.size(), hipMemcpyHostToDevice));
     VS_HIP(hipMemcpy(b.xkey, keys.data(), keys.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -444,8 +222,14 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
     const bool fp64 = sizeof(Real) == 8;
     std::vector<Prelude> pre(h->n);
     for (int k = 0; k < h->n; ++k) {
-        int rc = prelude(specs[k], fp64, pre[k], h->stream);
-        if (rc != VS_OK) return fail(rc, "tube " + std::to_string(k) + ": " + g_last_error);
+        std::string err;
+        const hipStream_t st = h->stream;
+        const MaxChordFn chord = [st](const std::vector<float>& y, const std::vector<float>& z, int sections, int npt,
+                                      std::vector<float>& dist, std::vector<int>& far_idx, std::string& e) {
+            return max_chord_on_device(st, y, z, sections, npt, dist, far_idx, e);
+        };
+        int rc = prelude(specs[k], fp64, pre[k], chord, err);
+        if (rc != VS_OK) return fail(rc, "tube " + std::to_string(k) + ": " + err);
     }
     h->desc.resize(h->n);
     h->dims.resize(h->n);
@@ -585,47 +369,21 @@ int check_handle(const vs_handle* h) {
     return VS_OK;
 }
 
-// Maps the 16-bit face codes of the flux-form setup to class bytes: the
-// distinct (damping triple, face-form bits) of the batch become classes
-// 1..63 (class 0 = every damping factor zero and no face-form bit, the pad
-// cells).  Per class the table holds aP, ax, ay and two 0/1 factors:
-// s = 1 unless the cell's own pressure is left out of its face updates
-// (walls carrying an admittance, excitation) and r = 1 unless the x-face
-// carries the source instead of the right neighbour's pressure (excitation).
-// The step kernels read s and r from the sign bits of their ax, ay entries
-// (ax, ay >= 0; the sign bit is set where s = 0, resp. r = 0).
+// Class bytes and class table of the FP32 flux-form kernel from the face
+// codes its setup kernel wrote (enumerate_fast_classes).  The step kernels
+// read s and r from the sign bits of their ax, ay entries.
 int build_fast_classes(vs_handle* h) {
     FastBufs& f = h->fb;
     const size_t n = (size_t)h->total_cells;
     std::vector<uint16_t> code(n);
     VS_HIP(hipStreamSynchronize(h->stream));
     VS_HIP(hipMemcpy(code.data(), f.code, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
-    // bits [3, 12) = code_x + 8 code_y + 64 code_p; bit 12 self-off, bit 13 source face
-    std::vector<int> cls_of(2048, -1);
-    const int zero = 1 + 8 * 1 + 64 * 1;
-    cls_of[zero] = 0;
-    int ncls = 1;
     std::vector<uint8_t> c8(n);
-    for (size_t i = 0; i < n; ++i) {
-        const int d = (code[i] >> 3) & 0x7ff;
-        if (cls_of[d] < 0) {
-            if (ncls == kFastClasses)
-                return fail(VS_E_INVALID, "more than " + std::to_string(kFastClasses - 1) + " cell classes in one batch");
-            cls_of[d] = ncls++;
-        }
-        c8[i] = (uint8_t)cls_of[d];
-    }
     std::vector<float> tab(5 * kFastClasses, 0.0f);
-    for (int d = 0; d < 2048; ++d) {
-        const int c = cls_of[d];
-        if (c < 0) continue;
-        const bool s0 = (d & 0x200) != 0, r0 = (d & 0x400) != 0;
-        tab[c] = f.lut[(d >> 6) & 7];
-        tab[kFastClasses + c] = s0 ? -f.lut[d & 7] : f.lut[d & 7];
-        tab[2 * kFastClasses + c] = r0 ? -f.lut[(d >> 3) & 7] : f.lut[(d >> 3) & 7];
-        tab[3 * kFastClasses + c] = s0 ? 0.0f : 1.0f;
-        tab[4 * kFastClasses + c] = r0 ? 0.0f : 1.0f;
-    }
+    int ncls = 0;
+    std::string err;
+    const int rc = enumerate_fast_classes(code.data(), n, f.lut, c8.data(), tab.data(), ncls, err);
+    if (rc != VS_OK) return fail(rc, err);
     VS_HIP(hipMemcpy(f.cls8, c8.data(), n, hipMemcpyHostToDevice));
     VS_HIP(hipMemcpy(f.ctab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
     return VS_OK;

@@ -6,34 +6,14 @@
 #include <string>
 #include <vector>
 
+#include "vs_host.h"
+
 namespace vs {
-
-// Cell classes: the reference's CellTypes enum (include/SolverParams.h:21-35).
-enum CellType : int {
-    kWall = 0, kAir = 1, kExcitation = 2, kPml0 = 3, kPml1, kPml2, kPml3, kPml4, kPml5,
-    kDynamic = 9, kDead = 10, kNoPressure = 11, kHead = 12, kNumTypes = 13
-};
-
-// Class byte stored per cell in HBM: low nibble = CellType, bit 4 = the cell
-// carries a boundary segment id (z_inv applies, FDTDSolver.cu:1068).
-constexpr uint8_t kTypeMask = 0x0f;
-constexpr uint8_t kSegWallBit = 0x10;
 
 // Status bits raised by device code (read back after setup / each run).
 constexpr unsigned kStatusSqrtDomain = 1u;   // negative depth sqrt argument
 constexpr unsigned kStatusNonFinite = 2u;    // non-finite probe sample
 constexpr unsigned kStatusBarrierTimeout = 4u;   // a cooperative step's grid barrier gave up waiting
-
-// Physics constants and per-type tables of one tube, stored in double.
-// FP32 tubes hold the float32-evaluated values widened exactly.
-struct PhysTables {
-    double dt, ds, rho, c, K1, K2, rds;   // rds = rho * ds
-    double beta[16];
-    double sigma[16];
-    double air_normal[4];
-    double wall_normal[4];
-    double src_dir[4];
-};
 
 // Device descriptor of one tube inside a batch.
 struct TubeDesc {
@@ -248,31 +228,12 @@ struct Bufs {
     Real* xtab;          // per tube: kExactClasses x kExactSlots coefficients of each class
 };
 
-// Exact step: the 17 coefficients of the reference (FDTDSolver.cu:860-1056)
-// depend only on the (cur, right, top) cell types, so each tube holds them
-// per class: slots 0..16 the coefficients, 17 = Dirichlet cell, 18 = wall
-// cell carrying a boundary segment (z_inv applies), 19 = flags of the terms
-// the velocity update cannot skip (kExactSourceTerms, kExactWallTerms).
-constexpr int kExactClasses = 64;
-constexpr int kExactSlots = 20;        // 17 coefficients, noPressure, wall, term flags
-constexpr int kExactSourceTerms = 1;   // term flags: the class has source coefficients
-constexpr int kExactWallTerms = 2;     //             the class carries a wall segment
-
-// Single-plane-symmetric cross-section of one tube section (vs_plane.hip):
-// the aligned closed contour (3997 points: 1999 on the upper half and their
-// mirror images), its bounding values and its maximum chord (the section's
-// sagittal width).
-constexpr int kPlanePoints = 3997;
-struct PlaneSection {
-    std::vector<float> y, z;
-    float minY = 0, minZ = 0, maxZ = 0;
-    float diameter = 0;
-    int grow_steps = 0;   // 0.01 mm growth steps taken to reach the area
-};
-int plane_sections(const float* area_cm2, int n, std::vector<PlaneSection>& out, std::string& err, hipStream_t st);
+// Maximum chords of single-plane-symmetric contours on the device (the
+// max-chord search of plane_sections), on stream `st`; synchronises it.
+int max_chord_on_device(hipStream_t st, const std::vector<float>& y, const std::vector<float>& z, int sections,
+                        int npt, std::vector<float>& dist, std::vector<int>& far_idx, std::string& err);
 
 // Per-cell coefficients of the FP32 flux-form kernel (vs_fast.hip).
-constexpr int kFastClasses = 64;   // cell classes of one batch (class bytes 0..63)
 struct FastBufs {
     float* kP; float* gX; float* gY;     // pressure and face coefficients
     float* Uxf; float* Uyf;              // flux state U = V * depth
