@@ -286,7 +286,9 @@ int vs_write_wav(const char* path, const float* samples, size_t n, int rate);
  * its num_steps in the handle's precision, keeps a state checkpoint
  * (P, Ux, Uy) every checkpoint_every steps in device memory, and writes the
  * probe traces (tube-major, num_steps values per tube) into probe when probe
- * is non-NULL.  It leaves the handle's ordinary state (vs_run) untouched.
+ * is non-NULL.  probe may be host memory or device memory of the handle's
+ * device (the copy's direction is taken from the pointer); it is complete
+ * when the call returns.  It leaves the handle's ordinary state (vs_run) untouched.
  * Table sources only (source_type 0 or 1): a two-mass source feeds back the
  * pressure and is rejected with VS_E_INVALID.
  *
@@ -294,7 +296,10 @@ int vs_write_wav(const char* path, const float* samples, size_t n, int rate);
  * J = sum_tubes sum_n gprobe[n] * probe[n] (gprobe tube-major like probe)
  * with respect to each section's area_cm2, depth_scale and wall admittance
  * mu3d (outputs section-major per tube, tubes concatenated; n_sections_total
- * = sum of num_sections).  Segments of checkpoint_every steps are recomputed
+ * = sum of num_sections).  gprobe may be host memory or device memory of
+ * the handle's device; a device gprobe must be complete before the call (the
+ * library reads it on its own stream).  The three gradient outputs are host
+ * memory.  Segments of checkpoint_every steps are recomputed
  * from their checkpoint and reversed step by step on the GPU; the per-cell
  * coefficient gradients are then chained on the host through the depth
  * clamps, the depth-P average, the half-cell smoothing, the ellipse depth

@@ -39,3 +39,15 @@ def test_null_arguments_rejected_without_device():
     assert L.vs_adjoint_forward(None, 512, None, 0) == -1
     assert L.vs_adjoint_backward(None, None, 0, None, None, None, 0) == -1
     assert L.vs_destroy(None) == 0
+
+
+def test_device_trace_buffers_are_checked():
+    """The adjoint's device-memory trace arguments must be contiguous float64
+    GPU tensors of the right size; anything else is refused before any call."""
+    import torch
+    from vocalsim.solver import _check_device_f64
+
+    for bad in (torch.zeros(10, dtype=torch.float64), torch.zeros(10, dtype=torch.float32),
+                torch.zeros(20, dtype=torch.float64)[::2], [0.0] * 10):
+        with pytest.raises(ValueError):
+            _check_device_f64(bad, 10, "probe output")

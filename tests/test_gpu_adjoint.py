@@ -208,6 +208,33 @@ def test_torch_function_backward(hip_device):
     assert abs(g - fd) <= FD_REL * abs(fd)
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_torch_function_device_and_host_tensors_agree(hip_device, precision):
+    """GPU tensors (the library writes the probe traces into device memory and
+    reads the gradient traces from it) and CPU tensors (host copies) give the
+    same probe and gradients bit for bit, and the probe is the ABI's."""
+    import torch
+    from vocalsim.adjoint import tube_probe
+
+    specs = [base_tube(seed=s, num_steps=700) for s in (1, 2)]
+    res = {}
+    for dev in ("cpu", "cuda"):
+        print(f"[torch adjoint] tensors on {dev}, precision {precision}", flush=True)
+        T = lambda name: torch.tensor(np.stack([getattr(sp, name) for sp in specs]), dtype=torch.float64,
+                                      device=dev).requires_grad_(True)
+        A, D, M = T("areas_cm2"), T("depth_scale"), T("mu3d")
+        p = tube_probe(A, D, M, specs, precision, 256)
+        assert p.device.type == dev and p.dtype == torch.float64 and p.shape == (2, 700)
+        w = torch.tensor(np.stack([loss_weights(700, k) for k in range(2)]), dtype=torch.float64, device=dev)
+        (p * w).sum().backward()
+        res[dev] = [x.detach().cpu().numpy() for x in (p, A.grad, D.grad, M.grad)]
+    for name, a, b in zip(("probe", "area", "depth scale", "mu3d"), res["cpu"], res["cuda"]):
+        assert np.array_equal(a, b), name
+    with Solver(specs, precision=precision, threads=1024) as s:
+        probes = s.adjoint_forward(256)
+    assert np.array_equal(res["cuda"][0], np.stack(probes))
+
+
 def test_log_spectrum_loss_gradient_four_tubes(hip_device):
     """The gradient workload: log-magnitude spectra (0-5 kHz, 1024 bins, FP64
     DFT of the probe) of 4 tubes against targets from 4 other seeds, FP64

@@ -1559,6 +1559,16 @@ int adj_pick_forward_threads(vs_handle* h) {
     return rc;
 }
 
+// Copies the forward pass's probe traces to `probe` (host or device memory:
+// the direction is taken from the pointer) and waits for the stream, so the
+// traces are complete when the entry point returns.
+int adj_probe_out(vs_handle* h, const double* src, double* probe) {
+    if (probe)
+        VS_HIP(hipMemcpyAsync(probe, src, sizeof(double) * (size_t)h->total_src, hipMemcpyDefault, h->stream));
+    VS_HIP(hipStreamSynchronize(h->stream));
+    return VS_OK;
+}
+
 // Forward pass on chip: the FP32 flux-form step kernel in S-step launches,
 // the whole state copied to a checkpoint before each.
 int adj_forward_fast(vs_handle* h, int S, double* probe) {
@@ -1582,11 +1592,9 @@ int adj_forward_fast(vs_handle* h, int S, double* probe) {
                                         nullptr, h->stream);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint forward: ") + hipGetErrorString(e));
     }
-    VS_HIP(hipStreamSynchronize(h->stream));
     h->adj_forward = true;
     h->adj_onchip = true;
-    if (probe) VS_HIP(hipMemcpy(probe, a.probe, sizeof(double) * (size_t)h->total_src, hipMemcpyDeviceToHost));
-    return VS_OK;
+    return adj_probe_out(h, a.probe, probe);
 }
 
 // Reverse pass on chip.  Per segment (last first): the segment is recomputed
@@ -1674,12 +1682,9 @@ int adj_forward(vs_handle* h, int S, double* probe, size_t len) {
         hipError_t e = launch_adj_fwd<Real>(h->d_desc, h->d_order, h->n, b, a, t0, S, 1, -1, h->threads, h->stream);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint forward: ") + hipGetErrorString(e));
     }
-    VS_HIP(hipStreamSynchronize(h->stream));
     h->adj_forward = true;
     h->adj_onchip = false;
-    if (probe)
-        VS_HIP(hipMemcpy(probe, a.probe, sizeof(double) * (size_t)h->total_src, hipMemcpyDeviceToHost));
-    return VS_OK;
+    return adj_probe_out(h, a.probe, probe);
 }
 
 template <typename T>
@@ -1701,7 +1706,8 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
     Bufs<Real>& b = bufs_of<Real>(h);
     const int S = h->adj_S;
     const auto t_start = std::chrono::steady_clock::now();
-    VS_HIP(hipMemcpyAsync(a.gprobe, gprobe, sizeof(double) * glen, hipMemcpyHostToDevice, h->stream));
+    // gprobe may be host or device memory (the direction is taken from the pointer)
+    VS_HIP(hipMemcpyAsync(a.gprobe, gprobe, sizeof(double) * glen, hipMemcpyDefault, h->stream));
     const size_t cr = sizeof(Real) * (size_t)h->total_cells, cd = sizeof(double) * (size_t)h->total_cells;
     VS_HIP(hipMemsetAsync(a.LP, 0, cr, h->stream));
     VS_HIP(hipMemsetAsync(a.LX, 0, cr, h->stream));

@@ -54,19 +54,29 @@ class TubeProbe(torch.autograd.Function):
             run.append(sp.replace(areas_cm2=a[b].astype(np.float32), depth_scale=d[b].astype(np.float32),
                                   mu3d=m[b].astype(np.float32)))
         solver = Solver(run, precision=precision, threads=threads)
-        probes = solver.adjoint_forward(checkpoint_every)
-        if len({p.size for p in probes}) != 1:
+        steps = {d["num_steps"] for d in solver.dims}
+        if len(steps) != 1:
             solver.close()
             raise ValueError("all tubes of one call must have the same number of steps")
+        N = steps.pop()
         ctx.solver = solver
         ctx.meta = (area.device, area.dtype)
-        return torch.from_numpy(np.stack(probes)).to(area.device)
+        if area.is_cuda:
+            # probe traces written by the library straight into device memory
+            out = torch.empty((B, N), dtype=torch.float64, device=area.device)
+            solver.adjoint_forward_to(out, checkpoint_every)
+            return out
+        return torch.from_numpy(np.stack(solver.adjoint_forward(checkpoint_every)))
 
     @staticmethod
     def backward(ctx, gprobe):
         device, dtype = ctx.meta
-        g = gprobe.detach().to("cpu", torch.float64).contiguous().numpy()
-        ga, gs, gm = ctx.solver.adjoint_backward([g[b] for b in range(g.shape[0])])
+        g = gprobe.detach().to(torch.float64).contiguous()
+        if g.is_cuda:
+            ga, gs, gm = ctx.solver.adjoint_backward_from(g)
+        else:
+            g = g.numpy()
+            ga, gs, gm = ctx.solver.adjoint_backward([g[b] for b in range(g.shape[0])])
         global _last_backward_timing
         _last_backward_timing = ctx.solver.adjoint_timing()
         ctx.solver.close()

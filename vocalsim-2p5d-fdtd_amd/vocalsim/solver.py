@@ -10,6 +10,13 @@ from . import _abi
 from ._abi import check, load
 
 
+def _check_device_f64(t, n: int, what: str):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+            and t.numel() == n):
+        raise ValueError(f"{what} must be a contiguous float64 GPU tensor of {n} values")
+
+
 class Solver:
     """Owns a vs_handle: a batch of tubes with their device state.
 
@@ -137,6 +144,32 @@ class Solver:
         ns = sum(d["num_sections"] for d in self.dims)
         ga, gs, gm = np.zeros(ns), np.zeros(ns), np.zeros(ns)
         check(load().vs_adjoint_backward(self.h, g.ctypes.data_as(_abi._D), g.size,
+                                         ga.ctypes.data_as(_abi._D), gs.ctypes.data_as(_abi._D),
+                                         gm.ctypes.data_as(_abi._D), ns))
+        split = np.cumsum([d["num_sections"] for d in self.dims])[:-1]
+        return np.split(ga, split), np.split(gs, split), np.split(gm, split)
+
+    def adjoint_forward_to(self, out, checkpoint_every: int = 512):
+        """:meth:`adjoint_forward` writing the tube-major probe traces straight
+        into ``out``, a contiguous float64 torch tensor on the GPU with one
+        value per tube step (no host copy)."""
+        total = sum(d["num_steps"] for d in self.dims)
+        _check_device_f64(out, total, "probe output")
+        check(load().vs_adjoint_forward(self.h, int(checkpoint_every), ctypes.cast(out.data_ptr(), _abi._D), total))
+        self._adj_total = total
+        return out
+
+    def adjoint_backward_from(self, gprobe):
+        """:meth:`adjoint_backward` reading the tube-major gradient traces from
+        ``gprobe``, a contiguous float64 torch tensor on the GPU (read by the
+        library on its own stream once the caller's current stream is done)."""
+        import torch
+        total = sum(d["num_steps"] for d in self.dims)
+        _check_device_f64(gprobe, total, "gradient traces")
+        torch.cuda.current_stream(gprobe.device).synchronize()
+        ns = sum(d["num_sections"] for d in self.dims)
+        ga, gs, gm = np.zeros(ns), np.zeros(ns), np.zeros(ns)
+        check(load().vs_adjoint_backward(self.h, ctypes.cast(gprobe.data_ptr(), _abi._D), total,
                                          ga.ctypes.data_as(_abi._D), gs.ctypes.data_as(_abi._D),
                                          gm.ctypes.data_as(_abi._D), ns))
         split = np.cumsum([d["num_sections"] for d in self.dims])[:-1]
