@@ -355,6 +355,17 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         double pexc = 0.0;
         ptab[lane] = f.ctab[lane];
         ftab[lane] = make_float2(f.ctab[kFastClasses + lane], f.ctab[2 * kFastClasses + lane]);
+        // The two-mass steps run one and two steps ahead of the cells.  While
+        // the cells do the pressure phase of ts, lane 0 evaluates the flow
+        // resistance and inertance of step ts + 1 (twomass_flow) and the part
+        // of step ts + 2's fold solve that does not depend on the glottal flow
+        // (twomass_pre).  After it, fed by the excitation cell's new pressure,
+        // it finishes step ts + 1 (twomass_finish: the source of the cells'
+        // next velocity phase) and the solve of ts + 2 (twomass_post).  Each
+        // phase thus carries about half of the step's double divisions; the
+        // operations and their order are those of twomass_step.
+        TwoMassNext nx;
+        TwoMassPre pp;
         if (lane == 0) {
             pexc = b.pexc[tube];
             if (twomass) tm = b.tm[tube];
@@ -363,20 +374,14 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 if (twomass) { tm.p1 = pexc; e = twomass_step(tm); }
                 else e = (double)b.source[D.src_off + t0];
                 sexe[t0 & 1] = (double)(float)e;
+                if (twomass && t0 + 1 < tend) twomass_solve(tm, nx);
             }
         }
         __syncthreads();
-        // The two-mass step of ts + 1 is split: the new fold positions and the
-        // flow resistance and inertance (twomass_solve, twomass_flow) do not
-        // read the pressure, so they run while the cells do the pressure phase
-        // of ts; only twomass_finish, fed by the excitation cell's new
-        // pressure, is left for the velocity phase.  The operations and their
-        // order are those of twomass_step.
-        TwoMassNext nx;
         for (int ts = t0; ts < tend; ++ts) {
             if (lane == 0 && twomass && ts + 1 < tend) {
-                twomass_solve(tm, nx);
                 twomass_flow(tm, nx);
+                if (ts + 2 < tend) twomass_pre(tm, nx.x1n, nx.x2n, tm.x1_curr, tm.x2_curr, pp);
             }
             __syncthreads();   // pressure phase done
             if (lane == 0) {
@@ -385,8 +390,12 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
                 b.strace[D.src_off + ts] = sexe[ts & 1];
                 if (ts + 1 < tend) {
                     double e;
-                    if (twomass) e = twomass_finish(tm, nx, pexc);
-                    else e = (double)b.source[D.src_off + ts + 1];
+                    if (twomass) {
+                        e = twomass_finish(tm, nx, pexc);
+                        if (ts + 2 < tend) twomass_post(tm, pp, nx);
+                    } else {
+                        e = (double)b.source[D.src_off + ts + 1];
+                    }
                     sexe[(ts + 1) & 1] = (double)(float)e;
                 }
             }
@@ -673,6 +682,17 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
             ftab4[lane] = c;
             ptab[lane] = f.ctab[lane];
         }
+        // The two-mass steps run one and two steps ahead of the cells.  While
+        // the cells do the pressure phase of ts, lane 0 evaluates the flow
+        // resistance and inertance of step ts + 1 (twomass_flow) and the part
+        // of step ts + 2's fold solve that does not depend on the glottal flow
+        // (twomass_pre).  After it, fed by the excitation cell's new pressure,
+        // it finishes step ts + 1 (twomass_finish: the source of the cells'
+        // next velocity phase) and the solve of ts + 2 (twomass_post).  Each
+        // phase thus carries about half of the step's double divisions; the
+        // operations and their order are those of twomass_step.
+        TwoMassNext nx;
+        TwoMassPre pp;
         if (lane == 0) {
             pexc = b.pexc[tube];
             if (twomass) tm = b.tm[tube];
@@ -681,14 +701,14 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
                 if (twomass) { tm.p1 = pexc; e = twomass_step(tm); }
                 else e = (double)b.source[D.src_off + t0];
                 sexe[t0 & 1] = (double)(float)e;
+                if (twomass && t0 + 1 < tend) twomass_solve(tm, nx);
             }
         }
         __syncthreads();
-        TwoMassNext nx;
         for (int ts = t0; ts < tend; ++ts) {
             if (lane == 0 && twomass && ts + 1 < tend) {
-                twomass_solve(tm, nx);
                 twomass_flow(tm, nx);
+                if (ts + 2 < tend) twomass_pre(tm, nx.x1n, nx.x2n, tm.x1_curr, tm.x2_curr, pp);
             }
             __syncthreads();   // pressure phase done
             if (lane == 0) {
@@ -697,8 +717,12 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
                 b.strace[D.src_off + ts] = sexe[ts & 1];
                 if (ts + 1 < tend) {
                     double e;
-                    if (twomass) e = twomass_finish(tm, nx, pexc);
-                    else e = (double)b.source[D.src_off + ts + 1];
+                    if (twomass) {
+                        e = twomass_finish(tm, nx, pexc);
+                        if (ts + 2 < tend) twomass_post(tm, pp, nx);
+                    } else {
+                        e = (double)b.source[D.src_off + ts + 1];
+                    }
                     sexe[(ts + 1) & 1] = (double)(float)e;
                 }
             }
