@@ -290,6 +290,16 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
     // host staging of the per-section and per-step inputs
     std::vector<float> cum; std::vector<int> diam; std::vector<Real> minor, dsc, zs, srct;
     std::vector<TwoMassState> tms(h->n);
+    srct.reserve((size_t)src);
+    // tubes whose table source has the same parameters share one table: the
+    // first such tube's values are copied (a table is num_steps evaluations
+    // of exp or sin on the host)
+    struct SrcKey {
+        int type, steps;
+        double amp, t0, width, dt;
+        size_t off;
+    };
+    std::vector<SrcKey> tables;
     for (int k = 0; k < h->n; ++k) {
         const Prelude& p = pre[k];
         cum.insert(cum.end(), p.cum.begin(), p.cum.end());
@@ -304,9 +314,28 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
             h->sec_mu.push_back((double)(specs[k].mu3d ? specs[k].mu3d[i] : mu_def));
             h->sec_scale.push_back(p.dscale[i]);
         }
-        std::vector<Real> tab;
-        source_table<Real>(specs[k], p.ph, p.num_steps, tab);
-        srct.insert(srct.end(), tab.begin(), tab.end());
+        const vs_tube_spec& s = specs[k];
+        const SrcKey key{s.source_type, p.num_steps, s.source_type == 1 ? s.gauss_amp : 0.0,
+                         s.source_type == 1 ? s.gauss_t0 : 0.0, s.source_type == 1 ? s.gauss_width : 0.0, p.ph.dt,
+                         srct.size()};
+        const SrcKey* same = nullptr;
+        for (const SrcKey& t : tables)
+            if (t.type == key.type && t.steps == key.steps && t.amp == key.amp && t.t0 == key.t0 &&
+                t.width == key.width && t.dt == key.dt) {
+                same = &t;
+                break;
+            }
+        if (same) {
+            // srct has room for every tube's table (reserved above), so
+            // appending from its own front does not reallocate
+            const size_t off = same->off;
+            for (int i = 0; i < p.num_steps; ++i) srct.push_back(srct[off + i]);
+        } else {
+            std::vector<Real> tab;
+            source_table<Real>(s, p.ph, p.num_steps, tab);
+            srct.insert(srct.end(), tab.begin(), tab.end());
+            tables.push_back(key);
+        }
         const double q = specs[k].tm_q > 0 ? specs[k].tm_q : 1.0;
         const double ps = specs[k].tm_ps > 0 ? specs[k].tm_ps : (double)686.46f;
         twomass_init(tms[k], q, ps, specs[k].srate_mul);
