@@ -202,6 +202,9 @@ WIDE_CASES = {
     "no_pml_vowel": [tubes.vowel_tube(1, num_steps=700, source=tubes.SOURCE_GAUSSIAN).replace(pml=0)],
     "vowel_twomass": [tubes.vowel_tube(0, num_steps=900)],
     "sine": [tubes.vowel_tube(2, num_steps=700, source=tubes.SOURCE_SINE)],
+    # 4 cm of 0.5 cm^2: about 1 700 padded cells, the 4-cells-per-lane shape
+    "short_narrow": [tubes.uniform_tube(num_steps=700).replace(lengths_cm=np.array([4.0]),
+                                                                areas_cm2=np.array([0.5]))],
 }
 
 
@@ -220,6 +223,8 @@ def test_wide_layout_equals_odd_layout(hip_device, name, threads):
             if wide:
                 if K == 0 or K % 8 != 4:
                     pytest.skip(f"the wide form does not hold this batch at {threads} threads (K = {K})")
+                if name == "short_narrow" and threads == 1024:
+                    assert K == 4
             else:
                 assert K % 2 == 1
             print(f"[wide vs odd] case={name} threads={threads} wide={wide} K={K}", flush=True)
