@@ -138,6 +138,7 @@ struct vs_handle {
     // on-chip FP32 reverse pass: lanes shape, slabs and scratch
     int adj_K = 0, adj_G = 0, adj_B = 0, adj_fast_S = 0, adj_sum_groups = 0;
     int adj_fwd_threads = 0;   // work-group size of the checkpointing forward (timed once)
+    int adj_fwd_wide = 0;      // the checkpointing forward runs the wide-access form
     FastAdj fa{};
     float* adj_rec[2] = {nullptr, nullptr};   // double-buffered record slabs
     float* adj_adj[2] = {nullptr, nullptr};
@@ -1532,58 +1533,65 @@ int adj_check(vs_handle* h) {
     return VS_OK;
 }
 
-// Work-group size of the checkpointing forward.  Its state is kept in cell
-// layout, so it need not share the lanes shape of the recording and reverse
-// launches; the fastest of the shapes that fit is timed once per handle (64
-// steps each, on the adjoint's own state, which the forward pass then clears).
-// (batch size, largest padded interior, widest padded row) -> forward shape,
+// Shape of the checkpointing forward.  Its state is kept in cell layout, so
+// it need not share the lanes shape of the recording and reverse launches:
+// the fastest of the shapes that fit, odd-run or wide-access form (which give
+// the same values), is timed once per handle (64 steps each, on the
+// adjoint's own state, which the forward pass then clears).
+// (batch size, largest padded interior and widest padded row in both layouts,
+// wide form allowed) -> (threads, wide),
 // shared by the handles of one process (a differentiable call builds a new
 // handle per call).
-std::vector<std::pair<std::array<int, 3>, int>> g_adj_shape_cache;
+std::vector<std::pair<std::array<int, 6>, std::array<int, 2>>> g_adj_shape_cache;
 std::mutex g_adj_shape_mutex;
 
 int adj_pick_forward_threads(vs_handle* h) {
     if (h->adj_fwd_threads) return VS_OK;
-    const std::array<int, 3> key{h->n, h->max_padded, h->max_wp};
+    const std::array<int, 6> key{h->n, h->max_padded, h->max_wp, h->max_padded4, h->max_wp4, h->fast_wide};
     {
         std::lock_guard<std::mutex> lk(g_adj_shape_mutex);
         for (auto& kv : g_adj_shape_cache)
             if (kv.first == key) {
-                h->adj_fwd_threads = kv.second;
+                h->adj_fwd_threads = kv.second[0];
+                h->adj_fwd_wide = kv.second[1];
                 return VS_OK;
             }
     }
     const Bufs<float> b = adj_fast_bufs(h);
     const FastBufs f = adj_fast_flux(h);
     float best = 1e30f;
-    int best_t = kAdjThreads;
+    int best_t = kAdjThreads, best_w = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     VS_HIP(hipEventCreate(&e0));
     VS_HIP(hipEventCreate(&e1));
     int rc = VS_OK;
-    for (int t = kAdjThreads; t >= kAdjThreads - 4 * 64 && rc == VS_OK; t -= 64) {
-        int K = 0, G = 0;
-        if (!fast_fits(h, t, &K, &G)) continue;
-        hipError_t e = configure_fast(K);
-        float ms = 1e30f;
-        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
-            e = hipEventRecord(e0, h->stream);
-            if (e == hipSuccess) e = launch_fast_step(K, h->d_desc, h->d_order, h->n, b, f, 0, 64, t, G, nullptr, h->stream);
-            if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
-            if (e == hipSuccess) e = hipEventSynchronize(e1);
-            float m = 0.f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
-            if (e == hipSuccess) ms = std::min(ms, m);
+    for (int wide = 0; wide < 2 && rc == VS_OK; ++wide)
+        for (int t = kAdjThreads; t >= kAdjThreads - 4 * 64 && rc == VS_OK; t -= 64) {
+            int K = 0, G = 0;
+            if (wide ? !fast_wide_fits(h, t, &K, &G) : !fast_fits(h, t, &K, &G)) continue;
+            hipError_t e = wide ? configure_fast_wide(K) : configure_fast(K);
+            float ms = 1e30f;
+            for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+                e = hipEventRecord(e0, h->stream);
+                if (e == hipSuccess)
+                    e = wide ? launch_fast_wide(K, h->d_desc, h->d_order, h->n, b, f, 0, 64, t, G, nullptr, h->stream)
+                             : launch_fast_step(K, h->d_desc, h->d_order, h->n, b, f, 0, 64, t, G, nullptr, h->stream);
+                if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float m = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
+                if (e == hipSuccess) ms = std::min(ms, m);
+            }
+            if (e != hipSuccess) rc = fail(VS_E_DEVICE, std::string("adjoint forward shape: ") + hipGetErrorString(e));
+            else if (ms < best) { best = ms; best_t = t; best_w = wide; }
         }
-        if (e != hipSuccess) rc = fail(VS_E_DEVICE, std::string("adjoint forward shape: ") + hipGetErrorString(e));
-        else if (ms < best) { best = ms; best_t = t; }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc == VS_OK) {
         h->adj_fwd_threads = best_t;
+        h->adj_fwd_wide = best_w;
         std::lock_guard<std::mutex> lk(g_adj_shape_mutex);
-        g_adj_shape_cache.push_back({key, best_t});
+        g_adj_shape_cache.push_back({key, {best_t, best_w}});
     }
     return rc;
 }
@@ -1598,15 +1606,22 @@ int adj_probe_out(vs_handle* h, const double* src, double* probe) {
     return VS_OK;
 }
 
-// Forward pass on chip: the FP32 flux-form step kernel in S-step launches,
-// the whole state copied to a checkpoint before each.
+// Forward pass on chip: the FP32 flux-form step kernel (the form and shape
+// adj_pick_forward_threads timed) in S-step launches, the whole state copied
+// to a checkpoint before each.
 int adj_forward_fast(vs_handle* h, int S, double* probe) {
     int rc;
     if ((rc = adj_cells<float>(h)) != VS_OK || (rc = adj_fast_prepare(h, S)) != VS_OK ||
         (rc = adj_pick_forward_threads(h)) != VS_OK)
         return rc;
     int FK = 0, FG = 0;
-    if (!fast_fits(h, h->adj_fwd_threads, &FK, &FG)) return fail(VS_E_INVALID, "forward shape no longer fits");
+    const bool wide = h->adj_fwd_wide != 0;
+    if (wide ? !fast_wide_fits(h, h->adj_fwd_threads, &FK, &FG) : !fast_fits(h, h->adj_fwd_threads, &FK, &FG))
+        return fail(VS_E_INVALID, "forward shape no longer fits");
+    {
+        const hipError_t e = wide ? configure_fast_wide(FK) : configure_fast(FK);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("configure: ") + hipGetErrorString(e));
+    }
     AdjBufs<float>& a = h->af;
     const size_t cb = sizeof(float) * (size_t)h->total_cells;
     VS_HIP(hipMemsetAsync(a.P, 0, cb, h->stream));
@@ -1617,8 +1632,10 @@ int adj_forward_fast(vs_handle* h, int S, double* probe) {
     for (int t0 = 0; t0 < h->max_steps; t0 += S) {
         if ((rc = adj_state_copy(h, h->adj_ck + (long long)(t0 / S) * 3 * h->total_cells, true, h->stream)) != VS_OK)
             return rc;
-        hipError_t e = launch_fast_step(FK, h->d_desc, h->d_order, h->n, b, f, t0, S, h->adj_fwd_threads, FG,
-                                        nullptr, h->stream);
+        hipError_t e = wide ? launch_fast_wide(FK, h->d_desc, h->d_order, h->n, b, f, t0, S, h->adj_fwd_threads, FG,
+                                               nullptr, h->stream)
+                            : launch_fast_step(FK, h->d_desc, h->d_order, h->n, b, f, t0, S, h->adj_fwd_threads, FG,
+                                               nullptr, h->stream);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint forward: ") + hipGetErrorString(e));
     }
     h->adj_forward = true;
