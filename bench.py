@@ -241,6 +241,17 @@ def main():
     lds_peak = max(lds_peaks.values())
     log(f"[bench] LDS read bandwidth (GB/s): " + ", ".join(f"{k} {v:.0f}" for k, v in lds_peaks.items()))
     flops = cells * BLOCK * FLOPS_PER_CELL / (mean_launch_ms * 1e-3) / 1e12 if mean_launch_ms > 0 else 0.0
+    # every work-group steps layout_k slots on each of its threads - 64 cell
+    # lanes, the count the batch's largest tube needs; the same LDS traffic
+    # counted over those slots is what each CU actually moves
+    slots = args.tubes * layout_k * (threads - 64) if layout_k else 0
+    slots_info = None
+    if slots and mean_launch_ms > 0:
+        slot_gbs = slots * BLOCK * LDS_BYTES_PER_CELL[layout] / (mean_launch_ms * 1e-3) / 1e9
+        slots_info = {"slots_per_step": slots, "real_cell_fraction": cells / slots,
+                      "achieved_gbs_over_slots": slot_gbs, "frac_over_slots": slot_gbs / lds_peak,
+                      "note": "frac above counts real cells only; all work-groups step the largest tube's "
+                              "slot count, so frac = frac_over_slots x real_cell_fraction"}
 
     # ---------------- plain FP64 single tube ----------------
     spec64 = tubes.vowel_tube(0, num_steps=33075)
@@ -379,6 +390,7 @@ def main():
                               f"{BLOCK} steps per launch; peak = LDS read bandwidth measured on this device "
                               f"(vs_measure_lds_bandwidth, widest read)",
                      "lds_peak_by_width_gbs": lds_peaks,
+                     "slots": slots_info,
                      "valu": {"achieved_tflops": flops, "peak_tflops": FP32_PEAK_TFLOPS,
                               "frac": flops / FP32_PEAK_TFLOPS,
                               "basis": f"{FLOPS_PER_CELL} algorithmic flops per cell-update"},
