@@ -36,6 +36,18 @@ int fail(int code, const std::string& msg) {
             return fail(VS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));        \
     } while (0)
 
+// A copy between host and device memory on a handle's stream, waited for:
+// ordered after the handle's earlier work and complete on return.  A plain
+// hipMemcpy runs on the null stream, which the handles' non-blocking streams
+// do not wait for, and from pageable memory it may return before the data
+// has landed; a kernel on the handle's stream could then read the buffer
+// before it is written.
+hipError_t copy_on(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+}
+
 // Excitation table (GlottalExcitations.cu:9-52) in Real.
 template <typename Real>
 void source_table(const vs_tube_spec& s, const PhysTables& ph, int num_steps, std::vector<Real>& out) {
@@ -200,7 +212,7 @@ Bufs<double>& bufs_of<double>(vs_handle* h) { return h->bd; }
 template <typename Real>
 int build_exact_classes(vs_handle* h, Bufs<Real>& b) {
     std::vector<uint8_t> cls((size_t)h->total_cells), xc((size_t)h->total_cells, 0);
-    VS_HIP(hipMemcpy(cls.data(), b.cls, cls.size(), hipMemcpyDeviceToHost));
+    VS_HIP(copy_on(h->stream, cls.data(), b.cls, cls.size(), hipMemcpyDeviceToHost));
     std::vector<int> keys((size_t)h->n * kExactClasses, 0), nk(h->n, 0);
     for (int k = 0; k < h->n; ++k) {
         const TubeDesc& d = h->desc[k];
@@ -209,9 +221,9 @@ int build_exact_classes(vs_handle* h, Bufs<Real>& b) {
                                                keys.data() + (size_t)k * kExactClasses, nk[k], err);
         if (rc != VS_OK) return fail(rc, "tube " + std::to_string(k) + ": " + err);
     }
-    VS_HIP(hipMemcpy(b.xcls, xc.data(), xc.size(), hipMemcpyHostToDevice));
-    VS_HIP(hipMemcpy(b.xkey, keys.data(), keys.size() * sizeof(int), hipMemcpyHostToDevice));
-    VS_HIP(hipMemcpy(b.xn, nk.data(), nk.size() * sizeof(int), hipMemcpyHostToDevice));
+    VS_HIP(copy_on(h->stream, b.xcls, xc.data(), xc.size(), hipMemcpyHostToDevice));
+    VS_HIP(copy_on(h->stream, b.xkey, keys.data(), keys.size() * sizeof(int), hipMemcpyHostToDevice));
+    VS_HIP(copy_on(h->stream, b.xn, nk.data(), nk.size() * sizeof(int), hipMemcpyHostToDevice));
     hipError_t e = launch_exact_classes<Real>(h->d_desc, h->n, b, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("class coefficients: ") + hipGetErrorString(e));
@@ -370,9 +382,9 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
                 box[3 * (so + i) + 2] = P.maxZ;
             }
         }
-        VS_HIP(hipMemcpy(b.poly_y, py.data(), py.size() * sizeof(float), hipMemcpyHostToDevice));
-        VS_HIP(hipMemcpy(b.poly_z, pz.data(), pz.size() * sizeof(float), hipMemcpyHostToDevice));
-        VS_HIP(hipMemcpy(b.pbox, box.data(), box.size() * sizeof(float), hipMemcpyHostToDevice));
+        VS_HIP(copy_on(st, b.poly_y, py.data(), py.size() * sizeof(float), hipMemcpyHostToDevice));
+        VS_HIP(copy_on(st, b.poly_z, pz.data(), pz.size() * sizeof(float), hipMemcpyHostToDevice));
+        VS_HIP(copy_on(st, b.pbox, box.data(), box.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     VS_HIP(hipMemcpyAsync(h->d_desc, h->desc.data(), h->n * sizeof(TubeDesc), hipMemcpyHostToDevice, st));
     VS_HIP(hipMemcpyAsync(h->d_order, order.data(), h->n * sizeof(int), hipMemcpyHostToDevice, st));
@@ -389,7 +401,7 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
     if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("setup launch: ") + hipGetErrorString(e));
     if (e2 != hipSuccess) return fail(VS_E_DEVICE, std::string("setup: ") + hipGetErrorString(e2));
     unsigned status = 0;
-    VS_HIP(hipMemcpy(&status, b.status, sizeof(unsigned), hipMemcpyDeviceToHost));
+    VS_HIP(copy_on(st, &status, b.status, sizeof(unsigned), hipMemcpyDeviceToHost));
     if (status & kStatusSqrtDomain) return fail(VS_E_GEOMETRY, "depth sqrt domain error");
     return build_exact_classes<Real>(h, b);
 }
@@ -407,15 +419,15 @@ int build_fast_classes(vs_handle* h) {
     const size_t n = (size_t)h->total_cells;
     std::vector<uint16_t> code(n);
     VS_HIP(hipStreamSynchronize(h->stream));
-    VS_HIP(hipMemcpy(code.data(), f.code, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    VS_HIP(copy_on(h->stream, code.data(), f.code, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
     std::vector<uint8_t> c8(n);
     std::vector<float> tab(5 * kFastClasses, 0.0f);
     int ncls = 0;
     std::string err;
     const int rc = enumerate_fast_classes(code.data(), n, f.lut, c8.data(), tab.data(), ncls, err);
     if (rc != VS_OK) return fail(rc, err);
-    VS_HIP(hipMemcpy(f.cls8, c8.data(), n, hipMemcpyHostToDevice));
-    VS_HIP(hipMemcpy(f.ctab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    VS_HIP(copy_on(h->stream, f.cls8, c8.data(), n, hipMemcpyHostToDevice));
+    VS_HIP(copy_on(h->stream, f.ctab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
     return VS_OK;
 }
 
@@ -818,11 +830,11 @@ int vs_field_ptr(vs_handle* h, int tube, int field, void** dptr, size_t* bytes) 
             const size_t n = nb / sizeof(double);
             std::vector<double> host(n);
             VS_HIP(hipStreamSynchronize(h->stream));
-            VS_HIP(hipMemcpy(host.data(), p64, nb, hipMemcpyDeviceToHost));
+            VS_HIP(copy_on(h->stream, host.data(), p64, nb, hipMemcpyDeviceToHost));
             std::vector<float> f(host.begin(), host.end());
             float* p = nullptr;
             if ((rc = h->alloc(&p, n)) != VS_OK) return rc;
-            VS_HIP(hipMemcpy(p, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+            VS_HIP(copy_on(h->stream, p, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
             h->f32_dev.push_back({key, p});
             *dptr = p;
             if (bytes) *bytes = n * sizeof(float);
@@ -861,7 +873,7 @@ int vs_field_ptr(vs_handle* h, int tube, int field, void** dptr, size_t* bytes) 
             if (rc != VS_OK) return rc;
             void* p = nullptr;
             if ((rc = h->alloc((char**)&p, nb)) != VS_OK) return rc;
-            VS_HIP(hipMemcpy(p, host.data(), nb, hipMemcpyHostToDevice));
+            VS_HIP(copy_on(h->stream, p, host.data(), nb, hipMemcpyHostToDevice));
             slot[tube] = p;
         }
         *dptr = slot[tube];
@@ -885,7 +897,7 @@ int vs_get_field(vs_handle* h, int tube, int field, void* dst, size_t bytes) {
         std::vector<uint8_t> cls(HW);
         VS_HIP(hipStreamSynchronize(h->stream));
         const uint8_t* src = (h->precision == 64 ? h->bd.cls : h->bf.cls) + d.cell_off;
-        VS_HIP(hipMemcpy(cls.data(), src, HW, hipMemcpyDeviceToHost));
+        VS_HIP(copy_on(h->stream, cls.data(), src, HW, hipMemcpyDeviceToHost));
         int* o = static_cast<int*>(dst);
         for (size_t i = 0; i < HW; ++i) o[i] = cls[i] & kTypeMask;
         return VS_OK;
@@ -896,13 +908,13 @@ int vs_get_field(vs_handle* h, int tube, int field, void* dst, size_t bytes) {
         std::vector<uint8_t> cls(HW);
         VS_HIP(hipStreamSynchronize(h->stream));
         const uint8_t* src = (h->precision == 64 ? h->bd.cls : h->bf.cls) + d.cell_off;
-        VS_HIP(hipMemcpy(cls.data(), src, HW, hipMemcpyDeviceToHost));
+        VS_HIP(copy_on(h->stream, cls.data(), src, HW, hipMemcpyDeviceToHost));
         std::vector<double> zc(d.W);
         if (h->precision == 64) {
-            VS_HIP(hipMemcpy(zc.data(), h->bd.zcol + d.col_off, d.W * 8, hipMemcpyDeviceToHost));
+            VS_HIP(copy_on(h->stream, zc.data(), h->bd.zcol + d.col_off, d.W * 8, hipMemcpyDeviceToHost));
         } else {
             std::vector<float> zf(d.W);
-            VS_HIP(hipMemcpy(zf.data(), h->bf.zcol + d.col_off, d.W * 4, hipMemcpyDeviceToHost));
+            VS_HIP(copy_on(h->stream, zf.data(), h->bf.zcol + d.col_off, d.W * 4, hipMemcpyDeviceToHost));
             for (int c = 0; c < d.W; ++c) zc[c] = zf[c];
         }
         for (size_t i = 0; i < HW; ++i) {
@@ -923,7 +935,7 @@ int vs_get_field(vs_handle* h, int tube, int field, void* dst, size_t bytes) {
             : launch_coeffs<float>(h->d_desc, &d, tube, h->bf, (float*)tmp, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e == hipSuccess)
-            e = hipMemcpy(dst, (char*)tmp + (size_t)(field - VS_F_COEFF0) * ni * es, ni * es, hipMemcpyDeviceToHost);
+            e = copy_on(h->stream, dst, (char*)tmp + (size_t)(field - VS_F_COEFF0) * ni * es, ni * es, hipMemcpyDeviceToHost);
         (void)hipFree(tmp);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("coefficients: ") + hipGetErrorString(e));
         return VS_OK;
@@ -934,7 +946,7 @@ int vs_get_field(vs_handle* h, int tube, int field, void* dst, size_t bytes) {
     if (rc != VS_OK) return rc;
     if (bytes < count * esz) return fail(VS_E_RANGE, "destination too small");
     VS_HIP(hipStreamSynchronize(h->stream));
-    VS_HIP(hipMemcpy(dst, p, count * esz, hipMemcpyDeviceToHost));
+    VS_HIP(copy_on(h->stream, dst, p, count * esz, hipMemcpyDeviceToHost));
     return VS_OK;
 }
 
@@ -950,10 +962,10 @@ int vs_sync(vs_handle* h) {
     if (h->d_bar) {
         unsigned* st = h->precision == 64 ? h->bd.status : h->bf.status;
         unsigned status = 0;
-        VS_HIP(hipMemcpy(&status, st, sizeof(unsigned), hipMemcpyDeviceToHost));
+        VS_HIP(copy_on(h->stream, &status, st, sizeof(unsigned), hipMemcpyDeviceToHost));
         if (status & kStatusBarrierTimeout) {
             const unsigned clear = status & ~kStatusBarrierTimeout;
-            VS_HIP(hipMemcpy(st, &clear, sizeof(unsigned), hipMemcpyHostToDevice));
+            VS_HIP(copy_on(h->stream, st, &clear, sizeof(unsigned), hipMemcpyHostToDevice));
             return fail(VS_E_DEVICE, "cooperative exact step: a grid barrier timed out (results invalid)");
         }
     }
@@ -995,7 +1007,7 @@ int vs_get_probe(vs_handle* h, int tube, double* dst, size_t n) {
     const size_t m = std::min<size_t>(n, (size_t)d.num_steps);
     double* p = (h->precision == 64 ? h->bd.probe : h->bf.probe) + d.src_off;
     VS_HIP(hipStreamSynchronize(h->stream));
-    VS_HIP(hipMemcpy(dst, p, m * sizeof(double), hipMemcpyDeviceToHost));
+    VS_HIP(copy_on(h->stream, dst, p, m * sizeof(double), hipMemcpyDeviceToHost));
     return VS_OK;
 }
 
@@ -1006,7 +1018,7 @@ int vs_get_source_trace(vs_handle* h, int tube, double* dst, size_t n) {
     const size_t m = std::min<size_t>(n, (size_t)d.num_steps);
     double* p = (h->precision == 64 ? h->bd.strace : h->bf.strace) + d.src_off;
     VS_HIP(hipStreamSynchronize(h->stream));
-    VS_HIP(hipMemcpy(dst, p, m * sizeof(double), hipMemcpyDeviceToHost));
+    VS_HIP(copy_on(h->stream, dst, p, m * sizeof(double), hipMemcpyDeviceToHost));
     return VS_OK;
 }
 
@@ -1734,9 +1746,9 @@ int adj_forward(vs_handle* h, int S, double* probe, size_t len) {
 }
 
 template <typename T>
-int fetch(std::vector<T>& dst, const T* src, long long n) {
+int fetch(hipStream_t st, std::vector<T>& dst, const T* src, long long n) {
     dst.resize((size_t)n);
-    VS_HIP(hipMemcpy(dst.data(), src, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost));
+    VS_HIP(copy_on(st, dst.data(), src, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost));
     return VS_OK;
 }
 
@@ -1781,7 +1793,7 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
     hipError_t e = launch_depth_raw<Real>(h->d_desc, h->desc.data(), h->n, b, raw, raw + maxHW * h->n, maxHW, h->stream);
     hipError_t e2 = hipStreamSynchronize(h->stream);
     std::vector<Real> rawh;
-    if (e == hipSuccess && e2 == hipSuccess) rc = fetch(rawh, raw, maxHW * h->n * 2);
+    if (e == hipSuccess && e2 == hipSuccess) rc = fetch(h->stream, rawh, raw, maxHW * h->n * 2);
     (void)hipFree(raw);
     if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("raw depth: ") + hipGetErrorString(e));
     if (e2 != hipSuccess) return fail(VS_E_DEVICE, std::string("reverse pass: ") + hipGetErrorString(e2));
@@ -1793,22 +1805,22 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
     std::vector<uint8_t> cls;
     std::vector<int> seg, upper, lower;
     const long long C = h->total_cells;
-    if ((rc = fetch(Lp, a.GkP, C)) || (rc = fetch(Lx, a.Gx, C)) || (rc = fetch(Ly, a.Gy, C))) return rc;
+    if ((rc = fetch(h->stream, Lp, a.GkP, C)) || (rc = fetch(h->stream, Lx, a.Gx, C)) || (rc = fetch(h->stream, Ly, a.Gy, C))) return rc;
     if (!h->adj_onchip) {
         // the per-step kernels accumulate d J / d (kP, cx, cy)
         std::vector<Real> kP, cx, cy;
-        if ((rc = fetch(kP, a.kP, C)) || (rc = fetch(cx, a.cx, C)) || (rc = fetch(cy, a.cy, C))) return rc;
+        if ((rc = fetch(h->stream, kP, a.kP, C)) || (rc = fetch(h->stream, cx, a.cx, C)) || (rc = fetch(h->stream, cy, a.cy, C))) return rc;
         for (long long q = 0; q < C; ++q) {
             Lp[q] *= (double)kP[q];
             Lx[q] *= (double)cx[q];
             Ly[q] *= (double)cy[q];
         }
     }
-    if ((rc = fetch(dP, b.dP, C)) || (rc = fetch(dX, b.dX, C)) || (rc = fetch(dY, b.dY, C)) ||
-        (rc = fetch(cls, b.cls, C)) || (rc = fetch(zcol, b.zcol, h->total_cols)) ||
-        (rc = fetch(seg, b.seg, h->total_walls)) || (rc = fetch(upper, b.upper, h->total_walls)) ||
-        (rc = fetch(lower, b.lower, h->total_walls)) || (rc = fetch(minor, b.minor, h->total_secs)) ||
-        (rc = fetch(dscale, b.dscale, h->total_secs)))
+    if ((rc = fetch(h->stream, dP, b.dP, C)) || (rc = fetch(h->stream, dX, b.dX, C)) || (rc = fetch(h->stream, dY, b.dY, C)) ||
+        (rc = fetch(h->stream, cls, b.cls, C)) || (rc = fetch(h->stream, zcol, b.zcol, h->total_cols)) ||
+        (rc = fetch(h->stream, seg, b.seg, h->total_walls)) || (rc = fetch(h->stream, upper, b.upper, h->total_walls)) ||
+        (rc = fetch(h->stream, lower, b.lower, h->total_walls)) || (rc = fetch(h->stream, minor, b.minor, h->total_secs)) ||
+        (rc = fetch(h->stream, dscale, b.dscale, h->total_secs)))
         return rc;
     // tubes are independent: each reads the shared host copies and writes
     // only its own sections, so they are chained on up to 16 host threads
@@ -1988,7 +2000,7 @@ int vs_probe_audio(vs_handle* h, int tube, float* out, size_t cap, size_t* n_out
     const double* probe = h->precision == 64 ? h->bd.probe : h->bf.probe;
     if (e == hipSuccess) e = launch_audio(probe, dj, (int)jobs.size(), dout, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)total * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = copy_on(h->stream, out, dout, (size_t)total * sizeof(float), hipMemcpyDeviceToHost);
     (void)hipFree(dj);
     if (dout) (void)hipFree(dout);
     if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("audio: ") + hipGetErrorString(e));
