@@ -46,7 +46,8 @@ enum vs_status {
     VS_E_DEVICE = -3,      /* HIP runtime error */
     VS_E_RANGE = -4,       /* index / size out of range */
     VS_E_STATE = -5,       /* call not valid in the current state */
-    VS_E_NUMERIC = -6      /* non-finite value detected on the device */
+    VS_E_NUMERIC = -6,     /* non-finite value detected on the device */
+    VS_E_HOST_MEMORY = -7  /* host memory allocation failed */
 };
 
 enum vs_precision { VS_FP32 = 32, VS_FP64 = 64 };

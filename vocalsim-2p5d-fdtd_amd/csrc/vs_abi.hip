@@ -48,6 +48,22 @@ hipError_t copy_on(hipStream_t st, void* dst, const void* src, size_t bytes, hip
     return e;
 }
 
+// Runs the body of an entry point.  A C++ exception (host memory exhausted,
+// or a size no standard container can hold) becomes a status code instead of
+// crossing the C ABI.
+template <typename F>
+int guarded(const char* name, F&& body) noexcept {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(VS_E_HOST_MEMORY, std::string(name) + ": host memory allocation failed");
+    } catch (const std::exception& e) {
+        return fail(VS_E_STATE, std::string(name) + ": " + e.what());
+    } catch (...) {
+        return fail(VS_E_STATE, std::string(name) + ": unexpected exception");
+    }
+}
+
 // Excitation table (GlottalExcitations.cu:9-52) in Real.
 template <typename Real>
 void source_table(const vs_tube_spec& s, const PhysTables& ph, int num_steps, std::vector<Real>& out) {
@@ -763,354 +779,386 @@ const char* vs_last_error(void) { return g_last_error.c_str(); }
 int vs_abi_version(void) { return VS_ABI_VERSION; }
 
 int vs_create(const vs_tube_spec* specs, int n_tubes, int precision, vs_handle** out) {
-    g_last_error.clear();
-    if (!specs || !out || n_tubes <= 0) return fail(VS_E_INVALID, "vs_create: need specs, n_tubes > 0 and an output handle");
-    if (precision != VS_FP32 && precision != VS_FP64) return fail(VS_E_INVALID, "precision must be 32 or 64");
-    *out = nullptr;
-    std::unique_ptr<vs_handle> h(new vs_handle());
-    h->n = n_tubes;
-    h->precision = precision;
-    VS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    VS_HIP(hipEventCreate(&h->ev0));
-    VS_HIP(hipEventCreate(&h->ev1));
-    {
-        int dev = 0;
-        VS_HIP(hipGetDevice(&dev));
-        hipDeviceProp_t prop;
-        VS_HIP(hipGetDeviceProperties(&prop, dev));
-        h->lds_limit = (int)std::max(prop.sharedMemPerBlockOptin, prop.sharedMemPerBlock);
-    }
-    const int rc = precision == VS_FP64 ? build<double>(h.get(), specs) : build<float>(h.get(), specs);
-    if (rc != VS_OK) return rc;
-    *out = h.release();
-    return VS_OK;
+    return guarded("vs_create", [&]() -> int {
+        g_last_error.clear();
+        if (!specs || !out || n_tubes <= 0) return fail(VS_E_INVALID, "vs_create: need specs, n_tubes > 0 and an output handle");
+        if (precision != VS_FP32 && precision != VS_FP64) return fail(VS_E_INVALID, "precision must be 32 or 64");
+        *out = nullptr;
+        std::unique_ptr<vs_handle> h(new vs_handle());
+        h->n = n_tubes;
+        h->precision = precision;
+        VS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        VS_HIP(hipEventCreate(&h->ev0));
+        VS_HIP(hipEventCreate(&h->ev1));
+        {
+            int dev = 0;
+            VS_HIP(hipGetDevice(&dev));
+            hipDeviceProp_t prop;
+            VS_HIP(hipGetDeviceProperties(&prop, dev));
+            h->lds_limit = (int)std::max(prop.sharedMemPerBlockOptin, prop.sharedMemPerBlock);
+        }
+        const int rc = precision == VS_FP64 ? build<double>(h.get(), specs) : build<float>(h.get(), specs);
+        if (rc != VS_OK) return rc;
+        *out = h.release();
+        return VS_OK;
+    });
 }
 
 int vs_destroy(vs_handle* h) {
-    if (!h) return VS_OK;
-    (void)hipStreamSynchronize(h->stream);
-    delete h;
-    return VS_OK;
+    return guarded("vs_destroy", [&]() -> int {
+        if (!h) return VS_OK;
+        (void)hipStreamSynchronize(h->stream);
+        delete h;
+        return VS_OK;
+    });
 }
 
 int vs_num_tubes(const vs_handle* h, int* n) {
-    if (check_handle(h) || !n) return fail(VS_E_INVALID, "vs_num_tubes: bad argument");
-    *n = h->n;
-    return VS_OK;
+    return guarded("vs_num_tubes", [&]() -> int {
+        if (check_handle(h) || !n) return fail(VS_E_INVALID, "vs_num_tubes: bad argument");
+        *n = h->n;
+        return VS_OK;
+    });
 }
 
 int vs_get_dims(const vs_handle* h, int tube, vs_dims* out) {
-    if (check_handle(h) || !out) return fail(VS_E_INVALID, "vs_get_dims: bad argument");
-    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
-    *out = h->dims[tube];
-    return VS_OK;
+    return guarded("vs_get_dims", [&]() -> int {
+        if (check_handle(h) || !out) return fail(VS_E_INVALID, "vs_get_dims: bad argument");
+        if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+        *out = h->dims[tube];
+        return VS_OK;
+    });
 }
 
 int vs_field_ptr(vs_handle* h, int tube, int field, void** dptr, size_t* bytes) {
-    if (check_handle(h) || !dptr) return fail(VS_E_INVALID, "vs_field_ptr: bad argument");
-    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
-    if (field & VS_F_AS_F32) {
-        field &= ~VS_F_AS_F32;
-        const bool real_field = field == VS_F_DEPTH_P || field == VS_F_DEPTH_X || field == VS_F_DEPTH_Y ||
-                                field == VS_F_Z_INV || field == VS_F_SOURCE || field == VS_F_PRESSURE ||
-                                field == VS_F_VX || field == VS_F_VY ||
-                                (field >= VS_F_COEFF0 && field < VS_F_COEFF0 + 17);
-        if (!real_field) return fail(VS_E_INVALID, "VS_F_AS_F32 applies to floating-point fields only");
-        if (h->precision == 64) {
-            const long long key = ((long long)tube << 32) | field;
-            for (auto& kv : h->f32_dev)
-                if (kv.first == key) {
-                    *dptr = kv.second;
-                    return VS_OK;
-                }
-            void* p64 = nullptr;
-            size_t nb = 0;
-            int rc = vs_field_ptr(h, tube, field, &p64, &nb);
-            if (rc != VS_OK) return rc;
-            const size_t n = nb / sizeof(double);
-            std::vector<double> host(n);
-            VS_HIP(hipStreamSynchronize(h->stream));
-            VS_HIP(copy_on(h->stream, host.data(), p64, nb, hipMemcpyDeviceToHost));
-            std::vector<float> f(host.begin(), host.end());
-            float* p = nullptr;
-            if ((rc = h->alloc(&p, n)) != VS_OK) return rc;
-            VS_HIP(copy_on(h->stream, p, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
-            h->f32_dev.push_back({key, p});
-            *dptr = p;
-            if (bytes) *bytes = n * sizeof(float);
+    return guarded("vs_field_ptr", [&]() -> int {
+        if (check_handle(h) || !dptr) return fail(VS_E_INVALID, "vs_field_ptr: bad argument");
+        if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+        if (field & VS_F_AS_F32) {
+            field &= ~VS_F_AS_F32;
+            const bool real_field = field == VS_F_DEPTH_P || field == VS_F_DEPTH_X || field == VS_F_DEPTH_Y ||
+                                    field == VS_F_Z_INV || field == VS_F_SOURCE || field == VS_F_PRESSURE ||
+                                    field == VS_F_VX || field == VS_F_VY ||
+                                    (field >= VS_F_COEFF0 && field < VS_F_COEFF0 + 17);
+            if (!real_field) return fail(VS_E_INVALID, "VS_F_AS_F32 applies to floating-point fields only");
+            if (h->precision == 64) {
+                const long long key = ((long long)tube << 32) | field;
+                for (auto& kv : h->f32_dev)
+                    if (kv.first == key) {
+                        *dptr = kv.second;
+                        return VS_OK;
+                    }
+                void* p64 = nullptr;
+                size_t nb = 0;
+                int rc = vs_field_ptr(h, tube, field, &p64, &nb);
+                if (rc != VS_OK) return rc;
+                const size_t n = nb / sizeof(double);
+                std::vector<double> host(n);
+                VS_HIP(hipStreamSynchronize(h->stream));
+                VS_HIP(copy_on(h->stream, host.data(), p64, nb, hipMemcpyDeviceToHost));
+                std::vector<float> f(host.begin(), host.end());
+                float* p = nullptr;
+                if ((rc = h->alloc(&p, n)) != VS_OK) return rc;
+                VS_HIP(copy_on(h->stream, p, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+                h->f32_dev.push_back({key, p});
+                *dptr = p;
+                if (bytes) *bytes = n * sizeof(float);
+                return VS_OK;
+            }
+        }
+        const TubeDesc& d = h->desc[tube];
+        const size_t es = elem_size(h), HW = (size_t)d.H * d.W, ni = (size_t)(d.H - 2) * (d.W - 2);
+        if (h->coeff_dev.empty()) {
+            h->coeff_dev.assign(h->n, nullptr);
+            h->types_dev.assign(h->n, nullptr);
+            h->zinv_dev.assign(h->n, nullptr);
+        }
+        if (field >= VS_F_COEFF0 && field < VS_F_COEFF0 + 17) {
+            if (!h->coeff_dev[tube]) {
+                void* p = nullptr;
+                int rc = h->alloc((char**)&p, ni * 17 * es);
+                if (rc != VS_OK) return rc;
+                hipError_t e = h->precision == 64
+                    ? launch_coeffs<double>(h->d_desc, &d, tube, h->bd, (double*)p, h->stream)
+                    : launch_coeffs<float>(h->d_desc, &d, tube, h->bf, (float*)p, h->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+                if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("coefficients: ") + hipGetErrorString(e));
+                h->coeff_dev[tube] = p;
+            }
+            *dptr = (char*)h->coeff_dev[tube] + (size_t)(field - VS_F_COEFF0) * ni * es;
+            if (bytes) *bytes = ni * es;
             return VS_OK;
         }
-    }
-    const TubeDesc& d = h->desc[tube];
-    const size_t es = elem_size(h), HW = (size_t)d.H * d.W, ni = (size_t)(d.H - 2) * (d.W - 2);
-    if (h->coeff_dev.empty()) {
-        h->coeff_dev.assign(h->n, nullptr);
-        h->types_dev.assign(h->n, nullptr);
-        h->zinv_dev.assign(h->n, nullptr);
-    }
-    if (field >= VS_F_COEFF0 && field < VS_F_COEFF0 + 17) {
-        if (!h->coeff_dev[tube]) {
-            void* p = nullptr;
-            int rc = h->alloc((char**)&p, ni * 17 * es);
-            if (rc != VS_OK) return rc;
-            hipError_t e = h->precision == 64
-                ? launch_coeffs<double>(h->d_desc, &d, tube, h->bd, (double*)p, h->stream)
-                : launch_coeffs<float>(h->d_desc, &d, tube, h->bf, (float*)p, h->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-            if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("coefficients: ") + hipGetErrorString(e));
-            h->coeff_dev[tube] = p;
+        if (field == VS_F_CELL_TYPE || field == VS_F_Z_INV) {
+            std::vector<void*>& slot = field == VS_F_CELL_TYPE ? h->types_dev : h->zinv_dev;
+            const size_t nb = HW * (field == VS_F_CELL_TYPE ? sizeof(int) : es);
+            if (!slot[tube]) {
+                std::vector<char> host(nb);
+                int rc = vs_get_field(h, tube, field, host.data(), nb);
+                if (rc != VS_OK) return rc;
+                void* p = nullptr;
+                if ((rc = h->alloc((char**)&p, nb)) != VS_OK) return rc;
+                VS_HIP(copy_on(h->stream, p, host.data(), nb, hipMemcpyHostToDevice));
+                slot[tube] = p;
+            }
+            *dptr = slot[tube];
+            if (bytes) *bytes = nb;
+            return VS_OK;
         }
-        *dptr = (char*)h->coeff_dev[tube] + (size_t)(field - VS_F_COEFF0) * ni * es;
-        if (bytes) *bytes = ni * es;
+        size_t count = 0, esz = 0;
+        int rc = locate(h, tube, field, dptr, &count, &esz);
+        if (rc != VS_OK) return rc;
+        if (bytes) *bytes = count * esz;
         return VS_OK;
-    }
-    if (field == VS_F_CELL_TYPE || field == VS_F_Z_INV) {
-        std::vector<void*>& slot = field == VS_F_CELL_TYPE ? h->types_dev : h->zinv_dev;
-        const size_t nb = HW * (field == VS_F_CELL_TYPE ? sizeof(int) : es);
-        if (!slot[tube]) {
-            std::vector<char> host(nb);
-            int rc = vs_get_field(h, tube, field, host.data(), nb);
-            if (rc != VS_OK) return rc;
-            void* p = nullptr;
-            if ((rc = h->alloc((char**)&p, nb)) != VS_OK) return rc;
-            VS_HIP(copy_on(h->stream, p, host.data(), nb, hipMemcpyHostToDevice));
-            slot[tube] = p;
-        }
-        *dptr = slot[tube];
-        if (bytes) *bytes = nb;
-        return VS_OK;
-    }
-    size_t count = 0, esz = 0;
-    int rc = locate(h, tube, field, dptr, &count, &esz);
-    if (rc != VS_OK) return rc;
-    if (bytes) *bytes = count * esz;
-    return VS_OK;
+    });
 }
 
 int vs_get_field(vs_handle* h, int tube, int field, void* dst, size_t bytes) {
-    if (check_handle(h) || !dst) return fail(VS_E_INVALID, "vs_get_field: bad argument");
-    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
-    const TubeDesc& d = h->desc[tube];
-    const size_t HW = (size_t)d.H * d.W;
-    if (field == VS_F_CELL_TYPE) {
-        if (bytes < HW * sizeof(int)) return fail(VS_E_RANGE, "destination too small");
-        std::vector<uint8_t> cls(HW);
-        VS_HIP(hipStreamSynchronize(h->stream));
-        const uint8_t* src = (h->precision == 64 ? h->bd.cls : h->bf.cls) + d.cell_off;
-        VS_HIP(copy_on(h->stream, cls.data(), src, HW, hipMemcpyDeviceToHost));
-        int* o = static_cast<int*>(dst);
-        for (size_t i = 0; i < HW; ++i) o[i] = cls[i] & kTypeMask;
-        return VS_OK;
-    }
-    if (field == VS_F_Z_INV) {
-        const size_t es = elem_size(h);
-        if (bytes < HW * es) return fail(VS_E_RANGE, "destination too small");
-        std::vector<uint8_t> cls(HW);
-        VS_HIP(hipStreamSynchronize(h->stream));
-        const uint8_t* src = (h->precision == 64 ? h->bd.cls : h->bf.cls) + d.cell_off;
-        VS_HIP(copy_on(h->stream, cls.data(), src, HW, hipMemcpyDeviceToHost));
-        std::vector<double> zc(d.W);
-        if (h->precision == 64) {
-            VS_HIP(copy_on(h->stream, zc.data(), h->bd.zcol + d.col_off, d.W * 8, hipMemcpyDeviceToHost));
-        } else {
-            std::vector<float> zf(d.W);
-            VS_HIP(copy_on(h->stream, zf.data(), h->bf.zcol + d.col_off, d.W * 4, hipMemcpyDeviceToHost));
-            for (int c = 0; c < d.W; ++c) zc[c] = zf[c];
+    return guarded("vs_get_field", [&]() -> int {
+        if (check_handle(h) || !dst) return fail(VS_E_INVALID, "vs_get_field: bad argument");
+        if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+        const TubeDesc& d = h->desc[tube];
+        const size_t HW = (size_t)d.H * d.W;
+        if (field == VS_F_CELL_TYPE) {
+            if (bytes < HW * sizeof(int)) return fail(VS_E_RANGE, "destination too small");
+            std::vector<uint8_t> cls(HW);
+            VS_HIP(hipStreamSynchronize(h->stream));
+            const uint8_t* src = (h->precision == 64 ? h->bd.cls : h->bf.cls) + d.cell_off;
+            VS_HIP(copy_on(h->stream, cls.data(), src, HW, hipMemcpyDeviceToHost));
+            int* o = static_cast<int*>(dst);
+            for (size_t i = 0; i < HW; ++i) o[i] = cls[i] & kTypeMask;
+            return VS_OK;
         }
-        for (size_t i = 0; i < HW; ++i) {
-            const double z = (cls[i] & kSegWallBit) ? zc[i % d.W] : 0.0;
-            if (h->precision == 64) static_cast<double*>(dst)[i] = z;
-            else static_cast<float*>(dst)[i] = (float)z;
+        if (field == VS_F_Z_INV) {
+            const size_t es = elem_size(h);
+            if (bytes < HW * es) return fail(VS_E_RANGE, "destination too small");
+            std::vector<uint8_t> cls(HW);
+            VS_HIP(hipStreamSynchronize(h->stream));
+            const uint8_t* src = (h->precision == 64 ? h->bd.cls : h->bf.cls) + d.cell_off;
+            VS_HIP(copy_on(h->stream, cls.data(), src, HW, hipMemcpyDeviceToHost));
+            std::vector<double> zc(d.W);
+            if (h->precision == 64) {
+                VS_HIP(copy_on(h->stream, zc.data(), h->bd.zcol + d.col_off, d.W * 8, hipMemcpyDeviceToHost));
+            } else {
+                std::vector<float> zf(d.W);
+                VS_HIP(copy_on(h->stream, zf.data(), h->bf.zcol + d.col_off, d.W * 4, hipMemcpyDeviceToHost));
+                for (int c = 0; c < d.W; ++c) zc[c] = zf[c];
+            }
+            for (size_t i = 0; i < HW; ++i) {
+                const double z = (cls[i] & kSegWallBit) ? zc[i % d.W] : 0.0;
+                if (h->precision == 64) static_cast<double*>(dst)[i] = z;
+                else static_cast<float*>(dst)[i] = (float)z;
+            }
+            return VS_OK;
         }
+        if (field >= VS_F_COEFF0 && field < VS_F_COEFF0 + 17) {
+            const size_t ni = (size_t)(d.H - 2) * (d.W - 2);
+            const size_t es = elem_size(h);
+            if (bytes < ni * es) return fail(VS_E_RANGE, "destination too small");
+            void* tmp = nullptr;
+            VS_HIP(hipMalloc(&tmp, ni * 17 * es));
+            hipError_t e = h->precision == 64
+                ? launch_coeffs<double>(h->d_desc, &d, tube, h->bd, (double*)tmp, h->stream)
+                : launch_coeffs<float>(h->d_desc, &d, tube, h->bf, (float*)tmp, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            if (e == hipSuccess)
+                e = copy_on(h->stream, dst, (char*)tmp + (size_t)(field - VS_F_COEFF0) * ni * es, ni * es, hipMemcpyDeviceToHost);
+            (void)hipFree(tmp);
+            if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("coefficients: ") + hipGetErrorString(e));
+            return VS_OK;
+        }
+        void* p = nullptr;
+        size_t count = 0, esz = 0;
+        int rc = locate(h, tube, field, &p, &count, &esz);
+        if (rc != VS_OK) return rc;
+        if (bytes < count * esz) return fail(VS_E_RANGE, "destination too small");
+        VS_HIP(hipStreamSynchronize(h->stream));
+        VS_HIP(copy_on(h->stream, dst, p, count * esz, hipMemcpyDeviceToHost));
         return VS_OK;
-    }
-    if (field >= VS_F_COEFF0 && field < VS_F_COEFF0 + 17) {
-        const size_t ni = (size_t)(d.H - 2) * (d.W - 2);
-        const size_t es = elem_size(h);
-        if (bytes < ni * es) return fail(VS_E_RANGE, "destination too small");
-        void* tmp = nullptr;
-        VS_HIP(hipMalloc(&tmp, ni * 17 * es));
-        hipError_t e = h->precision == 64
-            ? launch_coeffs<double>(h->d_desc, &d, tube, h->bd, (double*)tmp, h->stream)
-            : launch_coeffs<float>(h->d_desc, &d, tube, h->bf, (float*)tmp, h->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-        if (e == hipSuccess)
-            e = copy_on(h->stream, dst, (char*)tmp + (size_t)(field - VS_F_COEFF0) * ni * es, ni * es, hipMemcpyDeviceToHost);
-        (void)hipFree(tmp);
-        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("coefficients: ") + hipGetErrorString(e));
-        return VS_OK;
-    }
-    void* p = nullptr;
-    size_t count = 0, esz = 0;
-    int rc = locate(h, tube, field, &p, &count, &esz);
-    if (rc != VS_OK) return rc;
-    if (bytes < count * esz) return fail(VS_E_RANGE, "destination too small");
-    VS_HIP(hipStreamSynchronize(h->stream));
-    VS_HIP(copy_on(h->stream, dst, p, count * esz, hipMemcpyDeviceToHost));
-    return VS_OK;
+    });
 }
 
 int vs_run(vs_handle* h, int n_steps) {
-    if (check_handle(h)) return VS_E_INVALID;
-    if (n_steps < 0) return fail(VS_E_INVALID, "n_steps must be >= 0");
-    return h->precision == 64 ? run_steps<double>(h, n_steps) : run_steps<float>(h, n_steps);
+    return guarded("vs_run", [&]() -> int {
+        if (check_handle(h)) return VS_E_INVALID;
+        if (n_steps < 0) return fail(VS_E_INVALID, "n_steps must be >= 0");
+        return h->precision == 64 ? run_steps<double>(h, n_steps) : run_steps<float>(h, n_steps);
+    });
 }
 
 int vs_sync(vs_handle* h) {
-    if (check_handle(h)) return VS_E_INVALID;
-    VS_HIP(hipStreamSynchronize(h->stream));
-    if (h->d_bar) {
-        unsigned* st = h->precision == 64 ? h->bd.status : h->bf.status;
-        unsigned status = 0;
-        VS_HIP(copy_on(h->stream, &status, st, sizeof(unsigned), hipMemcpyDeviceToHost));
-        if (status & kStatusBarrierTimeout) {
-            const unsigned clear = status & ~kStatusBarrierTimeout;
-            VS_HIP(copy_on(h->stream, st, &clear, sizeof(unsigned), hipMemcpyHostToDevice));
-            return fail(VS_E_DEVICE, "cooperative exact step: a grid barrier timed out (results invalid)");
+    return guarded("vs_sync", [&]() -> int {
+        if (check_handle(h)) return VS_E_INVALID;
+        VS_HIP(hipStreamSynchronize(h->stream));
+        if (h->d_bar) {
+            unsigned* st = h->precision == 64 ? h->bd.status : h->bf.status;
+            unsigned status = 0;
+            VS_HIP(copy_on(h->stream, &status, st, sizeof(unsigned), hipMemcpyDeviceToHost));
+            if (status & kStatusBarrierTimeout) {
+                const unsigned clear = status & ~kStatusBarrierTimeout;
+                VS_HIP(copy_on(h->stream, st, &clear, sizeof(unsigned), hipMemcpyHostToDevice));
+                return fail(VS_E_DEVICE, "cooperative exact step: a grid barrier timed out (results invalid)");
+            }
         }
-    }
-    float ms = 0;
-    if (h->ev_recorded) VS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    h->last_ms = ms;
-    return VS_OK;
+        float ms = 0;
+        if (h->ev_recorded) VS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_ms = ms;
+        return VS_OK;
+    });
 }
 
 int vs_steps_done(const vs_handle* h, int* steps) {
-    if (check_handle(h) || !steps) return fail(VS_E_INVALID, "vs_steps_done: bad argument");
-    *steps = h->steps_done;
-    return VS_OK;
+    return guarded("vs_steps_done", [&]() -> int {
+        if (check_handle(h) || !steps) return fail(VS_E_INVALID, "vs_steps_done: bad argument");
+        *steps = h->steps_done;
+        return VS_OK;
+    });
 }
 
 int vs_reset(vs_handle* h) {
-    if (check_handle(h)) return VS_E_INVALID;
-    const size_t es = elem_size(h);
-    void* P = h->precision == 64 ? (void*)h->bd.P : (void*)h->bf.P;
-    void* Vx = h->precision == 64 ? (void*)h->bd.Vx : (void*)h->bf.Vx;
-    void* Vy = h->precision == 64 ? (void*)h->bd.Vy : (void*)h->bf.Vy;
-    VS_HIP(hipMemsetAsync(P, 0, h->total_cells * es, h->stream));
-    VS_HIP(hipMemsetAsync(Vx, 0, h->total_cells * es, h->stream));
-    VS_HIP(hipMemsetAsync(Vy, 0, h->total_cells * es, h->stream));
-    double* pexc = h->precision == 64 ? h->bd.pexc : h->bf.pexc;
-    VS_HIP(hipMemsetAsync(pexc, 0, h->n * sizeof(double), h->stream));
-    h->flux_valid = false;
-    TwoMassState* tm = h->precision == 64 ? h->bd.tm : h->bf.tm;
-    VS_HIP(hipMemcpyAsync(tm, h->tm0.data(), h->n * sizeof(TwoMassState), hipMemcpyHostToDevice, h->stream));
-    VS_HIP(hipStreamSynchronize(h->stream));
-    h->steps_done = 0;
-    return VS_OK;
+    return guarded("vs_reset", [&]() -> int {
+        if (check_handle(h)) return VS_E_INVALID;
+        const size_t es = elem_size(h);
+        void* P = h->precision == 64 ? (void*)h->bd.P : (void*)h->bf.P;
+        void* Vx = h->precision == 64 ? (void*)h->bd.Vx : (void*)h->bf.Vx;
+        void* Vy = h->precision == 64 ? (void*)h->bd.Vy : (void*)h->bf.Vy;
+        VS_HIP(hipMemsetAsync(P, 0, h->total_cells * es, h->stream));
+        VS_HIP(hipMemsetAsync(Vx, 0, h->total_cells * es, h->stream));
+        VS_HIP(hipMemsetAsync(Vy, 0, h->total_cells * es, h->stream));
+        double* pexc = h->precision == 64 ? h->bd.pexc : h->bf.pexc;
+        VS_HIP(hipMemsetAsync(pexc, 0, h->n * sizeof(double), h->stream));
+        h->flux_valid = false;
+        TwoMassState* tm = h->precision == 64 ? h->bd.tm : h->bf.tm;
+        VS_HIP(hipMemcpyAsync(tm, h->tm0.data(), h->n * sizeof(TwoMassState), hipMemcpyHostToDevice, h->stream));
+        VS_HIP(hipStreamSynchronize(h->stream));
+        h->steps_done = 0;
+        return VS_OK;
+    });
 }
 
 int vs_get_probe(vs_handle* h, int tube, double* dst, size_t n) {
-    if (check_handle(h) || !dst) return fail(VS_E_INVALID, "vs_get_probe: bad argument");
-    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
-    const TubeDesc& d = h->desc[tube];
-    const size_t m = std::min<size_t>(n, (size_t)d.num_steps);
-    double* p = (h->precision == 64 ? h->bd.probe : h->bf.probe) + d.src_off;
-    VS_HIP(hipStreamSynchronize(h->stream));
-    VS_HIP(copy_on(h->stream, dst, p, m * sizeof(double), hipMemcpyDeviceToHost));
-    return VS_OK;
+    return guarded("vs_get_probe", [&]() -> int {
+        if (check_handle(h) || !dst) return fail(VS_E_INVALID, "vs_get_probe: bad argument");
+        if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+        const TubeDesc& d = h->desc[tube];
+        const size_t m = std::min<size_t>(n, (size_t)d.num_steps);
+        double* p = (h->precision == 64 ? h->bd.probe : h->bf.probe) + d.src_off;
+        VS_HIP(hipStreamSynchronize(h->stream));
+        VS_HIP(copy_on(h->stream, dst, p, m * sizeof(double), hipMemcpyDeviceToHost));
+        return VS_OK;
+    });
 }
 
 int vs_get_source_trace(vs_handle* h, int tube, double* dst, size_t n) {
-    if (check_handle(h) || !dst) return fail(VS_E_INVALID, "vs_get_source_trace: bad argument");
-    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
-    const TubeDesc& d = h->desc[tube];
-    const size_t m = std::min<size_t>(n, (size_t)d.num_steps);
-    double* p = (h->precision == 64 ? h->bd.strace : h->bf.strace) + d.src_off;
-    VS_HIP(hipStreamSynchronize(h->stream));
-    VS_HIP(copy_on(h->stream, dst, p, m * sizeof(double), hipMemcpyDeviceToHost));
-    return VS_OK;
+    return guarded("vs_get_source_trace", [&]() -> int {
+        if (check_handle(h) || !dst) return fail(VS_E_INVALID, "vs_get_source_trace: bad argument");
+        if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+        const TubeDesc& d = h->desc[tube];
+        const size_t m = std::min<size_t>(n, (size_t)d.num_steps);
+        double* p = (h->precision == 64 ? h->bd.strace : h->bf.strace) + d.src_off;
+        VS_HIP(hipStreamSynchronize(h->stream));
+        VS_HIP(copy_on(h->stream, dst, p, m * sizeof(double), hipMemcpyDeviceToHost));
+        return VS_OK;
+    });
 }
 
 int vs_set_option(vs_handle* h, int option, int value) {
-    if (check_handle(h)) return VS_E_INVALID;
-    switch (option) {
-    case VS_OPT_STEPS_PER_LAUNCH:
-        if (value < 1) return fail(VS_E_INVALID, "steps per launch must be >= 1");
-        h->steps_per_launch = value; return VS_OK;
-    case VS_OPT_THREADS:
-        if (value < 64 || value > 1024 || value % 64) return fail(VS_E_INVALID, "threads must be a multiple of 64 in [64, 1024]");
-        h->threads = value; return VS_OK;
-    case VS_OPT_USE_GRAPH: h->use_graph = value ? 1 : 0; return VS_OK;
-    case VS_OPT_KERNEL:
-        if (value != VS_K_EXACT && value != VS_K_FAST) return fail(VS_E_INVALID, "unknown kernel family");
-        if (value == VS_K_FAST && h->precision != VS_FP32) return fail(VS_E_INVALID, "the flux-form kernel is FP32 only");
-        if (value != h->kernel && value == VS_K_FAST) h->flux_valid = false;
-        h->kernel = value; return VS_OK;
-    case VS_OPT_TIMING: h->timing = value ? 1 : 0; return VS_OK;
-    case VS_OPT_ADJOINT_KERNEL:
-        if (value != 0 && value != 1) return fail(VS_E_INVALID, "adjoint kernel must be 0 (on chip) or 1 (per step)");
-        if (value != h->adj_kernel) h->adj_forward = false;
-        h->adj_kernel = value; return VS_OK;
-    case VS_OPT_EXACT_GROUPS:
-        if (value < 0 || value > kCoopMaxGroups) return fail(VS_E_INVALID, "exact groups must be in [0, 64]");
-        h->exact_groups = value; return VS_OK;
-    case VS_OPT_EXACT_STEPS:
-        if (value < 0 || value > kCoopMaxSteps) return fail(VS_E_INVALID, "exact steps must be in [0, 32]");
-        h->exact_steps = value; return VS_OK;
-    case VS_OPT_BARRIER_SPIN:
-        if (value < 0) return fail(VS_E_INVALID, "barrier spin must be >= 0");
-        h->barrier_spin = value; return VS_OK;
-    case VS_OPT_FAST_LAYOUT_K: return fail(VS_E_INVALID, "VS_OPT_FAST_LAYOUT_K is read-only");
-    case VS_OPT_FAST_WIDE:
-        if (value != 0 && value != 1) return fail(VS_E_INVALID, "fast wide must be 0 or 1");
-        h->fast_wide = value; return VS_OK;
-    case VS_OPT_ADJOINT_BLOCK:
-        if (value < 0) return fail(VS_E_INVALID, "adjoint block must be >= 0");
-        if (value != h->adj_block) {   // slabs are re-sized by the next forward pass
-            h->adj_fast_S = 0;
-            if (h->adj_onchip) h->adj_forward = false;
+    return guarded("vs_set_option", [&]() -> int {
+        if (check_handle(h)) return VS_E_INVALID;
+        switch (option) {
+        case VS_OPT_STEPS_PER_LAUNCH:
+            if (value < 1) return fail(VS_E_INVALID, "steps per launch must be >= 1");
+            h->steps_per_launch = value; return VS_OK;
+        case VS_OPT_THREADS:
+            if (value < 64 || value > 1024 || value % 64) return fail(VS_E_INVALID, "threads must be a multiple of 64 in [64, 1024]");
+            h->threads = value; return VS_OK;
+        case VS_OPT_USE_GRAPH: h->use_graph = value ? 1 : 0; return VS_OK;
+        case VS_OPT_KERNEL:
+            if (value != VS_K_EXACT && value != VS_K_FAST) return fail(VS_E_INVALID, "unknown kernel family");
+            if (value == VS_K_FAST && h->precision != VS_FP32) return fail(VS_E_INVALID, "the flux-form kernel is FP32 only");
+            if (value != h->kernel && value == VS_K_FAST) h->flux_valid = false;
+            h->kernel = value; return VS_OK;
+        case VS_OPT_TIMING: h->timing = value ? 1 : 0; return VS_OK;
+        case VS_OPT_ADJOINT_KERNEL:
+            if (value != 0 && value != 1) return fail(VS_E_INVALID, "adjoint kernel must be 0 (on chip) or 1 (per step)");
+            if (value != h->adj_kernel) h->adj_forward = false;
+            h->adj_kernel = value; return VS_OK;
+        case VS_OPT_EXACT_GROUPS:
+            if (value < 0 || value > kCoopMaxGroups) return fail(VS_E_INVALID, "exact groups must be in [0, 64]");
+            h->exact_groups = value; return VS_OK;
+        case VS_OPT_EXACT_STEPS:
+            if (value < 0 || value > kCoopMaxSteps) return fail(VS_E_INVALID, "exact steps must be in [0, 32]");
+            h->exact_steps = value; return VS_OK;
+        case VS_OPT_BARRIER_SPIN:
+            if (value < 0) return fail(VS_E_INVALID, "barrier spin must be >= 0");
+            h->barrier_spin = value; return VS_OK;
+        case VS_OPT_FAST_LAYOUT_K: return fail(VS_E_INVALID, "VS_OPT_FAST_LAYOUT_K is read-only");
+        case VS_OPT_FAST_WIDE:
+            if (value != 0 && value != 1) return fail(VS_E_INVALID, "fast wide must be 0 or 1");
+            h->fast_wide = value; return VS_OK;
+        case VS_OPT_ADJOINT_BLOCK:
+            if (value < 0) return fail(VS_E_INVALID, "adjoint block must be >= 0");
+            if (value != h->adj_block) {   // slabs are re-sized by the next forward pass
+                h->adj_fast_S = 0;
+                if (h->adj_onchip) h->adj_forward = false;
+            }
+            h->adj_block = value; return VS_OK;
+        case VS_OPT_ENGINE_VARIANT: {
+            if (value != 0 && value != 1) return fail(VS_E_INVALID, "engine variant must be 0 or 1");
+            for (TubeDesc& d : h->desc) d.variant = value;
+            VS_HIP(hipMemcpyAsync(h->d_desc, h->desc.data(), h->n * sizeof(TubeDesc), hipMemcpyHostToDevice, h->stream));
+            VS_HIP(hipStreamSynchronize(h->stream));
+            return VS_OK;
         }
-        h->adj_block = value; return VS_OK;
-    case VS_OPT_ENGINE_VARIANT: {
-        if (value != 0 && value != 1) return fail(VS_E_INVALID, "engine variant must be 0 or 1");
-        for (TubeDesc& d : h->desc) d.variant = value;
-        VS_HIP(hipMemcpyAsync(h->d_desc, h->desc.data(), h->n * sizeof(TubeDesc), hipMemcpyHostToDevice, h->stream));
-        VS_HIP(hipStreamSynchronize(h->stream));
-        return VS_OK;
-    }
-    default: return fail(VS_E_INVALID, "unknown option");
-    }
+        default: return fail(VS_E_INVALID, "unknown option");
+        }
+    });
 }
 
 int vs_get_option(const vs_handle* h, int option, int* value) {
-    if (check_handle(h) || !value) return fail(VS_E_INVALID, "vs_get_option: bad argument");
-    switch (option) {
-    case VS_OPT_STEPS_PER_LAUNCH: *value = h->steps_per_launch; return VS_OK;
-    case VS_OPT_THREADS: *value = h->threads; return VS_OK;
-    case VS_OPT_USE_GRAPH: *value = h->use_graph; return VS_OK;
-    case VS_OPT_KERNEL: *value = h->kernel; return VS_OK;
-    case VS_OPT_TIMING: *value = h->timing; return VS_OK;
-    case VS_OPT_ENGINE_VARIANT: *value = h->desc.empty() ? 0 : h->desc[0].variant; return VS_OK;
-    case VS_OPT_ADJOINT_KERNEL: *value = h->adj_kernel; return VS_OK;
-    case VS_OPT_ADJOINT_BLOCK: *value = h->adj_block; return VS_OK;
-    case VS_OPT_EXACT_GROUPS: *value = h->exact_groups; return VS_OK;
-    case VS_OPT_EXACT_STEPS: *value = h->exact_steps; return VS_OK;
-    case VS_OPT_BARRIER_SPIN: *value = h->barrier_spin; return VS_OK;
-    case VS_OPT_FAST_WIDE: *value = h->fast_wide; return VS_OK;
-    case VS_OPT_FAST_LAYOUT_K: {
-        int Kw = 0, K = 0;
-        if (fast_wide_fits(h, h->threads, &Kw, nullptr)) *value = Kw;
-        else if (fast_fits(h, h->threads, &K, nullptr)) *value = K;
-        else *value = 0;
-        return VS_OK;
-    }
-    default: return fail(VS_E_INVALID, "unknown option");
-    }
+    return guarded("vs_get_option", [&]() -> int {
+        if (check_handle(h) || !value) return fail(VS_E_INVALID, "vs_get_option: bad argument");
+        switch (option) {
+        case VS_OPT_STEPS_PER_LAUNCH: *value = h->steps_per_launch; return VS_OK;
+        case VS_OPT_THREADS: *value = h->threads; return VS_OK;
+        case VS_OPT_USE_GRAPH: *value = h->use_graph; return VS_OK;
+        case VS_OPT_KERNEL: *value = h->kernel; return VS_OK;
+        case VS_OPT_TIMING: *value = h->timing; return VS_OK;
+        case VS_OPT_ENGINE_VARIANT: *value = h->desc.empty() ? 0 : h->desc[0].variant; return VS_OK;
+        case VS_OPT_ADJOINT_KERNEL: *value = h->adj_kernel; return VS_OK;
+        case VS_OPT_ADJOINT_BLOCK: *value = h->adj_block; return VS_OK;
+        case VS_OPT_EXACT_GROUPS: *value = h->exact_groups; return VS_OK;
+        case VS_OPT_EXACT_STEPS: *value = h->exact_steps; return VS_OK;
+        case VS_OPT_BARRIER_SPIN: *value = h->barrier_spin; return VS_OK;
+        case VS_OPT_FAST_WIDE: *value = h->fast_wide; return VS_OK;
+        case VS_OPT_FAST_LAYOUT_K: {
+            int Kw = 0, K = 0;
+            if (fast_wide_fits(h, h->threads, &Kw, nullptr)) *value = Kw;
+            else if (fast_fits(h, h->threads, &K, nullptr)) *value = K;
+            else *value = 0;
+            return VS_OK;
+        }
+        default: return fail(VS_E_INVALID, "unknown option");
+        }
+    });
 }
 
 int vs_stream(vs_handle* h, void** stream) {
-    if (check_handle(h) || !stream) return fail(VS_E_INVALID, "vs_stream: bad argument");
-    *stream = (void*)h->stream;
-    return VS_OK;
+    return guarded("vs_stream", [&]() -> int {
+        if (check_handle(h) || !stream) return fail(VS_E_INVALID, "vs_stream: bad argument");
+        *stream = (void*)h->stream;
+        return VS_OK;
+    });
 }
 
 int vs_last_run_timing(const vs_handle* h, double* total_ms, int* launches, double* mean_launch_ms) {
-    if (check_handle(h)) return VS_E_INVALID;
-    if (total_ms) *total_ms = h->last_ms;
-    if (launches) *launches = h->last_launches;
-    if (mean_launch_ms) *mean_launch_ms = h->last_mean_launch_ms;
-    return VS_OK;
+    return guarded("vs_last_run_timing", [&]() -> int {
+        if (check_handle(h)) return VS_E_INVALID;
+        if (total_ms) *total_ms = h->last_ms;
+        if (launches) *launches = h->last_launches;
+        if (mean_launch_ms) *mean_launch_ms = h->last_mean_launch_ms;
+        return VS_OK;
+    });
 }
 
 namespace {
@@ -1132,41 +1180,47 @@ int dtft_device(const double* d_x, int n, double fs, int hann, const double* fre
 }  // namespace
 
 int vs_dtft(const double* x, int n, double fs, int hann, const double* freqs, int nf, double* re, double* im) {
-    if (!x || !freqs || !re || !im || n <= 0 || nf <= 0 || !(fs > 0))
-        return fail(VS_E_INVALID, "vs_dtft: bad argument");
-    double* d_x = nullptr;
-    VS_HIP(hipMalloc(&d_x, sizeof(double) * n));
-    hipError_t e = hipMemcpy(d_x, x, sizeof(double) * n, hipMemcpyHostToDevice);
-    int rc = e == hipSuccess ? dtft_device(d_x, n, fs, hann, freqs, nf, re, im, nullptr)
-                             : fail(VS_E_DEVICE, std::string("vs_dtft copy: ") + hipGetErrorString(e));
-    (void)hipFree(d_x);
-    return rc;
+    return guarded("vs_dtft", [&]() -> int {
+        if (!x || !freqs || !re || !im || n <= 0 || nf <= 0 || !(fs > 0))
+            return fail(VS_E_INVALID, "vs_dtft: bad argument");
+        double* d_x = nullptr;
+        VS_HIP(hipMalloc(&d_x, sizeof(double) * n));
+        hipError_t e = hipMemcpy(d_x, x, sizeof(double) * n, hipMemcpyHostToDevice);
+        int rc = e == hipSuccess ? dtft_device(d_x, n, fs, hann, freqs, nf, re, im, nullptr)
+                                 : fail(VS_E_DEVICE, std::string("vs_dtft copy: ") + hipGetErrorString(e));
+        (void)hipFree(d_x);
+        return rc;
+    });
 }
 
 int vs_dtft_noalloc(const double* x, int n, double fs, int hann, const double* freqs, int nf, double* re,
                     double* im) {
-    if (!x || !freqs || !re || !im || n <= 0 || nf <= 0 || !(fs > 0))
-        return fail(VS_E_INVALID, "vs_dtft_noalloc: bad argument");
-    if (n > kNoAllocMaxSamples || nf > kNoAllocMaxFreqs)
-        return fail(VS_E_RANGE, "vs_dtft_noalloc: at most " + std::to_string(kNoAllocMaxSamples) + " samples and " +
-                                    std::to_string(kNoAllocMaxFreqs) + " frequencies");
-    const hipError_t e = dtft_noalloc(x, n, fs, hann, freqs, nf, re, im);
-    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("vs_dtft_noalloc: ") + hipGetErrorString(e));
-    return VS_OK;
+    return guarded("vs_dtft_noalloc", [&]() -> int {
+        if (!x || !freqs || !re || !im || n <= 0 || nf <= 0 || !(fs > 0))
+            return fail(VS_E_INVALID, "vs_dtft_noalloc: bad argument");
+        if (n > kNoAllocMaxSamples || nf > kNoAllocMaxFreqs)
+            return fail(VS_E_RANGE, "vs_dtft_noalloc: at most " + std::to_string(kNoAllocMaxSamples) + " samples and " +
+                                        std::to_string(kNoAllocMaxFreqs) + " frequencies");
+        const hipError_t e = dtft_noalloc(x, n, fs, hann, freqs, nf, re, im);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("vs_dtft_noalloc: ") + hipGetErrorString(e));
+        return VS_OK;
+    });
 }
 
 int vs_probe_dtft(vs_handle* h, int tube, int kind, int hann, const double* freqs, int nf, double* re, double* im) {
-    if (check_handle(h) || !freqs || !re || !im || nf <= 0) return fail(VS_E_INVALID, "vs_probe_dtft: bad argument");
-    if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
-    if (kind != 0 && kind != 1) return fail(VS_E_INVALID, "kind must be 0 (probe) or 1 (source)");
-    const TubeDesc& d = h->desc[tube];
-    const int n = std::min(h->steps_done, d.num_steps);
-    if (n <= 0) return fail(VS_E_STATE, "no steps advanced yet");
-    const double* base = kind == 0 ? (h->precision == 64 ? h->bd.probe : h->bf.probe)
-                                    : (h->precision == 64 ? h->bd.strace : h->bf.strace);
-    const double fs = 1.0 / d.ph.dt;
-    VS_HIP(hipStreamSynchronize(h->stream));
-    return dtft_device(base + d.src_off, n, fs, hann, freqs, nf, re, im, h->stream);
+    return guarded("vs_probe_dtft", [&]() -> int {
+        if (check_handle(h) || !freqs || !re || !im || nf <= 0) return fail(VS_E_INVALID, "vs_probe_dtft: bad argument");
+        if (tube < 0 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+        if (kind != 0 && kind != 1) return fail(VS_E_INVALID, "kind must be 0 (probe) or 1 (source)");
+        const TubeDesc& d = h->desc[tube];
+        const int n = std::min(h->steps_done, d.num_steps);
+        if (n <= 0) return fail(VS_E_STATE, "no steps advanced yet");
+        const double* base = kind == 0 ? (h->precision == 64 ? h->bd.probe : h->bf.probe)
+                                        : (h->precision == 64 ? h->bd.strace : h->bf.strace);
+        const double fs = 1.0 / d.ph.dt;
+        VS_HIP(hipStreamSynchronize(h->stream));
+        return dtft_device(base + d.src_off, n, fs, hann, freqs, nf, re, im, h->stream);
+    });
 }
 
 namespace {
@@ -1183,125 +1237,133 @@ std::mutex g_tune_mutex;
 }  // namespace
 
 int vs_autotune(vs_handle* h, int timed_steps, char* report, size_t report_len) {
-    if (check_handle(h)) return VS_E_INVALID;
-    if (h->steps_done != 0) return fail(VS_E_STATE, "vs_autotune needs a handle with no steps advanced");
-    if (timed_steps <= 0) timed_steps = 64;
-    const TuneKey key{h->n, h->max_interior, h->precision, h->kernel, h->fast_wide, h->cells_per_step};
-    std::string rep = "{\"candidates\": [";
-    std::unique_lock<std::mutex> cache_lock(g_tune_mutex);
-    for (auto& kv : g_tune_cache)
-        if (kv.first == key) {
-            h->threads = kv.second.first;
-            h->steps_per_launch = kv.second.second;
-            if (report && report_len) {
-                std::string r = "{\"cached\": true, \"threads\": " + std::to_string(h->threads) +
-                                ", \"steps_per_launch\": " + std::to_string(h->steps_per_launch) + "}";
-                std::snprintf(report, report_len, "%s", r.c_str());
+    return guarded("vs_autotune", [&]() -> int {
+        if (check_handle(h)) return VS_E_INVALID;
+        if (h->steps_done != 0) return fail(VS_E_STATE, "vs_autotune needs a handle with no steps advanced");
+        if (timed_steps <= 0) timed_steps = 64;
+        const TuneKey key{h->n, h->max_interior, h->precision, h->kernel, h->fast_wide, h->cells_per_step};
+        std::string rep = "{\"candidates\": [";
+        std::unique_lock<std::mutex> cache_lock(g_tune_mutex);
+        for (auto& kv : g_tune_cache)
+            if (kv.first == key) {
+                h->threads = kv.second.first;
+                h->steps_per_launch = kv.second.second;
+                if (report && report_len) {
+                    std::string r = "{\"cached\": true, \"threads\": " + std::to_string(h->threads) +
+                                    ", \"steps_per_launch\": " + std::to_string(h->steps_per_launch) + "}";
+                    std::snprintf(report, report_len, "%s", r.c_str());
+                }
+                return VS_OK;
             }
-            return VS_OK;
-        }
-    std::vector<int> threads;
-    if (h->kernel == VS_K_FAST) {
-        for (int t = 1024; t >= 128; t -= 64) {
-            const int K = fast_cells_per_lane(h->max_padded, t);
-            if (fast_wide_fits(h, t, nullptr, nullptr) ||
-                (K <= kFastMaxK && (long long)(fast_lds_bytes(K, t, fast_guard_lanes(K, h->max_wp)) +
-                                       fast_lds_static_bytes()) <= h->lds_limit))
-                threads.push_back(t);
-        }
-    } else {
-        threads = {1024, 768, 512, 256};
-    }
-    cache_lock.unlock();
-    if (threads.empty()) return fail(VS_E_INVALID, "no work-group shape fits this batch");
-    const int spls[3] = {128, 256, 512};
-    const int warm = 16;
-    const int old_timing = h->timing, old_graph = h->use_graph;
-    h->timing = 0;
-    h->use_graph = 0;
-    double best = 1e300;
-    int best_t = h->threads, best_s = h->steps_per_launch;
-    bool first = true;
-    for (int t : threads) {
-        for (int spl : spls) {
-            if (spl > 2 * timed_steps && spl != 128) continue;
-            h->threads = t;
-            h->steps_per_launch = spl;
-            int rc = vs_reset(h);
-            if (rc == VS_OK) rc = vs_run(h, warm);
-            if (rc == VS_OK) rc = vs_sync(h);
-            if (rc == VS_OK) rc = vs_run(h, timed_steps);
-            if (rc == VS_OK) rc = vs_sync(h);
-            if (rc != VS_OK) {
-                h->timing = old_timing; h->use_graph = old_graph;
-                return rc;
+        std::vector<int> threads;
+        if (h->kernel == VS_K_FAST) {
+            for (int t = 1024; t >= 128; t -= 64) {
+                const int K = fast_cells_per_lane(h->max_padded, t);
+                if (fast_wide_fits(h, t, nullptr, nullptr) ||
+                    (K <= kFastMaxK && (long long)(fast_lds_bytes(K, t, fast_guard_lanes(K, h->max_wp)) +
+                                           fast_lds_static_bytes()) <= h->lds_limit))
+                    threads.push_back(t);
             }
-            const int ran = h->steps_done - warm;
-            if (ran <= 0) continue;
-            const double us = h->last_ms * 1e3 / ran;
-            rep += std::string(first ? "" : ", ") + "{\"threads\": " + std::to_string(t) +
-                   ", \"steps_per_launch\": " + std::to_string(spl) + ", \"us_per_step\": " + std::to_string(us) + "}";
-            first = false;
-            if (us < best) { best = us; best_t = t; best_s = spl; }
+        } else {
+            threads = {1024, 768, 512, 256};
         }
-    }
-    h->threads = best_t;
-    h->steps_per_launch = best_s;
-    h->timing = old_timing;
-    h->use_graph = old_graph;
-    const int rc = vs_reset(h);
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mutex);
-        g_tune_cache.push_back({key, {best_t, best_s}});
-    }
-    rep += "], \"threads\": " + std::to_string(best_t) + ", \"steps_per_launch\": " + std::to_string(best_s) +
-           ", \"us_per_step\": " + std::to_string(best) + "}";
-    if (report && report_len) std::snprintf(report, report_len, "%s", rep.c_str());
-    return rc;
+        cache_lock.unlock();
+        if (threads.empty()) return fail(VS_E_INVALID, "no work-group shape fits this batch");
+        const int spls[3] = {128, 256, 512};
+        const int warm = 16;
+        const int old_timing = h->timing, old_graph = h->use_graph;
+        h->timing = 0;
+        h->use_graph = 0;
+        double best = 1e300;
+        int best_t = h->threads, best_s = h->steps_per_launch;
+        bool first = true;
+        for (int t : threads) {
+            for (int spl : spls) {
+                if (spl > 2 * timed_steps && spl != 128) continue;
+                h->threads = t;
+                h->steps_per_launch = spl;
+                int rc = vs_reset(h);
+                if (rc == VS_OK) rc = vs_run(h, warm);
+                if (rc == VS_OK) rc = vs_sync(h);
+                if (rc == VS_OK) rc = vs_run(h, timed_steps);
+                if (rc == VS_OK) rc = vs_sync(h);
+                if (rc != VS_OK) {
+                    h->timing = old_timing; h->use_graph = old_graph;
+                    return rc;
+                }
+                const int ran = h->steps_done - warm;
+                if (ran <= 0) continue;
+                const double us = h->last_ms * 1e3 / ran;
+                rep += std::string(first ? "" : ", ") + "{\"threads\": " + std::to_string(t) +
+                       ", \"steps_per_launch\": " + std::to_string(spl) + ", \"us_per_step\": " + std::to_string(us) + "}";
+                first = false;
+                if (us < best) { best = us; best_t = t; best_s = spl; }
+            }
+        }
+        h->threads = best_t;
+        h->steps_per_launch = best_s;
+        h->timing = old_timing;
+        h->use_graph = old_graph;
+        const int rc = vs_reset(h);
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mutex);
+            g_tune_cache.push_back({key, {best_t, best_s}});
+        }
+        rep += "], \"threads\": " + std::to_string(best_t) + ", \"steps_per_launch\": " + std::to_string(best_s) +
+               ", \"us_per_step\": " + std::to_string(best) + "}";
+        if (report && report_len) std::snprintf(report, report_len, "%s", rep.c_str());
+        return rc;
+    });
 }
 
 int vs_copy_to_host(void* dst, const void* src, size_t bytes) {
-    if (!dst || !src) return fail(VS_E_INVALID, "vs_copy_to_host: null pointer");
-    if (bytes == 0) return VS_OK;
-    hipPointerAttribute_t attr{};
-    const hipError_t e = hipPointerGetAttributes(&attr, src);
-    if (e == hipSuccess && attr.type == hipMemoryTypeDevice) {
-        VS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-    } else {
-        (void)hipGetLastError();
-        std::memcpy(dst, src, bytes);
-    }
-    return VS_OK;
+    return guarded("vs_copy_to_host", [&]() -> int {
+        if (!dst || !src) return fail(VS_E_INVALID, "vs_copy_to_host: null pointer");
+        if (bytes == 0) return VS_OK;
+        hipPointerAttribute_t attr{};
+        const hipError_t e = hipPointerGetAttributes(&attr, src);
+        if (e == hipSuccess && attr.type == hipMemoryTypeDevice) {
+            VS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        } else {
+            (void)hipGetLastError();
+            std::memcpy(dst, src, bytes);
+        }
+        return VS_OK;
+    });
 }
 
 int vs_cells_per_step(const vs_handle* h, long long* cells) {
-    if (check_handle(h) || !cells) return fail(VS_E_INVALID, "vs_cells_per_step: bad argument");
-    *cells = h->cells_per_step;
-    return VS_OK;
+    return guarded("vs_cells_per_step", [&]() -> int {
+        if (check_handle(h) || !cells) return fail(VS_E_INVALID, "vs_cells_per_step: bad argument");
+        *cells = h->cells_per_step;
+        return VS_OK;
+    });
 }
 
 int vs_measure_lds_bandwidth(int width_bytes, double* bytes_per_s) {
-    if (!bytes_per_s || (width_bytes != 4 && width_bytes != 8 && width_bytes != 16))
-        return fail(VS_E_INVALID, "vs_measure_lds_bandwidth: width must be 4, 8 or 16 bytes");
-    int dev = 0, cus = 0;
-    VS_HIP(hipGetDevice(&dev));
-    VS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int groups = 8 * std::max(cus, 1), iters = 512, words = width_bytes / 4;
-    float* d_out = nullptr;
-    VS_HIP(hipMalloc(&d_out, sizeof(float) * (size_t)groups * 1024));
-    hipStream_t st = nullptr;
-    hipError_t e = hipStreamCreate(&st);
-    float best = 0.f;
-    for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {   // first launch warms up
-        float ms = 0.f;
-        e = measure_lds_read(words, groups, iters, d_out, st, &ms);
-        if (e == hipSuccess && rep > 0 && (best == 0.f || ms < best)) best = ms;
-    }
-    if (st) (void)hipStreamDestroy(st);
-    (void)hipFree(d_out);
-    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("lds measurement: ") + hipGetErrorString(e));
-    *bytes_per_s = (double)groups * 1024.0 * iters * 8.0 * width_bytes / (best * 1e-3);
-    return VS_OK;
+    return guarded("vs_measure_lds_bandwidth", [&]() -> int {
+        if (!bytes_per_s || (width_bytes != 4 && width_bytes != 8 && width_bytes != 16))
+            return fail(VS_E_INVALID, "vs_measure_lds_bandwidth: width must be 4, 8 or 16 bytes");
+        int dev = 0, cus = 0;
+        VS_HIP(hipGetDevice(&dev));
+        VS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int groups = 8 * std::max(cus, 1), iters = 512, words = width_bytes / 4;
+        float* d_out = nullptr;
+        VS_HIP(hipMalloc(&d_out, sizeof(float) * (size_t)groups * 1024));
+        hipStream_t st = nullptr;
+        hipError_t e = hipStreamCreate(&st);
+        float best = 0.f;
+        for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {   // first launch warms up
+            float ms = 0.f;
+            e = measure_lds_read(words, groups, iters, d_out, st, &ms);
+            if (e == hipSuccess && rep > 0 && (best == 0.f || ms < best)) best = ms;
+        }
+        if (st) (void)hipStreamDestroy(st);
+        (void)hipFree(d_out);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("lds measurement: ") + hipGetErrorString(e));
+        *bytes_per_s = (double)groups * 1024.0 * iters * 8.0 * width_bytes / (best * 1e-3);
+        return VS_OK;
+    });
 }
 
 }  // extern "C"
@@ -1933,27 +1995,33 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
 extern "C" {
 
 int vs_adjoint_timing(const vs_handle* h, double* reverse_ms, double* chain_ms) {
-    if (check_handle(h)) return VS_E_INVALID;
-    if (reverse_ms) *reverse_ms = h->adj_reverse_ms;
-    if (chain_ms) *chain_ms = h->adj_chain_ms;
-    return VS_OK;
+    return guarded("vs_adjoint_timing", [&]() -> int {
+        if (check_handle(h)) return VS_E_INVALID;
+        if (reverse_ms) *reverse_ms = h->adj_reverse_ms;
+        if (chain_ms) *chain_ms = h->adj_chain_ms;
+        return VS_OK;
+    });
 }
 
 int vs_adjoint_forward(vs_handle* h, int checkpoint_every, double* probe, size_t probe_len) {
-    if (int rc = check_handle(h)) return rc;
-    if (checkpoint_every < 1 || checkpoint_every > (1 << 20))
-        return fail(VS_E_INVALID, "checkpoint_every must be in [1, 2^20]");
-    return h->precision == VS_FP64 ? adj_forward<double>(h, checkpoint_every, probe, probe_len)
-                                   : adj_forward<float>(h, checkpoint_every, probe, probe_len);
+    return guarded("vs_adjoint_forward", [&]() -> int {
+        if (int rc = check_handle(h)) return rc;
+        if (checkpoint_every < 1 || checkpoint_every > (1 << 20))
+            return fail(VS_E_INVALID, "checkpoint_every must be in [1, 2^20]");
+        return h->precision == VS_FP64 ? adj_forward<double>(h, checkpoint_every, probe, probe_len)
+                                       : adj_forward<float>(h, checkpoint_every, probe, probe_len);
+    });
 }
 
 int vs_adjoint_backward(vs_handle* h, const double* gprobe, size_t gprobe_len, double* g_area, double* g_depth_scale,
                         double* g_mu3d, size_t n_sections_total) {
-    if (int rc = check_handle(h)) return rc;
-    if (!g_area || !g_depth_scale || !g_mu3d) return fail(VS_E_INVALID, "null gradient buffer");
-    return h->precision == VS_FP64
-               ? adj_backward<double>(h, gprobe, gprobe_len, g_area, g_depth_scale, g_mu3d, n_sections_total)
-               : adj_backward<float>(h, gprobe, gprobe_len, g_area, g_depth_scale, g_mu3d, n_sections_total);
+    return guarded("vs_adjoint_backward", [&]() -> int {
+        if (int rc = check_handle(h)) return rc;
+        if (!g_area || !g_depth_scale || !g_mu3d) return fail(VS_E_INVALID, "null gradient buffer");
+        return h->precision == VS_FP64
+                   ? adj_backward<double>(h, gprobe, gprobe_len, g_area, g_depth_scale, g_mu3d, n_sections_total)
+                   : adj_backward<float>(h, gprobe, gprobe_len, g_area, g_depth_scale, g_mu3d, n_sections_total);
+    });
 }
 
 }  // extern "C"
@@ -1964,83 +2032,89 @@ int vs_adjoint_backward(vs_handle* h, const double* gprobe, size_t gprobe_len, d
 extern "C" {
 
 int vs_butter2_lowpass(double cutoff_hz, double fs, double* b3, double* a3) {
-    if (!b3 || !a3) return fail(VS_E_INVALID, "null coefficient buffer");
-    if (!(cutoff_hz > 0.0) || !(cutoff_hz < 0.5 * fs)) return fail(VS_E_INVALID, "cutoff must lie in (0, fs / 2)");
-    const AudioFilter f = butter2_lowpass(cutoff_hz, fs);
-    for (int k = 0; k < 3; ++k) { b3[k] = f.b[k]; a3[k] = f.a[k]; }
-    return VS_OK;
+    return guarded("vs_butter2_lowpass", [&]() -> int {
+        if (!b3 || !a3) return fail(VS_E_INVALID, "null coefficient buffer");
+        if (!(cutoff_hz > 0.0) || !(cutoff_hz < 0.5 * fs)) return fail(VS_E_INVALID, "cutoff must lie in (0, fs / 2)");
+        const AudioFilter f = butter2_lowpass(cutoff_hz, fs);
+        for (int k = 0; k < 3; ++k) { b3[k] = f.b[k]; a3[k] = f.a[k]; }
+        return VS_OK;
+    });
 }
 
 int vs_probe_audio(vs_handle* h, int tube, float* out, size_t cap, size_t* n_out) {
-    if (int rc = check_handle(h)) return rc;
-    if (tube < -1 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
-    const int t0 = tube < 0 ? 0 : tube, t1 = tube < 0 ? h->n : tube + 1;
-    std::vector<AudioJob> jobs;
-    long long total = 0;
-    for (int k = t0; k < t1; ++k) {
-        const TubeDesc& d = h->desc[k];
-        AudioJob J{};
-        J.in_off = d.src_off;
-        J.n = std::min(h->steps_done, d.num_steps);
-        J.decim = d.srate_mul;   // fs = 44100 * srate_mul
-        J.flt = butter2_lowpass(5050.0, 44100.0 * d.srate_mul);
-        J.out_off = total;
-        total += J.decim == 1 ? J.n : (J.n + J.decim - 1) / J.decim;
-        jobs.push_back(J);
-    }
-    if (n_out) *n_out = (size_t)total;
-    if (!out) return VS_OK;
-    if (cap < (size_t)total) return fail(VS_E_RANGE, "audio buffer needs " + std::to_string(total) + " samples");
-    if (total == 0) return VS_OK;
-    AudioJob* dj = nullptr;
-    float* dout = nullptr;
-    VS_HIP(hipMalloc(&dj, jobs.size() * sizeof(AudioJob)));
-    hipError_t e = hipMalloc(&dout, (size_t)total * sizeof(float));
-    if (e == hipSuccess) e = hipMemcpyAsync(dj, jobs.data(), jobs.size() * sizeof(AudioJob), hipMemcpyHostToDevice, h->stream);
-    const double* probe = h->precision == 64 ? h->bd.probe : h->bf.probe;
-    if (e == hipSuccess) e = launch_audio(probe, dj, (int)jobs.size(), dout, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e == hipSuccess) e = copy_on(h->stream, out, dout, (size_t)total * sizeof(float), hipMemcpyDeviceToHost);
-    (void)hipFree(dj);
-    if (dout) (void)hipFree(dout);
-    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("audio: ") + hipGetErrorString(e));
-    return VS_OK;
+    return guarded("vs_probe_audio", [&]() -> int {
+        if (int rc = check_handle(h)) return rc;
+        if (tube < -1 || tube >= h->n) return fail(VS_E_RANGE, "tube index out of range");
+        const int t0 = tube < 0 ? 0 : tube, t1 = tube < 0 ? h->n : tube + 1;
+        std::vector<AudioJob> jobs;
+        long long total = 0;
+        for (int k = t0; k < t1; ++k) {
+            const TubeDesc& d = h->desc[k];
+            AudioJob J{};
+            J.in_off = d.src_off;
+            J.n = std::min(h->steps_done, d.num_steps);
+            J.decim = d.srate_mul;   // fs = 44100 * srate_mul
+            J.flt = butter2_lowpass(5050.0, 44100.0 * d.srate_mul);
+            J.out_off = total;
+            total += J.decim == 1 ? J.n : (J.n + J.decim - 1) / J.decim;
+            jobs.push_back(J);
+        }
+        if (n_out) *n_out = (size_t)total;
+        if (!out) return VS_OK;
+        if (cap < (size_t)total) return fail(VS_E_RANGE, "audio buffer needs " + std::to_string(total) + " samples");
+        if (total == 0) return VS_OK;
+        AudioJob* dj = nullptr;
+        float* dout = nullptr;
+        VS_HIP(hipMalloc(&dj, jobs.size() * sizeof(AudioJob)));
+        hipError_t e = hipMalloc(&dout, (size_t)total * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpyAsync(dj, jobs.data(), jobs.size() * sizeof(AudioJob), hipMemcpyHostToDevice, h->stream);
+        const double* probe = h->precision == 64 ? h->bd.probe : h->bf.probe;
+        if (e == hipSuccess) e = launch_audio(probe, dj, (int)jobs.size(), dout, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) e = copy_on(h->stream, out, dout, (size_t)total * sizeof(float), hipMemcpyDeviceToHost);
+        (void)hipFree(dj);
+        if (dout) (void)hipFree(dout);
+        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("audio: ") + hipGetErrorString(e));
+        return VS_OK;
+    });
 }
 
 int vs_write_wav(const char* path, const float* samples, size_t n, int rate) {
-    if (!path || (!samples && n)) return fail(VS_E_INVALID, "vs_write_wav: bad argument");
-    if (rate <= 0) return fail(VS_E_INVALID, "sample rate must be positive");
-    FILE* f = std::fopen(path, "wb");
-    if (!f) return fail(VS_E_INVALID, std::string("cannot open ") + path);
-    auto u32 = [&](uint32_t v) { unsigned char b[4] = {(unsigned char)v, (unsigned char)(v >> 8), (unsigned char)(v >> 16), (unsigned char)(v >> 24)}; std::fwrite(b, 1, 4, f); };
-    auto u16 = [&](uint16_t v) { unsigned char b[2] = {(unsigned char)v, (unsigned char)(v >> 8)}; std::fwrite(b, 1, 2, f); };
-    const uint32_t data_bytes = (uint32_t)(n * 4);
-    // RIFF / WAVE with an 18-byte fmt chunk (IEEE float, cbSize 0) and a fact
-    // chunk, the layout scipy.io.wavfile.write uses for float32 data
-    std::fwrite("RIFF", 1, 4, f);
-    u32(4 + (8 + 18) + (8 + 4) + (8 + data_bytes));
-    std::fwrite("WAVE", 1, 4, f);
-    std::fwrite("fmt ", 1, 4, f);
-    u32(18);
-    u16(3);                    // WAVE_FORMAT_IEEE_FLOAT
-    u16(1);                    // channels
-    u32((uint32_t)rate);
-    u32((uint32_t)rate * 4);   // bytes per second
-    u16(4);                    // block align
-    u16(32);                   // bits per sample
-    u16(0);                    // cbSize
-    std::fwrite("fact", 1, 4, f);
-    u32(4);
-    u32((uint32_t)n);
-    std::fwrite("data", 1, 4, f);
-    u32(data_bytes);
-    for (size_t i = 0; i < n; ++i) {
-        uint32_t v;
-        std::memcpy(&v, &samples[i], 4);
-        u32(v);
-    }
-    const bool ok = std::fclose(f) == 0;
-    return ok ? VS_OK : fail(VS_E_INVALID, std::string("write failed: ") + path);
+    return guarded("vs_write_wav", [&]() -> int {
+        if (!path || (!samples && n)) return fail(VS_E_INVALID, "vs_write_wav: bad argument");
+        if (rate <= 0) return fail(VS_E_INVALID, "sample rate must be positive");
+        FILE* f = std::fopen(path, "wb");
+        if (!f) return fail(VS_E_INVALID, std::string("cannot open ") + path);
+        auto u32 = [&](uint32_t v) { unsigned char b[4] = {(unsigned char)v, (unsigned char)(v >> 8), (unsigned char)(v >> 16), (unsigned char)(v >> 24)}; std::fwrite(b, 1, 4, f); };
+        auto u16 = [&](uint16_t v) { unsigned char b[2] = {(unsigned char)v, (unsigned char)(v >> 8)}; std::fwrite(b, 1, 2, f); };
+        const uint32_t data_bytes = (uint32_t)(n * 4);
+        // RIFF / WAVE with an 18-byte fmt chunk (IEEE float, cbSize 0) and a fact
+        // chunk, the layout scipy.io.wavfile.write uses for float32 data
+        std::fwrite("RIFF", 1, 4, f);
+        u32(4 + (8 + 18) + (8 + 4) + (8 + data_bytes));
+        std::fwrite("WAVE", 1, 4, f);
+        std::fwrite("fmt ", 1, 4, f);
+        u32(18);
+        u16(3);                    // WAVE_FORMAT_IEEE_FLOAT
+        u16(1);                    // channels
+        u32((uint32_t)rate);
+        u32((uint32_t)rate * 4);   // bytes per second
+        u16(4);                    // block align
+        u16(32);                   // bits per sample
+        u16(0);                    // cbSize
+        std::fwrite("fact", 1, 4, f);
+        u32(4);
+        u32((uint32_t)n);
+        std::fwrite("data", 1, 4, f);
+        u32(data_bytes);
+        for (size_t i = 0; i < n; ++i) {
+            uint32_t v;
+            std::memcpy(&v, &samples[i], 4);
+            u32(v);
+        }
+        const bool ok = std::fclose(f) == 0;
+        return ok ? VS_OK : fail(VS_E_INVALID, std::string("write failed: ") + path);
+    });
 }
 
 }  // extern "C"
