@@ -52,3 +52,23 @@ def test_tubes_stop_at_their_own_step_count(hip_device, precision, graph):
         assert got[i][0].size == COUNTS[i]
         for name, a, b in zip(("probe", "P", "Vx", "Vy"), got[i], ref):
             assert np.array_equal(a, b), f"tube {i} ({COUNTS[i]} steps): {name} differs from the tube run alone"
+
+
+def test_batch_larger_than_the_chip(hip_device):
+    # 300 tubes, one work-group each on a 256-CU device: the on-chip kernel's
+    # work-groups are independent (no grid-wide wait), so those that do not
+    # fit the first wave run after the largest tubes (the launch order is
+    # largest first) and give the same values
+    # the bench batch's first 150 geometries (all within the on-chip
+    # kernel's 1024-thread capacity), twice
+    half = tubes.batch_tubes(150, seed=0, num_steps=300)
+    sp = half + half[::-1]
+    print("[batch 300] FP32 on chip with graphs", flush=True)
+    with Solver(sp, precision=32, threads=1024, kernel=_abi.K_FAST, steps_per_launch=64) as s:
+        s.set_option(_abi.OPT_USE_GRAPH, 1)
+        s.run()
+        got = {i: s.probe(i) for i in (0, 149, 150, 299)}
+    for i, p in got.items():
+        with Solver([sp[i]], precision=32, threads=1024, kernel=_abi.K_FAST, steps_per_launch=64) as s1:
+            s1.run()
+            assert np.array_equal(p, s1.probe(0)), f"tube {i}"
