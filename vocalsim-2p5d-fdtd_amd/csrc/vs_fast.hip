@@ -348,6 +348,11 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
         // ------------------------- control wave -------------------------
         const int lane = t - T;
         const bool twomass = D.source_type == 2;
+        // The recurrence is a chain of dependent double divisions on one
+        // lane; at the highest wave priority its instructions issue ahead of
+        // the cell waves sharing its SIMD (with a Gaussian source the wave
+        // only moves the source, probe and trace values and keeps priority 0)
+        if (twomass) __builtin_amdgcn_s_setprio(3);
         const int jL = (D.listener / W - 1) * Wp + (D.listener % W - 1);
         const int jE = (D.excitation / W - 1) * Wp + (D.excitation % W - 1);
         const float* pL = Pg + jL;
@@ -667,6 +672,11 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
         // ------------------------- control wave -------------------------
         const int lane = t - T;
         const bool twomass = D.source_type == 2;
+        // The recurrence is a chain of dependent double divisions on one
+        // lane; at the highest wave priority its instructions issue ahead of
+        // the cell waves sharing its SIMD (with a Gaussian source the wave
+        // only moves the source, probe and trace values and keeps priority 0)
+        if (twomass) __builtin_amdgcn_s_setprio(3);
         const int jL = (D.listener / W - 1) * Wp + (D.listener % W - 1);
         const int jE = (D.excitation / W - 1) * Wp + (D.excitation % W - 1);
         const float* pL = Pg + jL;
