@@ -22,6 +22,7 @@ for _p in (os.path.join(ROOT, "vocalsim-2p5d-fdtd_amd"), os.path.join(ROOT, "ora
         sys.path.insert(0, _p)
 
 import oracle as orc  # noqa: E402
+import random_specs  # noqa: E402
 from vocalsim import tubes  # noqa: E402
 from vocalsim._abi import TubeSpecC, spec_to_c  # noqa: E402
 
@@ -51,8 +52,8 @@ def load(path: str):
     L.hs_plane_section.argtypes = [_c.c_float, _P(_c.c_float), _P(_c.c_float), _P(_c.c_double), _c.c_char_p, _c.c_int]
     L.hs_exact_classes.argtypes = [_P(_c.c_uint8), _c.c_int, _c.c_int, _P(_c.c_uint8), _P(_c.c_int), _P(_c.c_int),
                                    _c.c_char_p, _c.c_int]
-    L.hs_fast_classes.argtypes = [_P(_c.c_uint16), _c.c_long, _P(_c.c_float), _P(_c.c_uint8), _P(_c.c_float),
-                                  _P(_c.c_int), _c.c_char_p, _c.c_int]
+    L.hs_fast_classes.argtypes = [_P(_c.c_uint16), _P(_c.c_long), _P(_c.c_float), _c.c_int, _P(_c.c_uint8),
+                                  _P(_c.c_float), _P(_c.c_int), _c.c_char_p, _c.c_int]
     L.hs_planted_overflow.argtypes = [_c.c_int]
     return L
 
@@ -96,16 +97,19 @@ def exact_classes(L, cls):
     return rc, xcls, keys[:nk.value], err.value.decode()
 
 
-def fast_classes(L, code, lut):
+def fast_classes(L, code, lut, bounds=None):
+    """One damping table `lut` (8 factors) for all cells, or with `bounds`
+    (g + 1 cell indices) one table per group: `lut` of shape (g, 8)."""
     code = np.ascontiguousarray(code, dtype=np.uint16)
-    lut = np.ascontiguousarray(lut, dtype=np.float32)
+    luts = np.ascontiguousarray(np.reshape(lut, (-1, 8)), dtype=np.float32)
+    b = np.ascontiguousarray([0, code.size] if bounds is None else bounds, dtype=np.int64)
     c8 = np.zeros(code.size, dtype=np.uint8)
     nf = L.hs_fast_max()
     tab = np.full(5 * nf, np.nan, dtype=np.float32)
     ncls = ctypes.c_int(0)
     err = ctypes.create_string_buffer(256)
-    rc = L.hs_fast_classes(_ptr(code, _c.c_uint16), code.size, _ptr(lut, _c.c_float), _ptr(c8, _c.c_uint8),
-                           _ptr(tab, _c.c_float), ctypes.byref(ncls), err, 256)
+    rc = L.hs_fast_classes(_ptr(code, _c.c_uint16), _ptr(b, _c.c_long), _ptr(luts, _c.c_float), len(b) - 1,
+                           _ptr(c8, _c.c_uint8), _ptr(tab, _c.c_float), ctypes.byref(ncls), err, 256)
     return rc, c8, tab.reshape(5, nf), ncls.value, err.value.decode()
 
 
@@ -187,6 +191,23 @@ def check_prelude(L, names=None, with_plane_area_file=True):
                 assert np.array_equal(sec[:, 4], ds.astype(np.float64)), name
             n += 1
     return n
+
+
+def check_prelude_random(L, n=200, seed=2024):
+    """The prelude of seeded random specs (tests/random_specs.py: section
+    counts, uneven lengths, areas, shapes, 2D / 2.5D, PML, mouth, sample-rate
+    multiplier, sources) equals the oracle's frame dimensions and
+    per-section values at both precisions."""
+    for i, spec in enumerate(random_specs.random_specs(n, seed)):
+        for prec in (32, 64):
+            rc, dims, sec, msg = prelude(L, spec, prec == 64)
+            assert rc == 0, f"random spec {i}/{prec}: {msg}"
+            ref = orc.OracleTube(spec, prec)
+            assert dims == {k: ref.dims[k] for k in DIMS}, f"random spec {i}/{prec}: frame dimensions differ"
+            assert np.array_equal(sec[:, 1], ref.field("diam_cells")), f"random spec {i}/{prec}: diameters"
+            assert np.array_equal(sec[:, 2], ref.field("minor_axis")), f"random spec {i}/{prec}: minor axes"
+            assert np.array_equal(sec[:, 3], ref.field("z_section")), f"random spec {i}/{prec}: z_inv"
+    return 2 * n
 
 
 # The per-section depth scale is an input of this library (the reference has
@@ -303,10 +324,43 @@ def check_fast_classes(L):
     assert fast_classes(L, many, lut)[0] == 0
     rc, *_, msg = fast_classes(L, np.setdiff1d(np.arange(2048), [zero])[:64] << 3, lut)
     assert rc == -1 and "cell classes" in msg, (rc, msg)
+    # one table per group (tube): cells of two groups share a class where
+    # their damping factors and face-form bits are equal.  With equal tables
+    # the classes are those of one group; a table whose wall damping (code 2)
+    # differs gives classes of its own to exactly the cells that use code 2.
+    half = d.size // 2
+    bounds = [0, half, d.size]
+    same = fast_classes(L, code, np.stack([lut, lut]), bounds)
+    assert same[0] == 0 and np.array_equal(same[1], c8) and np.array_equal(same[2], tab) and same[3] == ncls
+    lut2 = lut.copy()
+    lut2[2] = np.float32(0.5)
+    rc2, c82, tab2, ncls2, msg2 = fast_classes(L, code, np.stack([lut, lut2]), bounds)
+    assert rc2 == 0, msg2
+    assert np.array_equal(c82[:half], c8[:half]), "the first group's classes change"
+
+    def entry(t, v):   # (aP, ax, ay) of face code v under table t, flags in the signs, as bit patterns
+        s0, r0 = bool(v & 0x200), bool(v & 0x400)
+        e = np.array([t[(v >> 6) & 7], (-1 if s0 else 1) * t[v & 7], (-1 if r0 else 1) * t[(v >> 3) & 7]],
+                     np.float32)
+        return tuple(e.view(np.uint32).tolist())
+
+    want = [entry(lut if i < half else lut2, int(v)) for i, v in enumerate(d)]
+    got = [tuple(tab2[:3, c].astype(np.float32).view(np.uint32).tolist()) for c in c82]
+    assert got == want, "a cell's class entry is not its own factors"
+    assert ncls2 == len(set(want)), (ncls2, len(set(want)))
+    uses2 = {int(v) for v in d if 2 in ((v >> 6) & 7, v & 7, (v >> 3) & 7)}
+    for i in range(half, d.size):
+        v = int(d[i])
+        if v not in uses2 and v in cls_of:
+            assert c82[i] == cls_of[v], "a class without the wall damping is not shared across tables"
+    # bounds must start at 0 and not decrease
+    assert fast_classes(L, code, np.stack([lut, lut]), [1, half, d.size])[0] == -1
+    assert fast_classes(L, code, np.stack([lut, lut]), [0, half + 1, half])[0] == -1
 
 
 def run_all(L, with_plane_area_file: bool = True) -> int:
     n = check_prelude(L, with_plane_area_file=with_plane_area_file)
+    n += check_prelude_random(L)
     check_bad_specs(L)
     check_plane_contours(L)
     check_exact_classes(L)

@@ -97,10 +97,13 @@ int hs_exact_classes(const uint8_t* cls, int H, int W, uint8_t* xcls, int* keys,
     return rc;
 }
 
-int hs_fast_classes(const uint16_t* code, long n, const float* lut, uint8_t* c8, float* tab, int* ncls, char* err,
-                    int errlen) {
+// Cells [bounds[g], bounds[g + 1]) read their damping factors from
+// luts[8 g .. 8 g + 8), g < groups.
+int hs_fast_classes(const uint16_t* code, const long* bounds, const float* luts, int groups, uint8_t* c8, float* tab,
+                    int* ncls, char* err, int errlen) {
     std::string e;
-    const int rc = vs::enumerate_fast_classes(code, (size_t)n, lut, c8, tab, *ncls, e);
+    std::vector<size_t> b(bounds, bounds + groups + 1);
+    const int rc = vs::enumerate_fast_classes(code, b.data(), luts, groups, c8, tab, *ncls, e);
     if (rc != VS_OK) put_err(e, err, errlen);
     return rc;
 }

@@ -1,6 +1,7 @@
 """The library's host-only set-up code (vocalsim-2p5d-fdtd_amd/csrc/vs_host.cpp)
-on the CPU: the float32 prelude of every case against the oracle's frame
-dimensions and per-section values, the single-plane contours against the
+on the CPU: the float32 prelude of every case, and of 200 seeded random
+specs, against the oracle's frame dimensions and per-section values, the
+single-plane contours against the
 oracle's, the cell-class enumeration of both step forms, and the refusal of
 bad specs.  The same checks run again on a build with AddressSanitizer and
 UndefinedBehaviorSanitizer, which must finish with no report; a planted
@@ -25,6 +26,10 @@ def host_lib(tmp_path_factory):
 
 def test_prelude_matches_oracle(host_lib):
     assert hsc.check_prelude(host_lib) == 2 * len(hsc.cases())
+
+
+def test_prelude_matches_oracle_on_random_specs(host_lib):
+    assert hsc.check_prelude_random(host_lib) == 400
 
 
 def test_prelude_refuses_what_the_oracle_refuses(host_lib):

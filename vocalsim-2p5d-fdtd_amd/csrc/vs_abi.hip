@@ -428,11 +428,25 @@ int check_handle(const vs_handle* h) {
 }
 
 // Class bytes and class table of the FP32 flux-form kernel from the face
-// codes its setup kernel wrote (enumerate_fast_classes).  The step kernels
-// read s and r from the sign bits of their ax, ay entries.
+// codes its setup kernel wrote (enumerate_fast_classes), each tube's damping
+// codes read through its own table: 1, 0, the wall damping 1 / (1 + dt) and
+// the five PML rings 1 / (1 + sigma_k).  The step kernels read s and r from
+// the sign bits of their ax, ay entries.
 int build_fast_classes(vs_handle* h) {
     FastBufs& f = h->fb;
     const size_t n = (size_t)h->total_cells;
+    // tube k's block of cells (its frame and the alignment gap after it)
+    std::vector<size_t> bounds(h->n + 1, n);
+    std::vector<float> luts(8 * (size_t)h->n);
+    for (int k = 0; k < h->n; ++k) {
+        const TubeDesc& d = h->desc[k];
+        bounds[k] = (size_t)d.cell_off;
+        float* lut = &luts[8 * (size_t)k];
+        lut[0] = 1.0f;
+        lut[1] = 0.0f;
+        lut[2] = (float)(1.0 / (1.0 + d.ph.dt));
+        for (int r = 1; r <= 5; ++r) lut[2 + r] = (float)(1.0 / (1.0 + d.ph.sigma[kPml0 + r]));
+    }
     std::vector<uint16_t> code(n);
     VS_HIP(hipStreamSynchronize(h->stream));
     VS_HIP(copy_on(h->stream, code.data(), f.code, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
@@ -440,7 +454,8 @@ int build_fast_classes(vs_handle* h) {
     std::vector<float> tab(5 * kFastClasses, 0.0f);
     int ncls = 0;
     std::string err;
-    const int rc = enumerate_fast_classes(code.data(), n, f.lut, c8.data(), tab.data(), ncls, err);
+    const int rc = enumerate_fast_classes(code.data(), bounds.data(), luts.data(), h->n, c8.data(), tab.data(),
+                                          ncls, err);
     if (rc != VS_OK) return fail(rc, err);
     VS_HIP(copy_on(h->stream, f.cls8, c8.data(), n, hipMemcpyHostToDevice));
     VS_HIP(copy_on(h->stream, f.ctab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -456,23 +471,19 @@ bool fast_wide_fits(const vs_handle* h, int threads, int* K_out, int* GW_out) {
     if (K == 0) return false;
     const int GW = h->max_wp4;
     if ((long long)(fast_wide_lds_bytes(K, threads, GW) + fast_wide_lds_static_bytes()) > h->lds_limit) return false;
-    for (const TubeDesc& d : h->desc)
-        if (d.srate_mul != h->desc[0].srate_mul) return false;
     if (K_out) *K_out = K;
     if (GW_out) *GW_out = GW;
     return true;
 }
 
 // Whether the batch fits the on-chip kernels at `threads` threads per tube
-// (cells per lane K <= kFastMaxK, LDS within the device limit, one srate_mul).
+// (cells per lane K <= kFastMaxK, LDS within the device limit).
 bool fast_fits(const vs_handle* h, int threads, int* K_out, int* G_out) {
     if (h->precision != VS_FP32 || threads < 128) return false;
     const int K = fast_cells_per_lane(h->max_padded, threads);
     if (K > kFastMaxK) return false;
     const int G = fast_guard_lanes(K, h->max_wp);
     if ((long long)(fast_lds_bytes(K, threads, G) + fast_lds_static_bytes()) > h->lds_limit) return false;
-    for (const TubeDesc& d : h->desc)
-        if (d.srate_mul != h->desc[0].srate_mul) return false;
     if (K_out) *K_out = K;
     if (G_out) *G_out = G;
     return true;
@@ -498,13 +509,6 @@ int prepare_fast(vs_handle* h, int threads) {
             (rc = h->alloc(&f.Uxf, n)) || (rc = h->alloc(&f.Uyf, n)) || (rc = h->alloc(&f.code, n)) ||
             (rc = h->alloc(&f.cls8, n)) || (rc = h->alloc(&f.ctab, 5 * kFastClasses)))
             return rc;
-        const PhysTables& ph = h->desc[0].ph;
-        f.lut[0] = 1.0f;
-        f.lut[1] = 0.0f;
-        f.lut[2] = (float)(1.0 / (1.0 + ph.dt));
-        for (int k = 1; k <= 5; ++k) f.lut[2 + k] = (float)(1.0 / (1.0 + ph.sigma[kPml0 + k]));
-        for (const TubeDesc& d : h->desc)
-            if (d.srate_mul != h->desc[0].srate_mul) return fail(VS_E_INVALID, "one srate_mul per batch on the flux-form kernel");
         h->fast_ready = true;
     }
     {

@@ -79,7 +79,6 @@ struct FastBufs {
     uint16_t* code;                      // damping codes + face-form bits
     uint8_t* cls8;                       // class byte (< kFastClasses): damping triple + face-form bits
     float* ctab;                         // per class: aP, ax (sign: !s), ay (sign: !r), s, r (kFastClasses each)
-    float lut[8];                        // damping factor by code
 };
 
 // Buffers of the reverse-time (adjoint) pass (vs_adjoint.hip).  Per-cell

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <limits>
 
 namespace vs {
@@ -429,31 +430,63 @@ int enumerate_exact_classes(const uint8_t* cls, int H, int W, uint8_t* xcls, int
     return VS_OK;
 }
 
-int enumerate_fast_classes(const uint16_t* code, size_t n, const float lut[8], uint8_t* c8, float* tab, int& ncls,
-                           std::string& err) {
-    std::vector<int> cls_of(2048, -1);
-    const int zero = 1 + 8 * 1 + 64 * 1;
-    cls_of[zero] = 0;
-    ncls = 1;
-    for (size_t i = 0; i < n; ++i) {
-        const int d = (code[i] >> 3) & 0x7ff;
-        if (cls_of[d] < 0) {
-            if (ncls == kFastClasses) {
-                err = "more than " + std::to_string(kFastClasses - 1) + " cell classes in one batch";
-                return VS_E_INVALID;
-            }
-            cls_of[d] = ncls++;
-        }
-        c8[i] = (uint8_t)cls_of[d];
+int enumerate_fast_classes(const uint16_t* code, const size_t* bounds, const float* luts, int groups, uint8_t* c8,
+                           float* tab, int& ncls, std::string& err) {
+    if (groups < 1 || bounds[0] != 0) {
+        err = "fast classes: cell groups must start at cell 0";
+        return VS_E_INVALID;
     }
+    for (int g = 0; g < groups; ++g)
+        if (bounds[g + 1] < bounds[g]) {
+            err = "fast classes: cell group bounds must not decrease";
+            return VS_E_INVALID;
+        }
+    constexpr int kZero = 1 + 8 * 1 + 64 * 1;   // every damping factor zero, no face-form bit
+    // a class is the triple of factors (aP, ax, ay) with the two face-form bits
+    struct Key {
+        float p, x, y;
+        int bits;
+        bool operator==(const Key& o) const {
+            return std::memcmp(&p, &o.p, sizeof p) == 0 && std::memcmp(&x, &o.x, sizeof x) == 0 &&
+                   std::memcmp(&y, &o.y, sizeof y) == 0 && bits == o.bits;
+        }
+    };
+    std::vector<Key> keys;
+    auto key_of = [](const float* lut, int d) {
+        return Key{lut[(d >> 6) & 7], lut[d & 7], lut[(d >> 3) & 7], d & 0x600};
+    };
+    keys.push_back(key_of(luts, kZero));   // class 0 (a zero factor is the same in every table)
+    std::vector<int> cls_of(2048);
+    for (int g = 0; g < groups; ++g) {
+        const float* lut = luts + 8 * g;
+        std::fill(cls_of.begin(), cls_of.end(), -1);
+        cls_of[kZero] = 0;
+        for (size_t i = bounds[g]; i < bounds[g + 1]; ++i) {
+            const int d = (code[i] >> 3) & 0x7ff;
+            if (cls_of[d] < 0) {
+                const Key k = key_of(lut, d);
+                int c = 0;
+                while (c < (int)keys.size() && !(keys[c] == k)) ++c;
+                if (c == (int)keys.size()) {
+                    if (c == kFastClasses) {
+                        err = "more than " + std::to_string(kFastClasses - 1) + " cell classes in one batch";
+                        return VS_E_INVALID;
+                    }
+                    keys.push_back(k);
+                }
+                cls_of[d] = c;
+            }
+            c8[i] = (uint8_t)cls_of[d];
+        }
+    }
+    ncls = (int)keys.size();
     std::fill(tab, tab + 5 * kFastClasses, 0.0f);
-    for (int d = 0; d < 2048; ++d) {
-        const int c = cls_of[d];
-        if (c < 0) continue;
-        const bool s0 = (d & 0x200) != 0, r0 = (d & 0x400) != 0;
-        tab[c] = lut[(d >> 6) & 7];
-        tab[kFastClasses + c] = s0 ? -lut[d & 7] : lut[d & 7];
-        tab[2 * kFastClasses + c] = r0 ? -lut[(d >> 3) & 7] : lut[(d >> 3) & 7];
+    for (int c = 0; c < ncls; ++c) {
+        const Key& k = keys[c];
+        const bool s0 = (k.bits & 0x200) != 0, r0 = (k.bits & 0x400) != 0;
+        tab[c] = k.p;
+        tab[kFastClasses + c] = s0 ? -k.x : k.x;
+        tab[2 * kFastClasses + c] = r0 ? -k.y : k.y;
         tab[3 * kFastClasses + c] = s0 ? 0.0f : 1.0f;
         tab[4 * kFastClasses + c] = r0 ? 0.0f : 1.0f;
     }

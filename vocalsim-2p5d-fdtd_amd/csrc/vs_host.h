@@ -107,19 +107,24 @@ constexpr int kExactWallTerms = 2;     //             the class carries a wall s
 // the frame has more than kExactClasses keys.
 int enumerate_exact_classes(const uint8_t* cls, int H, int W, uint8_t* xcls, int* keys, int& nk, std::string& err);
 
-// Cell classes of the FP32 flux-form step: the distinct damping triples and
-// face-form bits of the batch (bits [3, 14) of the 16-bit face codes:
-// code_x + 8 code_y + 64 code_p, bit 12 self-off, bit 13 source face) become
-// classes 1 .. kFastClasses - 1 in first-seen order; class 0 is every damping
-// factor zero with no face-form bit (the pad cells).  Per class, `tab` (5 x
-// kFastClasses floats) holds aP, ax, ay and two 0/1 factors: s = 1 unless
-// the cell's own pressure is left out of its face updates (walls carrying an
-// admittance, excitation) and r = 1 unless the x-face carries the source
-// instead of the right neighbour's pressure (excitation).  ax and ay carry
-// s = 0, resp. r = 0, in their sign bits.  `lut` maps a 3-bit damping code
-// to its factor.  VS_E_INVALID when the batch has more classes.
+// Cell classes of the FP32 flux-form step: the distinct damping-factor
+// triples and face-form bits of the batch become classes 1 .. kFastClasses - 1
+// in first-seen order; class 0 is every damping factor zero with no
+// face-form bit (the pad cells).  A cell's 16-bit face code holds, in bits
+// [3, 14), code_x + 8 code_y + 64 code_p (3-bit damping codes), bit 12
+// self-off and bit 13 source face; the cells [bounds[g], bounds[g + 1]) of
+// group g (a tube) map damping codes to factors through luts[8 g .. 8 g + 8)
+// (the wall damping 1 / (1 + dt) depends on the tube's sample rate), so
+// cells of tubes with different tables share a class only where their
+// factors are equal.  Per class, `tab` (5 x kFastClasses floats) holds aP,
+// ax, ay and two 0/1 factors: s = 1 unless the cell's own pressure is left
+// out of its face updates (walls carrying an admittance, excitation) and
+// r = 1 unless the x-face carries the source instead of the right
+// neighbour's pressure (excitation).  ax and ay carry s = 0, resp. r = 0, in
+// their sign bits.  VS_E_INVALID when the batch has more classes or the
+// bounds do not run from 0 upwards.
 constexpr int kFastClasses = 64;
-int enumerate_fast_classes(const uint16_t* code, size_t n, const float lut[8], uint8_t* c8, float* tab, int& ncls,
-                           std::string& err);
+int enumerate_fast_classes(const uint16_t* code, const size_t* bounds, const float* luts, int groups, uint8_t* c8,
+                           float* tab, int& ncls, std::string& err);
 
 }  // namespace vs
