@@ -23,11 +23,11 @@ HBM_PEAK_GBS = 8000.0
 # code.  Odd-run form: reads P, Uy, Uy below, aP, P above, P right (4 B
 # each), (ax, ay) (8 B), the old Uy (4 B); writes P', Uy' (4 B each).  Wide
 # form (K = 4 mod 8): reads P, Uy, Uy below, P, P above, Uy (4 B each, as
-# 16-byte vectors), aP (4 B), (ax, ay, s, nr) (16 B), a run end per K cells;
-# writes P', Uy' (4 B each): 67 840 B per wave and step at K = 20, i.e. 53 B
-# per cell slot.  Rated on real cells only; the peak is measured live
-# (vs_measure_lds_bandwidth).
-LDS_BYTES_PER_CELL = {"odd": 44, "wide": 53}
+# 16-byte vectors), per pair of cells (aP, aP) (8 B) and (ax, ay, ax, ay)
+# (16 B), a run end per K cells; writes P', Uy' (4 B each): 57 600 B per
+# wave and step at K = 20, i.e. 45 B per cell slot.  Rated on real cells
+# only; the peak is measured live (vs_measure_lds_bandwidth).
+LDS_BYTES_PER_CELL = {"odd": 44, "wide": 45}
 # algorithmic flops per cell-update (pressure 15, Vx 18, Vy 18; counted from
 # FDTDEngineCPU.cpp:320-405) and MI355X's published dense FP32 vector peak
 FLOPS_PER_CELL = 51
@@ -202,7 +202,8 @@ def main():
     dims = s.dims
     layout_k = s.get_option(_abi.OPT_FAST_LAYOUT_K) if kernel == _abi.K_FAST else 0
     layout = "wide" if layout_k % 8 == 4 else "odd"
-    log(f"[bench] on-chip layout: {layout} form, K = {layout_k}")
+    pair_classes = s.get_option(_abi.OPT_FAST_PAIR_CLASSES) if kernel == _abi.K_FAST else 0
+    log(f"[bench] on-chip layout: {layout} form, K = {layout_k}, {pair_classes} pair classes")
     log(f"[bench] interior cells per step: {cells}, threads {threads}")
     # 256-step blocks replayed as hipGraphs of GRAPH_BLOCKS launches (the
     # remainder, if any, as ordinary launches); the warm-up captures the graph
@@ -375,7 +376,8 @@ def main():
                                f"half Gaussian / half two-mass, FP32 state, 256-step blocks replayed as "
                                f"hipGraphs of {GRAPH_BLOCKS}",
                    "tubes": args.tubes, "cells_per_step": cells, "block_steps": BLOCK,
-                   "graph_blocks": GRAPH_BLOCKS, "kernel": kernel, "threads": threads},
+                   "graph_blocks": GRAPH_BLOCKS, "kernel": kernel, "threads": threads,
+                   "cells_per_lane": layout_k, "pair_classes": pair_classes},
         "fp64_plain": {"value": fp64_rate, "cells_per_step": c64,
                        "workload": "44-section vowel tube, two-mass 800 Pa, FP64, one tube"},
         "ratio_batched_fp32_over_plain_fp64": value / fp64_rate if fp64_rate else None,

@@ -154,10 +154,14 @@ enum vs_option {
                                     accesses) whenever the batch fits it at the work-group size,
                                     else the odd-run form; 0: always the odd-run form.  Both
                                     give the same values (signed zeros aside). */
-    VS_OPT_FAST_LAYOUT_K = 12    /* read-only (vs_get_option): cells per lane of the on-chip
+    VS_OPT_FAST_LAYOUT_K = 12,   /* read-only (vs_get_option): cells per lane of the on-chip
                                     kernel the current work-group size runs, 4 (mod 8) for the
                                     wide-access form, odd for the odd-run form, 0 when neither
                                     fits the batch */
+    VS_OPT_FAST_PAIR_CLASSES = 13 /* read-only (vs_get_option): pair classes of the batch (the
+                                    ordered class pairs of neighbouring cells the wide-access form
+                                    reads its coefficients by), 0 before the on-chip kernel's
+                                    set-up has run; above 128 the batch runs the odd-run form */
 };
 
 enum vs_kernel {

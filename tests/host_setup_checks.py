@@ -54,6 +54,8 @@ def load(path: str):
                                    _c.c_char_p, _c.c_int]
     L.hs_fast_classes.argtypes = [_P(_c.c_uint16), _P(_c.c_long), _P(_c.c_float), _c.c_int, _P(_c.c_uint8),
                                   _P(_c.c_float), _P(_c.c_int), _c.c_char_p, _c.c_int]
+    L.hs_fast_pairs.argtypes = [_P(_c.c_uint8), _P(_c.c_float), _P(_c.c_int), _P(_c.c_int), _P(_c.c_longlong),
+                                _c.c_int, _P(_c.c_uint8), _P(_c.c_float), _P(_c.c_int), _c.c_char_p, _c.c_int]
     L.hs_planted_overflow.argtypes = [_c.c_int]
     return L
 
@@ -111,6 +113,27 @@ def fast_classes(L, code, lut, bounds=None):
     rc = L.hs_fast_classes(_ptr(code, _c.c_uint16), _ptr(b, _c.c_long), _ptr(luts, _c.c_float), len(b) - 1,
                            _ptr(c8, _c.c_uint8), _ptr(tab, _c.c_float), ctypes.byref(ncls), err, 256)
     return rc, c8, tab.reshape(5, nf), ncls.value, err.value.decode()
+
+
+def fast_pairs(L, c8, tab, frames, fill=0xEE):
+    """Pair classes over tube frames [(H, W, offset)] of the class bytes c8
+    with the class table tab (5 x kFastClasses): (rc, p8, face entries
+    [kFastPairs, 4], pressure entries [kFastPairs, 2], npairs, message).
+    p8 starts as `fill` everywhere."""
+    c8 = np.ascontiguousarray(c8, dtype=np.uint8)
+    tab = np.ascontiguousarray(tab, dtype=np.float32)
+    H = np.ascontiguousarray([f[0] for f in frames], dtype=np.int32)
+    W = np.ascontiguousarray([f[1] for f in frames], dtype=np.int32)
+    off = np.ascontiguousarray([f[2] for f in frames], dtype=np.int64)
+    p8 = np.full(c8.size, fill, dtype=np.uint8)
+    npmax = L.hs_fast_pairs_max()
+    ptab = np.full(6 * npmax, np.nan, dtype=np.float32)
+    n = ctypes.c_int(0)
+    err = ctypes.create_string_buffer(256)
+    rc = L.hs_fast_pairs(_ptr(c8, _c.c_uint8), _ptr(tab, _c.c_float), _ptr(H, _c.c_int), _ptr(W, _c.c_int),
+                         _ptr(off, _c.c_longlong), len(frames), _ptr(p8, _c.c_uint8), _ptr(ptab, _c.c_float),
+                         ctypes.byref(n), err, 256)
+    return rc, p8, ptab[:4 * npmax].reshape(npmax, 4), ptab[4 * npmax:].reshape(npmax, 2), n.value, err.value.decode()
 
 
 # ---------------------------------------------------------------------------
@@ -358,6 +381,59 @@ def check_fast_classes(L):
     assert fast_classes(L, code, np.stack([lut, lut]), [0, half + 1, half])[0] == -1
 
 
+def check_fast_pairs(L):
+    """Pair classes of the wide-access form: over each tube's padded rows
+    (interior width rounded up to a multiple of 4 past one pad cell), the
+    cells 2i, 2i + 1 of a row form a pair; a pair class is the ordered pair
+    of class bytes (class 0 beyond the interior), numbered in first-seen
+    order from pair class 0 = (0, 0), stored at the frame position of the
+    pair's first cell when that cell is an interior cell, with the table
+    entries of both cells copied bit for bit."""
+    rng = np.random.default_rng(7)
+    nf = L.hs_fast_max()
+    npmax = L.hs_fast_pairs_max()
+    tab = rng.uniform(-1.0, 1.0, (5, nf)).astype(np.float32)
+    tab[:, 0] = 0.0
+    tab[1, 5] = -0.0   # a signed zero (s = 0 with a zero factor) must survive the copy
+    frames, size = [], 0
+    for H, W in [(3, 3), (5, 4), (7, 9), (12, 17), (4, 30), (9, 6)]:
+        frames.append((H, W, size))
+        size += H * W + int(rng.integers(0, 70))   # gaps between tubes, as the batch layout aligns them
+    c8 = rng.integers(0, 9, size).astype(np.uint8)   # few classes, so pairs repeat
+    rc, p8, face, pres, npairs, msg = fast_pairs(L, c8, tab, frames)
+    assert rc == 0, msg
+    seen = {(0, 0): 0}
+    written = np.zeros(size, dtype=bool)
+    bits = lambda a: np.asarray(a, np.float32).view(np.uint32)
+    for H, W, off in frames:
+        Hi, Wi = H - 2, W - 2
+        Wp = (Wi + 1 + 3) // 4 * 4
+        assert Wp % 4 == 0 and Wp > Wi
+        for r in range(Hi):
+            for c in range(0, Wp, 2):
+                q = off + (r + 1) * W + (c + 1)
+                a = int(c8[q]) if c < Wi else 0
+                b = int(c8[q + 1]) if c + 1 < Wi else 0
+                p = seen.setdefault((a, b), len(seen))
+                if c < Wi:
+                    assert p8[q] == p, (H, W, r, c, p8[q], p)
+                    written[q] = True
+                assert np.array_equal(bits(face[p]), bits([tab[1, a], tab[2, a], tab[1, b], tab[2, b]])), (a, b)
+                assert np.array_equal(bits(pres[p]), bits([tab[0, a], tab[0, b]])), (a, b)
+    assert npairs == len(seen) <= npmax
+    assert np.all(p8[~written] == 0xEE), "a position that is not a pair's first interior cell was written"
+    assert np.isnan(face[npairs:]).sum() == 0 and not face[npairs:].any() and not pres[npairs:].any()
+    # more pair classes than the tables hold: still VS_OK, with the count
+    huge = rng.integers(0, 64, 40 * 130).astype(np.uint8)
+    rc, _, _, _, n_huge, msg = fast_pairs(L, huge, tab, [(40, 130, 0)])
+    assert rc == 0 and n_huge > npmax, (rc, n_huge, msg)
+    # class bytes must be classes; frames at least 3 x 3
+    bad = c8.copy()
+    bad[frames[2][2] + frames[2][1] + 1] = nf
+    assert fast_pairs(L, bad, tab, frames)[0] == -1
+    assert fast_pairs(L, c8, tab, [(2, 5, 0)])[0] == -1
+
+
 def run_all(L, with_plane_area_file: bool = True) -> int:
     n = check_prelude(L, with_plane_area_file=with_plane_area_file)
     n += check_prelude_random(L)
@@ -365,6 +441,7 @@ def run_all(L, with_plane_area_file: bool = True) -> int:
     check_plane_contours(L)
     check_exact_classes(L)
     check_fast_classes(L)
+    check_fast_pairs(L)
     return n
 
 

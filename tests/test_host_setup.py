@@ -48,6 +48,10 @@ def test_fast_classes(host_lib):
     hsc.check_fast_classes(host_lib)
 
 
+def test_fast_pair_classes(host_lib):
+    hsc.check_fast_pairs(host_lib)
+
+
 def _sanitizer_libs():
     libs = []
     for name in ("libasan.so", "libubsan.so"):

@@ -130,6 +130,7 @@ struct vs_handle {
     bool fast_ready = false;   // coefficients built
     bool flux_valid = false;   // Uxf/Uyf hold the current state
     bool fast_classes = false; // class bytes and tables built
+    int fast_pairs = 0;        // pair classes of the wide-access form (0 until the class bytes are built)
     int max_interior = 0;      // largest (H-2)*(W-2) of the batch
     int max_padded = 0;        // largest (H-2)*(W-1): interior with one pad column per row
     int max_wp = 0;            // widest padded row W-1
@@ -454,11 +455,29 @@ int build_fast_classes(vs_handle* h) {
     std::vector<float> tab(5 * kFastClasses, 0.0f);
     int ncls = 0;
     std::string err;
-    const int rc = enumerate_fast_classes(code.data(), bounds.data(), luts.data(), h->n, c8.data(), tab.data(),
-                                          ncls, err);
+    int rc = enumerate_fast_classes(code.data(), bounds.data(), luts.data(), h->n, c8.data(), tab.data(), ncls, err);
     if (rc != VS_OK) return fail(rc, err);
     VS_HIP(copy_on(h->stream, f.cls8, c8.data(), n, hipMemcpyHostToDevice));
     VS_HIP(copy_on(h->stream, f.ctab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    // pair classes of the wide-access form (used when there are at most kFastPairs)
+    std::vector<int> Hs(h->n), Ws(h->n);
+    std::vector<long long> offs(h->n);
+    for (int k = 0; k < h->n; ++k) {
+        Hs[k] = h->desc[k].H;
+        Ws[k] = h->desc[k].W;
+        offs[k] = h->desc[k].cell_off;
+    }
+    std::vector<uint8_t> p8(n, 0);
+    std::vector<float> ptab(6 * kFastPairs, 0.0f);
+    int npairs = 0;
+    rc = enumerate_fast_pairs(c8.data(), tab.data(), Hs.data(), Ws.data(), offs.data(), h->n, p8.data(), ptab.data(),
+                              npairs, err);
+    if (rc != VS_OK) return fail(rc, err);
+    h->fast_pairs = npairs;
+    if (npairs <= kFastPairs) {
+        VS_HIP(copy_on(h->stream, f.pcls8, p8.data(), n, hipMemcpyHostToDevice));
+        VS_HIP(copy_on(h->stream, f.ptab, ptab.data(), ptab.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     return VS_OK;
 }
 
@@ -467,6 +486,8 @@ int build_fast_classes(vs_handle* h) {
 // the guard is in floats.
 bool fast_wide_fits(const vs_handle* h, int threads, int* K_out, int* GW_out) {
     if (!h->fast_wide || h->precision != VS_FP32 || threads < 128) return false;
+    // the pair tables hold kFastPairs pair classes (known once the class bytes are built)
+    if (h->fast_classes && h->fast_pairs > kFastPairs) return false;
     const int K = fast_wide_cells_per_lane(h->max_padded4, threads);
     if (K == 0) return false;
     const int GW = h->max_wp4;
@@ -493,27 +514,25 @@ bool fast_fits(const vs_handle* h, int threads, int* K_out, int* G_out) {
 int prepare_fast(vs_handle* h, int threads) {
     if (h->precision != VS_FP32) return fail(VS_E_INVALID, "the flux-form kernel is FP32 only");
     if (threads < 128) return fail(VS_E_INVALID, "the on-chip kernel needs at least 128 threads");
-    int Kw = 0;
-    const bool wide = fast_wide_fits(h, threads, &Kw, nullptr);
     const int K = fast_cells_per_lane(h->max_padded, threads);
-    if (!wide && K > kFastMaxK) return fail(VS_E_INVALID, "tube too large for the on-chip kernel at this work-group size");
     const int G = fast_guard_lanes(K, h->max_wp);
-    if (!wide && (long long)(fast_lds_bytes(K, threads, G) + fast_lds_static_bytes()) > h->lds_limit)
+    const bool odd_fits = K <= kFastMaxK && (long long)(fast_lds_bytes(K, threads, G) + fast_lds_static_bytes()) <= h->lds_limit;
+    // before the class bytes exist, fast_wide_fits judges the size only
+    if (!odd_fits && !fast_wide_fits(h, threads, nullptr, nullptr)) {
+        if (K > kFastMaxK) return fail(VS_E_INVALID, "tube too large for the on-chip kernel at this work-group size");
         return fail(VS_E_INVALID, "on-chip kernel needs " + std::to_string(fast_lds_bytes(K, threads, G)) +
                     " bytes of LDS, device allows " + std::to_string(h->lds_limit));
+    }
     if (!h->fast_ready) {
         FastBufs& f = h->fb;
         const long long n = h->total_cells;
         int rc;
         if ((rc = h->alloc(&f.kP, n)) || (rc = h->alloc(&f.gX, n)) || (rc = h->alloc(&f.gY, n)) ||
             (rc = h->alloc(&f.Uxf, n)) || (rc = h->alloc(&f.Uyf, n)) || (rc = h->alloc(&f.code, n)) ||
-            (rc = h->alloc(&f.cls8, n)) || (rc = h->alloc(&f.ctab, 5 * kFastClasses)))
+            (rc = h->alloc(&f.cls8, n)) || (rc = h->alloc(&f.ctab, 5 * kFastClasses)) ||
+            (rc = h->alloc(&f.pcls8, n)) || (rc = h->alloc(&f.ptab, 6 * kFastPairs)))
             return rc;
         h->fast_ready = true;
-    }
-    {
-        hipError_t e = wide ? configure_fast_wide(Kw) : configure_fast(K);
-        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("configure: ") + hipGetErrorString(e));
     }
     if (!h->flux_valid) {
         hipError_t e = launch_fast_setup(h->d_desc, h->desc.data(), h->n, h->bf, h->fb, h->stream);
@@ -525,6 +544,14 @@ int prepare_fast(vs_handle* h, int threads) {
         }
         h->flux_valid = true;
     }
+    // the wide-access form also needs the batch's pair classes to fit its tables
+    int Kw = 0;
+    const bool wide = fast_wide_fits(h, threads, &Kw, nullptr);
+    if (!wide && !odd_fits)
+        return fail(VS_E_INVALID, "batch has " + std::to_string(h->fast_pairs) + " pair classes (the wide-access form holds " +
+                    std::to_string(kFastPairs) + ") and is too large for the odd-run form at this work-group size");
+    hipError_t e = wide ? configure_fast_wide(Kw) : configure_fast(K);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("configure: ") + hipGetErrorString(e));
     return VS_OK;
 }
 
@@ -1097,6 +1124,7 @@ int vs_set_option(vs_handle* h, int option, int value) {
             if (value < 0) return fail(VS_E_INVALID, "barrier spin must be >= 0");
             h->barrier_spin = value; return VS_OK;
         case VS_OPT_FAST_LAYOUT_K: return fail(VS_E_INVALID, "VS_OPT_FAST_LAYOUT_K is read-only");
+        case VS_OPT_FAST_PAIR_CLASSES: return fail(VS_E_INVALID, "VS_OPT_FAST_PAIR_CLASSES is read-only");
         case VS_OPT_FAST_WIDE:
             if (value != 0 && value != 1) return fail(VS_E_INVALID, "fast wide must be 0 or 1");
             h->fast_wide = value; return VS_OK;
@@ -1135,6 +1163,7 @@ int vs_get_option(const vs_handle* h, int option, int* value) {
         case VS_OPT_EXACT_STEPS: *value = h->exact_steps; return VS_OK;
         case VS_OPT_BARRIER_SPIN: *value = h->barrier_spin; return VS_OK;
         case VS_OPT_FAST_WIDE: *value = h->fast_wide; return VS_OK;
+        case VS_OPT_FAST_PAIR_CLASSES: *value = h->fast_classes ? h->fast_pairs : 0; return VS_OK;
         case VS_OPT_FAST_LAYOUT_K: {
             int Kw = 0, K = 0;
             if (fast_wide_fits(h, h->threads, &Kw, nullptr)) *value = Kw;

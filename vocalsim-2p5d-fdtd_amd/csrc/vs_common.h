@@ -79,6 +79,8 @@ struct FastBufs {
     uint16_t* code;                      // damping codes + face-form bits
     uint8_t* cls8;                       // class byte (< kFastClasses): damping triple + face-form bits
     float* ctab;                         // per class: aP, ax (sign: !s), ay (sign: !r), s, r (kFastClasses each)
+    uint8_t* pcls8;                      // wide form: pair class at the frame position of a pair's first cell
+    float* ptab;                         // per pair class: (ax, ay, ax, ay) x kFastPairs, then (aP, aP) x kFastPairs
 };
 
 // Buffers of the reverse-time (adjoint) pass (vs_adjoint.hip).  Per-cell
@@ -156,9 +158,8 @@ hipError_t launch_dtft(const double* d_x, long long stride, int nsig, int n, dou
 // LDS bytes of a launch.
 constexpr int kFastMaxK = 23;
 // Wide-access variant (k_fast_wide): K = 4 (mod 8) up to kFastWideMaxK, rows
-// padded to a multiple of 4 cells.
+// padded to a multiple of 4 cells (fast_wide_wp, vs_host.h).
 constexpr int kFastWideMaxK = 20;
-__host__ __device__ inline int fast_wide_wp(int interior_width) { return (interior_width + 1 + 3) & ~3; }
 int fast_wide_cells_per_lane(int max_padded_cells, int threads);
 size_t fast_wide_lds_bytes(int K, int threads, int guard_floats);
 size_t fast_wide_lds_static_bytes();

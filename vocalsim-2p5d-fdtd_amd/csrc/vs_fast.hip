@@ -488,36 +488,56 @@ __global__ __launch_bounds__(1024) void k_fast_step(const TubeDesc* __restrict__
 // lane run (t K words) and every row offset is 16-byte aligned: the run's own
 // P and Uy, the Uy of the cells below and the P of the cells above move as
 // ds_read_b128 / ds_write_b128, and with K / 4 odd the 16 lanes that one
-// b128 access serves per pass cover all 64 banks.  Guards of GW >= Wp zero
-// floats on both sides of the two regions keep every neighbour read of a
-// real or pad slot inside the region.  The class table holds (ax, ay, s, nr)
-// with s the cell's own-pressure weight and nr = 1 - r the source weight of
-// its x-face, so the face update needs no selects:
-//   fx = fma(nr, ev, fma(s, P, -fma(-nr, P_right, P_right))),  fy = fma(s, P, -P_top)
-// which for s, nr in {0, 1} rounds exactly as the selects of fast_faces, and
-// the step is written with the same fused operations as the contracted code
-// of k_fast_step, so both kernels give the same values (signed zeros aside).
+// b128 access serves per pass cover all 64 banks.  A zero guard of GW >= Wp
+// floats before the P region (the row above row 0), 4 floats after it (the
+// next lane's first P read by the last lane) and GW after the Uy region (the
+// row below the last row) keep every neighbour read of a real or pad slot
+// inside LDS.
+//
+// Class tables by cell pair.  K is even and runs start at even j, so the
+// cells j, j + 1 (j even) of a run form pairs that never cross a row.  A
+// pair's class byte indexes two tables: the face entry (ax, ay of both
+// cells, s = 0 and r = 0 in the sign bits, one 16-byte read per pair) and
+// the pressure entry (aP of both cells, one 8-byte read per pair).  Per cell
+// and step that is 12 bytes of table reads in two accesses per pair instead
+// of 20 bytes in four, and the LDS is the pipe this kernel waits on.  The
+// face update takes s and r from the sign bits as fast_faces does,
+//   fx = (s ? P : 0) - (r ? P_right : -ev),  fy = (s ? P : 0) - P_top,
+// and the step is written with the same fused operations as the contracted
+// code of k_fast_step, so both kernels give the same values (signed zeros
+// aside).
 // ---------------------------------------------------------------------------
 typedef float f4v __attribute__((ext_vector_type(4)));
 
+// Copies a two-mass state word by word with the 64 lanes of a wave.
+__device__ inline void copy_words(float* __restrict__ dst, const float* __restrict__ src, int lane) {
+    static_assert(sizeof(TwoMassState) % sizeof(float) == 0, "two-mass state copied in words");
+    for (int i = lane; i < (int)(sizeof(TwoMassState) / sizeof(float)); i += 64) dst[i] = src[i];
+}
+
+// Pair class of the run's pair h (cells 2h, 2h + 1): byte h of the packed words.
+template <int K>
+__device__ inline uint32_t pair_of(const uint32_t (&cp)[(K / 2 + 3) / 4], int h) {
+    return (cp[h >> 2] >> (8 * (h & 3))) & 0xffu;
+}
+
 template <int K>
 __device__ __forceinline__ void wide_pressure(float* __restrict__ P, const float* __restrict__ UY,
-                                              const float* __restrict__ UYB, const float* __restrict__ ptab,
+                                              const float* __restrict__ UYB, const float2* __restrict__ ptab2,
                                               const float (&Ux)[K], const float (&kP)[K],
-                                              const uint32_t (&cw)[(K + 3) / 4], float uxl0) {
+                                              const uint32_t (&cp)[(K / 2 + 3) / 4], float uxl0) {
 #pragma clang fp contract(off)
     f4v* const P4 = reinterpret_cast<f4v*>(P);
     const f4v* const U4 = reinterpret_cast<const f4v*>(UY);
     const f4v* const B4 = reinterpret_cast<const f4v*>(UYB);
 #pragma unroll
     for (int m = 0; m < K / 4; ++m) {
-        // a group's class-table reads go out with its state reads, so the
-        // group waits for LDS once
-        float aP[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) aP[i] = ptab[class_of<K>(cw, 4 * m + i)];
+        // a group's table reads (aP of its two pairs) go out with its state
+        // reads, so the group waits for LDS once
+        const float2 a01 = ptab2[pair_of<K>(cp, 2 * m)], a23 = ptab2[pair_of<K>(cp, 2 * m + 1)];
         const f4v p = P4[m], uy = U4[m], uyb = B4[m];
         __builtin_amdgcn_sched_barrier(0);
+        const float aP[4] = {a01.x, a01.y, a23.x, a23.y};
         f4v pn;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -535,7 +555,7 @@ __device__ __forceinline__ void wide_faces(const float* __restrict__ P, const fl
                                            float* __restrict__ UY, float* __restrict__ UXE,
                                            const f4v* __restrict__ ftab4, float (&Ux)[K],
                                            const float (&gX)[K], const float (&gY)[K],
-                                           const uint32_t (&cw)[(K + 3) / 4], float ev) {
+                                           const uint32_t (&cp)[(K / 2 + 3) / 4], float ev) {
 #pragma clang fp contract(off)
     const f4v* const P4 = reinterpret_cast<const f4v*>(P);
     const f4v* const A4 = reinterpret_cast<const f4v*>(PA);
@@ -543,28 +563,27 @@ __device__ __forceinline__ void wide_faces(const float* __restrict__ P, const fl
     f4v p = P4[0];
 #pragma unroll
     for (int m = 0; m < K / 4; ++m) {
-        // the group's class-table and state reads go out together (one wait);
-        // P of the next four cells, past the run the next lane's first cell
-        f4v c4[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) c4[i] = ftab4[class_of<K>(cw, 4 * m + i)];   // ax, ay, s, nr
+        // the group's table and state reads go out together (one wait); P of
+        // the next four cells, past the run the next lane's first cell
+        const f4v e01 = ftab4[pair_of<K>(cp, 2 * m)], e23 = ftab4[pair_of<K>(cp, 2 * m + 1)];
         f4v pnext;
         if (m + 1 < K / 4) pnext = P4[m + 1];
         else pnext[0] = P[K];
         const f4v pa = A4[m], uy = U4[m];
         __builtin_amdgcn_sched_barrier(0);
+        const float ax[4] = {e01[0], e01[2], e23[0], e23[2]};   // s = 0 in the sign bit
+        const float ay[4] = {e01[1], e01[3], e23[1], e23[3]};   // r = 0 in the sign bit
         f4v un;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int k = 4 * m + i;
-            const float pc = p[i];
             const float pr = i < 3 ? p[i + 1] : pnext[0];
-            const f4v c = c4[i];
-            const float rpr = __builtin_fmaf(-c[3], pr, pr);
-            const float fx = __builtin_fmaf(c[3], ev, __builtin_fmaf(c[2], pc, -rpr));
-            const float fy = __builtin_fmaf(c[2], pc, -pa[i]);
-            Ux[k] = __builtin_fmaf(Ux[k], c[0], gX[k] * fx);
-            un[i] = __builtin_fmaf(c[1], uy[i], gY[k] * fy);
+            // walls with admittance (s = 0): U = -g P_neighbour; excitation (s = r = 0): Ux = g exeV
+            const float ps = flag_off(ax[i]) ? 0.f : p[i];
+            const float prx = flag_off(ay[i]) ? -ev : pr;
+            const float fx = ps - prx, fy = ps - pa[i];
+            Ux[k] = __builtin_fmaf(Ux[k], fabsf(ax[i]), gX[k] * fx);
+            un[i] = __builtin_fmaf(fabsf(ay[i]), uy[i], gY[k] * fy);
         }
         U4[m] = un;
         p = pnext;
@@ -580,15 +599,15 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
     static_assert(K % 8 == 4, "the wide kernel needs K = 4 (mod 8)");
     extern __shared__ f4v lds4[];
     float* const lds = reinterpret_cast<float*>(lds4);
-    __shared__ f4v ftab4[kFastClasses];          // (ax, ay, s, nr) by class
-    __shared__ float ptab[kFastClasses];         // aP by class
+    __shared__ f4v ftab4[kFastPairs];            // (ax, ay, ax, ay) by pair class, s = 0 / r = 0 in the signs
+    __shared__ float2 ptab2[kFastPairs];         // (aP, aP) by pair class
     __shared__ double sexe[2];                   // source value of steps t, t + 1
     __shared__ TwoMassState tm;
     const int T = blockDim.x - 64, t = threadIdx.x;
     const int KT = K * T;
-    float* const Pg = lds + GW;                  // P[j], j in [-GW, KT + GW)
-    float* const Ug = lds + KT + 3 * GW;         // Uy[j]
-    float* const Xg = lds + 2 * KT + 4 * GW + 4; // Ux of lane l's last cell at Xg[l], Xg[-1] = 0
+    float* const Pg = lds + GW;                  // P[j], j in [-GW, KT + 4)
+    float* const Ug = Pg + KT + 4;               // Uy[j], j in [0, KT + GW)
+    float* const Xg = Ug + KT + GW + 4;          // Ux of lane l's last cell at Xg[l], Xg[-1] = 0
 
     const int tube = order[blockIdx.x];
     const TubeDesc& D = tubes[tube];
@@ -598,21 +617,19 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
     const int tend = min(t0 + nsteps, D.num_steps);
     const long long co = D.cell_off;
 
-    // zero guards: GW floats before and after both regions, and Xg[-1]
-    for (int i = t; i < 4 * GW + 1; i += blockDim.x) {
-        if (i < 4 * GW) {
-            const int r = i / (2 * GW), g = i - r * 2 * GW;
-            float* const base = r == 0 ? Pg : Ug;
-            base[g < GW ? g - GW : KT + (g - GW)] = 0.f;
-        } else {
-            Xg[-1] = 0.f;
-        }
+    // zero guards: GW floats before the P region and 4 after it, GW after
+    // the Uy region, and Xg[-1]
+    for (int i = t; i < 2 * GW + 5; i += blockDim.x) {
+        if (i < GW) Pg[i - GW] = 0.f;
+        else if (i < GW + 4) Pg[KT + (i - GW)] = 0.f;
+        else if (i < 2 * GW + 4) Ug[KT + (i - GW - 4)] = 0.f;
+        else Xg[-1] = 0.f;
     }
 
     const int jb = t * K;
     const bool cell_lane = t < T;
     float Ux[K], kP[K], gX[K], gY[K];
-    uint32_t cw[(K + 3) / 4];
+    uint32_t cp[(K / 2 + 3) / 4];   // pair class bytes of the run
     auto stage2 = [&](const float* __restrict__ a, const float* __restrict__ c2) {
         for_padded(KT, Wp, [&](int j, int r, int c) {
             float va = 0.f, vb = 0.f;
@@ -644,16 +661,18 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
         }
     }
     __syncthreads();
+    // pair classes sit at the frame position of each pair's first cell (even
+    // c); pairs of pad cells (c >= Wi) are pair class 0
     for_padded(KT, Wp, [&](int j, int r, int c) {
-        const uint32_t cd = (r < Hi && c < Wi) ? (uint32_t)f.cls8[co + (long long)(r + 1) * W + (c + 1)] : 0u;
+        const uint32_t cd = (r < Hi && c < Wi) ? (uint32_t)f.pcls8[co + (long long)(r + 1) * W + (c + 1)] : 0u;
         Ug[j] = __uint_as_float(cd);
     });
     __syncthreads();
     if (cell_lane) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint32_t cd = __float_as_uint(Ug[jb + k]);
-            if (k & 3) cw[k >> 2] |= cd << (8 * (k & 3)); else cw[k >> 2] = cd;
+        for (int h = 0; h < K / 2; ++h) {
+            const uint32_t cd = __float_as_uint(Ug[jb + 2 * h]);
+            if (h & 3) cp[h >> 2] |= cd << (8 * (h & 3)); else cp[h >> 2] = cd;
         }
     }
     __syncthreads();
@@ -682,15 +701,12 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
         const float* pL = Pg + jL;
         const float* pE = Pg + jE;
         double pexc = 0.0;
-        {
-            const float ax = f.ctab[kFastClasses + lane], ay = f.ctab[2 * kFastClasses + lane];
-            f4v c;
-            c[0] = fabsf(ax);
-            c[1] = fabsf(ay);
-            c[2] = flag_off(ax) ? 0.f : 1.f;   // s
-            c[3] = flag_off(ay) ? 1.f : 0.f;   // 1 - r
-            ftab4[lane] = c;
-            ptab[lane] = f.ctab[lane];
+        for (int q = lane; q < kFastPairs; q += 64) {
+            f4v e;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) e[i] = f.ptab[4 * q + i];
+            ftab4[q] = e;
+            ptab2[q] = make_float2(f.ptab[4 * kFastPairs + 2 * q], f.ptab[4 * kFastPairs + 2 * q + 1]);
         }
         // The two-mass steps run one and two steps ahead of the cells.  While
         // the cells do the pressure phase of ts, lane 0 evaluates the flow
@@ -703,9 +719,16 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
         // operations and their order are those of twomass_step.
         TwoMassNext nx;
         TwoMassPre pp;
+        if (twomass) {
+            // the wave copies the state into LDS word by word (not through
+            // lane 0's registers), then lane 0 reads it
+            copy_words(reinterpret_cast<float*>(&tm), reinterpret_cast<const float*>(b.tm + tube), lane);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
         if (lane == 0) {
             pexc = b.pexc[tube];
-            if (twomass) tm = b.tm[tube];
             if (t0 < tend) {
                 double e;
                 if (twomass) { tm.p1 = pexc; e = twomass_step(tm); }
@@ -738,10 +761,9 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
             }
             __syncthreads();   // velocity phase done
         }
-        if (lane == 0) {
-            b.pexc[tube] = pexc;
-            if (twomass) b.tm[tube] = tm;
-        }
+        if (lane == 0) b.pexc[tube] = pexc;
+        // lane 0's last writes to tm precede the last work-group barrier
+        if (twomass) copy_words(reinterpret_cast<float*>(b.tm + tube), reinterpret_cast<const float*>(&tm), lane);
     } else {
         // --------------------------- cell waves ---------------------------
         Xg[t] = Ux[K - 1];
@@ -750,19 +772,19 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
             int tl = t;
             asm volatile("" : "+v"(tl));
 #pragma unroll
-            for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
+            for (int k = 0; k < (K / 2 + 3) / 4; ++k) asm volatile("" : "+v"(cp[k]));
             {
                 const int j = tl * K;
-                wide_pressure<K>(Pg + j, Ug + j, Ug + j + Wp, ptab, Ux, kP, cw, Xg[tl - 1]);
+                wide_pressure<K>(Pg + j, Ug + j, Ug + j + Wp, ptab2, Ux, kP, cp, Xg[tl - 1]);
             }
             __syncthreads();
             const float ev = (float)sexe[ts & 1];
             {
                 asm volatile("" : "+v"(tl));
 #pragma unroll
-                for (int k = 0; k < (K + 3) / 4; ++k) asm volatile("" : "+v"(cw[k]));
+                for (int k = 0; k < (K / 2 + 3) / 4; ++k) asm volatile("" : "+v"(cp[k]));
                 const int j = tl * K;
-                wide_faces<K>(Pg + j, Pg + j - Wp, Ug + j, Xg + tl, ftab4, Ux, gX, gY, cw, ev);
+                wide_faces<K>(Pg + j, Pg + j - Wp, Ug + j, Xg + tl, ftab4, Ux, gX, gY, cp, ev);
             }
             __syncthreads();
         }
@@ -1204,11 +1226,11 @@ int fast_wide_cells_per_lane(int max_padded_cells, int threads) {
 
 size_t fast_wide_lds_bytes(int K, int threads, int guard_floats) {
     const int T = threads - 64;
-    return (size_t)(2 * K * T + 4 * guard_floats + 4 + T) * sizeof(float);
+    return (size_t)(2 * K * T + 2 * guard_floats + 8 + T) * sizeof(float);
 }
 
 size_t fast_wide_lds_static_bytes() {
-    return sizeof(TwoMassState) + kFastClasses * (4 * sizeof(float) + sizeof(float)) + 2 * sizeof(double) + 64;
+    return sizeof(TwoMassState) + kFastPairs * (4 * sizeof(float) + 2 * sizeof(float)) + 2 * sizeof(double) + 64;
 }
 
 hipError_t configure_fast_wide(int K) {

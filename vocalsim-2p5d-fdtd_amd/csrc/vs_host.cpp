@@ -493,4 +493,51 @@ int enumerate_fast_classes(const uint16_t* code, const size_t* bounds, const flo
     return VS_OK;
 }
 
+int enumerate_fast_pairs(const uint8_t* c8, const float* tab, const int* H, const int* W, const long long* off,
+                         int groups, uint8_t* p8, float* ptab, int& npairs, std::string& err) {
+    std::vector<int> id(kFastClasses * kFastClasses, -1);
+    std::vector<int> first, second;   // classes of the pair classes, in first-seen order
+    auto pair_id = [&](int a, int b) {
+        int& p = id[a * kFastClasses + b];
+        if (p < 0) {
+            p = (int)first.size();
+            first.push_back(a);
+            second.push_back(b);
+        }
+        return p;
+    };
+    pair_id(0, 0);   // pair class 0: two pad cells (every pair past a row's last real cell)
+    for (int g = 0; g < groups; ++g) {
+        if (H[g] < 3 || W[g] < 3) {
+            err = "fast pairs: tube frames must be at least 3 x 3";
+            return VS_E_INVALID;
+        }
+        const int Hi = H[g] - 2, Wi = W[g] - 2;
+        for (int r = 0; r < Hi; ++r)
+            for (int c = 0; c < Wi; c += 2) {
+                const long long q = off[g] + (long long)(r + 1) * W[g] + (c + 1);
+                const int a = c8[q], b = c + 1 < Wi ? c8[q + 1] : 0;
+                if (a >= kFastClasses || b >= kFastClasses) {
+                    err = "fast pairs: class byte out of range";
+                    return VS_E_INVALID;
+                }
+                const int p = pair_id(a, b);
+                if (p < kFastPairs) p8[q] = (uint8_t)p;
+            }
+    }
+    npairs = (int)first.size();
+    if (npairs > kFastPairs) return VS_OK;
+    std::fill(ptab, ptab + 6 * kFastPairs, 0.0f);
+    for (int p = 0; p < npairs; ++p) {
+        const int a = first[p], b = second[p];
+        ptab[4 * p + 0] = tab[kFastClasses + a];
+        ptab[4 * p + 1] = tab[2 * kFastClasses + a];
+        ptab[4 * p + 2] = tab[kFastClasses + b];
+        ptab[4 * p + 3] = tab[2 * kFastClasses + b];
+        ptab[4 * kFastPairs + 2 * p + 0] = tab[a];
+        ptab[4 * kFastPairs + 2 * p + 1] = tab[b];
+    }
+    return VS_OK;
+}
+
 }  // namespace vs
