@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
+import random_specs
 from vocalsim import Solver, tubes
 from vocalsim import _abi
 
@@ -205,7 +206,17 @@ WIDE_CASES = {
     # 4 cm of 0.5 cm^2: about 1 700 padded cells, the 4-cells-per-lane shape
     "short_narrow": [tubes.uniform_tube(num_steps=700).replace(lengths_cm=np.array([4.0]),
                                                                 areas_cm2=np.array([0.5]))],
+    # random tubes at four sample rates, 2D and 2.5D, that the on-chip kernel
+    # holds (about 100 pair classes); built when the case runs
+    "random_mixed_rates": None,
 }
+
+
+def _random_mixed_rates():
+    def padded(sp):
+        d = orc.OracleTube(sp, 32).dims
+        return (d["frameH"] - 2) * (d["frameW"] - 1)
+    return [sp for sp in random_specs.random_specs(24, 5, num_steps=500) if padded(sp) <= 18000]
 
 
 @pytest.mark.parametrize("threads", [1024, 512])
@@ -214,7 +225,7 @@ def test_wide_layout_equals_odd_layout(hip_device, name, threads):
     # the wide-access form (rows padded to 4 cells, 4 (mod 8) cells per lane,
     # 16-byte LDS accesses, select-free face update) gives the values of the
     # odd-run form: probes, source traces and final states compare equal
-    specs = WIDE_CASES[name]
+    specs = WIDE_CASES[name] or _random_mixed_rates()
     runs = {}
     for wide in (1, 0):
         with Solver(specs, precision=32, threads=threads, kernel=_abi.K_FAST, steps_per_launch=256) as s:
@@ -229,6 +240,12 @@ def test_wide_layout_equals_odd_layout(hip_device, name, threads):
                 assert K % 2 == 1
             print(f"[wide vs odd] case={name} threads={threads} wide={wide} K={K}", flush=True)
             s.run()
+            # the layout that ran: the wide form reads its class tables by
+            # cell pair and holds at most 128 pair classes
+            npairs = s.get_option(_abi.OPT_FAST_PAIR_CLASSES)
+            print(f"[wide vs odd]   pair classes {npairs}", flush=True)
+            assert 1 <= npairs <= 128
+            assert s.get_option(_abi.OPT_FAST_LAYOUT_K) == K
             runs[wide] = [(s.probe(i), s.source_trace(i), s.field("pressure", i), s.field("vx", i), s.field("vy", i))
                           for i in range(len(specs))]
     for a, b in zip(runs[1], runs[0]):
