@@ -41,11 +41,16 @@ GRAPH_BLOCKS = 4   # 256-step launches per captured step graph (the library's de
 # HBM bytes per 256-step launch of k_fast_wide<20> on the 256-tube batch at
 # 1024 threads (the autotuned shape), from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE passes, each scaled by the same counter over a plain copy kernel
-# of known size (FETCH x2.000, WRITE x1.000): 102.5 MB read, 62.7 MB written;
+# of known size (FETCH x2.000, WRITE x1.000): 102.4 MB read, 62.7 MB written;
 # profiles/k_fast_wide_batch256_fp32_pmc.md.
-PMC_TRAFFIC_BYTES = 165.16e6
+PMC_TRAFFIC_BYTES = 165.06e6
 PMC_THREADS = 1024
 PMC_LAYOUT_K = 20
+# Fraction of the time each pipe of k_fast_wide<20> is busy on this batch, from
+# rocprofv3 SQ counters (profiles/k_fast_wide_batch256_fp32_sq.md): VALU =
+# SQ_ACTIVE_INST_VALU x 4 waves per SIMD / SQ_WAVE_CYCLES, LDS =
+# SQ_LDS_IDX_ACTIVE / cycles per CU.  The VALU is the busiest pipe.
+PIPES_BUSY = {"valu": 0.84, "lds": 0.68}
 
 
 def log(msg):
@@ -388,22 +393,24 @@ def main():
                        "workload": "44-section vowel tube, two-mass 800 Pa, FP64, one tube"},
         "ratio_batched_fp32_over_plain_fp64": value / fp64_rate if fp64_rate else None,
         "precision_fp32_vs_fp64": precision,
-        "roofline": {"bound": "lds", "achieved": lds_achieved, "peak": lds_peak, "unit": "GB/s",
-                     "frac": lds_achieved / lds_peak if lds_peak > 0 else None,
+        "roofline": {"bound": "valu", "achieved": flops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": flops / FP32_PEAK_TFLOPS,
                      "traffic": PMC_TRAFFIC_BYTES if (kernel == _abi.K_FAST and threads == PMC_THREADS
                                                       and layout_k == PMC_LAYOUT_K and args.tubes == 256) else None,
                      "kernel": ("k_fast_wide<K>" if layout == "wide" else "k_fast_step<K>") + " (batched FP32)",
                      "cells_per_lane": layout_k, "mean_launch_ms": mean_launch_ms,
                      "mean_launch_basis": "HIP events around the timed region on the library's stream / launches",
                      "single_launch_ms": single_launch_ms,
-                     "basis": f"{LDS_BYTES_PER_CELL[layout]} B of LDS traffic per real cell and step x {cells} cells x "
-                              f"{BLOCK} steps per launch; peak = LDS read bandwidth measured on this device "
-                              f"(vs_measure_lds_bandwidth, widest read)",
-                     "lds_peak_by_width_gbs": lds_peaks,
-                     "slots": slots_info,
-                     "valu": {"achieved_tflops": flops, "peak_tflops": FP32_PEAK_TFLOPS,
-                              "frac": flops / FP32_PEAK_TFLOPS,
-                              "basis": f"{FLOPS_PER_CELL} algorithmic flops per cell-update"},
+                     "basis": f"{FLOPS_PER_CELL} algorithmic flops per cell-update x {cells} cells x {BLOCK} steps "
+                              f"per launch; peak = MI355X's published FP32 vector peak (it counts packed FP32 "
+                              f"FMAs, which this build does not emit: the unpacked issue limit is half of it)",
+                     "pipes_busy": PIPES_BUSY if (kernel == _abi.K_FAST and layout_k == PMC_LAYOUT_K) else None,
+                     "lds": {"achieved_gbs": lds_achieved, "peak_gbs": lds_peak,
+                             "frac": lds_achieved / lds_peak if lds_peak > 0 else None,
+                             "basis": f"{LDS_BYTES_PER_CELL[layout]} B of LDS traffic per real cell and step; peak = "
+                                      f"LDS read bandwidth measured on this device (vs_measure_lds_bandwidth, "
+                                      f"widest read)",
+                             "peak_by_width_gbs": lds_peaks, "slots": slots_info},
                      "hbm": {"algorithmic_gbs_if_streamed": hbm_alg, "peak_gbs": HBM_PEAK_GBS,
                              "compulsory_bytes_per_launch": cells * COMPULSORY_BYTES_PER_CELL,
                              "measured_bytes_per_launch": PMC_TRAFFIC_BYTES if layout_k == PMC_LAYOUT_K else None,
