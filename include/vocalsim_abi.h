@@ -158,10 +158,15 @@ enum vs_option {
                                     kernel the current work-group size runs, 4 (mod 8) for the
                                     wide-access form, odd for the odd-run form, 0 when neither
                                     fits the batch */
-    VS_OPT_FAST_PAIR_CLASSES = 13 /* read-only (vs_get_option): pair classes of the batch (the
+    VS_OPT_FAST_PAIR_CLASSES = 13, /* read-only (vs_get_option): pair classes of the batch (the
                                     ordered class pairs of neighbouring cells the wide-access form
                                     reads its coefficients by), 0 before the on-chip kernel's
                                     set-up has run; above 128 the batch runs the odd-run form */
+    VS_OPT_FAST_ROW_QUANTUM = 14 /* read-only (vs_get_option): the wide-access form pads rows to a
+                                    multiple of this many cells: its cells per lane (each column at
+                                    one slot of its lanes' runs, the source face handled in one
+                                    group of four cells), or 4 when such rows do not fit; 0 when
+                                    the wide-access form does not run */
 };
 
 enum vs_kernel {

@@ -55,7 +55,8 @@ def load(path: str):
     L.hs_fast_classes.argtypes = [_P(_c.c_uint16), _P(_c.c_long), _P(_c.c_float), _c.c_int, _P(_c.c_uint8),
                                   _P(_c.c_float), _P(_c.c_int), _c.c_char_p, _c.c_int]
     L.hs_fast_pairs.argtypes = [_P(_c.c_uint8), _P(_c.c_float), _P(_c.c_int), _P(_c.c_int), _P(_c.c_longlong),
-                                _c.c_int, _P(_c.c_uint8), _P(_c.c_float), _P(_c.c_int), _c.c_char_p, _c.c_int]
+                                _c.c_int, _P(_c.c_uint8), _P(_c.c_float), _P(_c.c_int), _P(_c.c_int), _c.c_char_p,
+                                _c.c_int]
     L.hs_planted_overflow.argtypes = [_c.c_int]
     return L
 
@@ -115,11 +116,12 @@ def fast_classes(L, code, lut, bounds=None):
     return rc, c8, tab.reshape(5, nf), ncls.value, err.value.decode()
 
 
-def fast_pairs(L, c8, tab, frames, fill=0xEE):
+def fast_pairs(L, c8, tab, frames, fill=0xEE, with_src=False):
     """Pair classes over tube frames [(H, W, offset)] of the class bytes c8
     with the class table tab (5 x kFastClasses): (rc, p8, face entries
-    [kFastPairs, 4], pressure entries [kFastPairs, 2], npairs, message).
-    p8 starts as `fill` everywhere."""
+    [kFastPairs, 4], pressure entries [kFastPairs, 2], npairs, message), and
+    with `with_src` the per-tube source columns after them.  p8 starts as
+    `fill` everywhere."""
     c8 = np.ascontiguousarray(c8, dtype=np.uint8)
     tab = np.ascontiguousarray(tab, dtype=np.float32)
     H = np.ascontiguousarray([f[0] for f in frames], dtype=np.int32)
@@ -129,11 +131,13 @@ def fast_pairs(L, c8, tab, frames, fill=0xEE):
     npmax = L.hs_fast_pairs_max()
     ptab = np.full(6 * npmax, np.nan, dtype=np.float32)
     n = ctypes.c_int(0)
+    src = np.full(len(frames), 99, dtype=np.int32)
     err = ctypes.create_string_buffer(256)
     rc = L.hs_fast_pairs(_ptr(c8, _c.c_uint8), _ptr(tab, _c.c_float), _ptr(H, _c.c_int), _ptr(W, _c.c_int),
                          _ptr(off, _c.c_longlong), len(frames), _ptr(p8, _c.c_uint8), _ptr(ptab, _c.c_float),
-                         ctypes.byref(n), err, 256)
-    return rc, p8, ptab[:4 * npmax].reshape(npmax, 4), ptab[4 * npmax:].reshape(npmax, 2), n.value, err.value.decode()
+                         ctypes.byref(n), _ptr(src, _c.c_int), err, 256)
+    out = (rc, p8, ptab[:4 * npmax].reshape(npmax, 4), ptab[4 * npmax:].reshape(npmax, 2), n.value, err.value.decode())
+    return out + (src,) if with_src else out
 
 
 # ---------------------------------------------------------------------------
@@ -423,6 +427,22 @@ def check_fast_pairs(L):
     assert npairs == len(seen) <= npmax
     assert np.all(p8[~written] == 0xEE), "a position that is not a pair's first interior cell was written"
     assert np.isnan(face[npairs:]).sum() == 0 and not face[npairs:].any() and not pres[npairs:].any()
+    # source columns: the interior column of a tube's cells whose class has
+    # r = 0 (tab[4] == 0), -1 without one, -2 when they span columns
+    tab_s = tab.copy()
+    tab_s[4, :] = 1.0
+    tab_s[4, 7] = 0.0   # class 7: the source face
+    cs = c8.copy()
+    cs[cs == 7] = 6
+    H2, W2, off2 = frames[3]   # 12 x 17 frame: put class 7 in interior column 4 of rows 2..6
+    for r in range(2, 7):
+        cs[off2 + (r + 1) * W2 + 5] = 7
+    H4, W4, off4 = frames[4]   # 4 x 30 frame: class 7 in columns 3 and 9
+    cs[off4 + 1 * W4 + 4] = 7
+    cs[off4 + 2 * W4 + 10] = 7
+    rc, *_, src = fast_pairs(L, cs, tab_s, frames, with_src=True)
+    assert rc == 0
+    assert list(src) == [-1, -1, -1, 4, -2, -1], list(src)
     # more pair classes than the tables hold: still VS_OK, with the count
     huge = rng.integers(0, 64, 40 * 130).astype(np.uint8)
     rc, _, _, _, n_huge, msg = fast_pairs(L, huge, tab, [(40, 130, 0)])

@@ -111,11 +111,12 @@ int hs_fast_classes(const uint16_t* code, const long* bounds, const float* luts,
 int hs_fast_pairs_max(void) { return vs::kFastPairs; }
 
 // Pair classes of the wide-access on-chip step over `groups` tube frames
-// H[g] x W[g] whose class bytes start at off[g] in c8.
+// H[g] x W[g] whose class bytes start at off[g] in c8, and each tube's
+// source column.
 int hs_fast_pairs(const uint8_t* c8, const float* tab, const int* H, const int* W, const long long* off, int groups,
-                  uint8_t* p8, float* ptab, int* npairs, char* err, int errlen) {
+                  uint8_t* p8, float* ptab, int* npairs, int* src_col, char* err, int errlen) {
     std::string e;
-    const int rc = vs::enumerate_fast_pairs(c8, tab, H, W, off, groups, p8, ptab, *npairs, e);
+    const int rc = vs::enumerate_fast_pairs(c8, tab, H, W, off, groups, p8, ptab, *npairs, src_col, e);
     if (rc != VS_OK) put_err(e, err, errlen);
     return rc;
 }

@@ -209,6 +209,10 @@ WIDE_CASES = {
     # random tubes at four sample rates, 2D and 2.5D, that the on-chip kernel
     # holds (about 100 pair classes); built when the case runs
     "random_mixed_rates": None,
+    # source columns in different groups of four run slots (PML on and off):
+    # every group handles the source
+    "mixed_pml": [tubes.vowel_tube(1, num_steps=700, source=tubes.SOURCE_GAUSSIAN).replace(pml=0),
+                  tubes.vowel_tube(2, num_steps=700, source=tubes.SOURCE_GAUSSIAN)],
 }
 
 
@@ -243,9 +247,11 @@ def test_wide_layout_equals_odd_layout(hip_device, name, threads):
             # the layout that ran: the wide form reads its class tables by
             # cell pair and holds at most 128 pair classes
             npairs = s.get_option(_abi.OPT_FAST_PAIR_CLASSES)
-            print(f"[wide vs odd]   pair classes {npairs}", flush=True)
+            q = s.get_option(_abi.OPT_FAST_ROW_QUANTUM)
+            print(f"[wide vs odd]   pair classes {npairs}, rows padded to a multiple of {q}", flush=True)
             assert 1 <= npairs <= 128
             assert s.get_option(_abi.OPT_FAST_LAYOUT_K) == K
+            assert q == (K if wide else 0)
             runs[wide] = [(s.probe(i), s.source_trace(i), s.field("pressure", i), s.field("vx", i), s.field("vy", i))
                           for i in range(len(specs))]
     for a, b in zip(runs[1], runs[0]):
@@ -253,6 +259,27 @@ def test_wide_layout_equals_odd_layout(hip_device, name, threads):
             assert np.array_equal(x, y)
     p64, _ = orc.OracleTube(specs[0], 64).run()
     assert rel_l2(runs[1][0][0], p64) <= 1e-3
+
+
+def test_wide_compact_rows_equal_odd_layout(hip_device):
+    # a 20 cm tube (49 x 281 interior cells) at 768 threads: its rows padded
+    # to a multiple of 20 cells need 14 700 slots, more than the 14 080 of
+    # 20 cells on 704 lanes, so the wide form pads them to a multiple of 4
+    # (the source handled in every group of four cells); it equals the
+    # odd-run form (K = 21)
+    base = tubes.vowel_tube(1, num_steps=700, source=tubes.SOURCE_GAUSSIAN)
+    spec = base.replace(lengths_cm=base.lengths_cm * (20.0 / base.lengths_cm.sum()))
+    runs = {}
+    for wide in (1, 0):
+        with Solver([spec], precision=32, threads=768, kernel=_abi.K_FAST, steps_per_launch=256) as s:
+            s.set_option(_abi.OPT_FAST_WIDE, wide)
+            s.run()
+            K, q = s.get_option(_abi.OPT_FAST_LAYOUT_K), s.get_option(_abi.OPT_FAST_ROW_QUANTUM)
+            print(f"[compact rows] wide={wide} K={K} row quantum={q}", flush=True)
+            assert (K, q) == ((20, 4) if wide else (21, 0))
+            runs[wide] = (s.probe(0), s.source_trace(0), s.field("pressure", 0), s.field("vx", 0), s.field("vy", 0))
+    for x, y in zip(runs[1], runs[0]):
+        assert np.array_equal(x, y)
 
 
 def test_wide_layout_graph_replays(hip_device):

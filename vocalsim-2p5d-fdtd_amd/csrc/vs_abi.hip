@@ -131,10 +131,14 @@ struct vs_handle {
     bool flux_valid = false;   // Uxf/Uyf hold the current state
     bool fast_classes = false; // class bytes and tables built
     int fast_pairs = 0;        // pair classes of the wide-access form (0 until the class bytes are built)
+    bool fast_wide_ok = true;  // the batch's pair classes and source columns suit the wide-access form
+    std::vector<int> src_cols; // per tube: interior column of its source faces, -1 if none
     int max_interior = 0;      // largest (H-2)*(W-2) of the batch
     int max_padded = 0;        // largest (H-2)*(W-1): interior with one pad column per row
     int max_wp = 0;            // widest padded row W-1
-    int max_padded4 = 0;       // the same with rows padded to a multiple of 4 (wide on-chip kernel)
+    int max_padded_w[3] = {};  // the same with rows padded to a multiple of K = kFastWideKs[i] (wide on-chip kernel)
+    int max_wp_w[3] = {};
+    int max_padded4 = 0;       // ... and to a multiple of 4 (its compact rows)
     int max_wp4 = 0;
     int fast_wide = 1;         // VS_OPT_FAST_WIDE
     int lds_limit = 0;
@@ -289,8 +293,13 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
         h->max_interior = std::max(h->max_interior, (p.H - 2) * (p.W - 2));
         h->max_padded = std::max(h->max_padded, (p.H - 2) * (p.W - 1));
         h->max_wp = std::max(h->max_wp, p.W - 1);
-        h->max_padded4 = std::max(h->max_padded4, (p.H - 2) * fast_wide_wp(p.W - 2));
-        h->max_wp4 = std::max(h->max_wp4, fast_wide_wp(p.W - 2));
+        for (int i = 0; i < 3; ++i) {
+            const int wp = fast_wide_wp(p.W - 2, kFastWideKs[i]);
+            h->max_padded_w[i] = std::max(h->max_padded_w[i], (p.H - 2) * wp);
+            h->max_wp_w[i] = std::max(h->max_wp_w[i], wp);
+        }
+        h->max_padded4 = std::max(h->max_padded4, (p.H - 2) * fast_wide_wp(p.W - 2, 4));
+        h->max_wp4 = std::max(h->max_wp4, fast_wide_wp(p.W - 2, 4));
         vs_dims& dm = h->dims[k];
         dm.frameH = p.H; dm.frameW = p.W; dm.domainH = p.domainH; dm.domainW = p.domainW;
         dm.listenerX = p.listenerX; dm.listenerY = p.listenerY;
@@ -469,31 +478,79 @@ int build_fast_classes(vs_handle* h) {
     }
     std::vector<uint8_t> p8(n, 0);
     std::vector<float> ptab(6 * kFastPairs, 0.0f);
+    std::vector<int> src_col(h->n, -1);
     int npairs = 0;
     rc = enumerate_fast_pairs(c8.data(), tab.data(), Hs.data(), Ws.data(), offs.data(), h->n, p8.data(), ptab.data(),
-                              npairs, err);
+                              npairs, src_col.data(), err);
     if (rc != VS_OK) return fail(rc, err);
     h->fast_pairs = npairs;
-    if (npairs <= kFastPairs) {
+    h->fast_wide_ok = npairs <= kFastPairs &&
+                      std::none_of(src_col.begin(), src_col.end(), [](int c) { return c == -2; });
+    h->src_cols = src_col;
+    if (h->fast_wide_ok) {
         VS_HIP(copy_on(h->stream, f.pcls8, p8.data(), n, hipMemcpyHostToDevice));
         VS_HIP(copy_on(h->stream, f.ptab, ptab.data(), ptab.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     return VS_OK;
 }
 
-// Whether the batch fits the wide-access on-chip kernel at `threads` threads
-// (VS_OPT_FAST_WIDE on, K <= kFastWideMaxK, LDS within the device limit);
-// the guard is in floats.
-bool fast_wide_fits(const vs_handle* h, int threads, int* K_out, int* GW_out) {
+// Launch shape of the wide-access on-chip kernel (wide_shape).
+struct WideShape {
+    int K = 0;    // cells per lane
+    int GW = 0;   // zero guard floats (the widest padded row)
+    int q = 0;    // rows padded to a multiple of q (K, or 4 for compact rows)
+    int ms = -2;  // lane-run group of the batch's source column (-1: none, -2: every group)
+};
+
+// The lane-run group (4 cells) holding the source column of every tube of
+// the batch in the wide-access form at K cells per lane (rows padded to a
+// multiple of K): -1 when no tube has a source face, -2 when the tubes'
+// groups differ (the kernel then handles the source in every group).
+int wide_source_group(const vs_handle* h, int K) {
+    int ms = -1;
+    for (int c : h->src_cols) {
+        if (c < 0) continue;
+        const int g = (c % K) / 4;
+        if (ms == -1) ms = g;
+        else if (ms != g) return -2;
+    }
+    return ms;
+}
+
+// Shape of the wide-access on-chip kernel for the batch at `threads`
+// threads: the smallest K of kFastWideKs whose rows fit, padded to a
+// multiple of K (q = K: a column sits at one slot of its lanes' runs in
+// every row, so the source is handled in one group, ms) or, when those rows
+// do not fit, to a multiple of 4 (q = 4, ms = -2: every group handles it);
+// the guard GW is in floats.  False when the batch does not fit
+// (VS_OPT_FAST_WIDE off, K > kFastWideMaxK, LDS beyond the device limit, or
+// pair classes / source columns the kernel cannot take).
+bool wide_shape(const vs_handle* h, int threads, WideShape& w) {
     if (!h->fast_wide || h->precision != VS_FP32 || threads < 128) return false;
-    // the pair tables hold kFastPairs pair classes (known once the class bytes are built)
-    if (h->fast_classes && h->fast_pairs > kFastPairs) return false;
-    const int K = fast_wide_cells_per_lane(h->max_padded4, threads);
-    if (K == 0) return false;
-    const int GW = h->max_wp4;
-    if ((long long)(fast_wide_lds_bytes(K, threads, GW) + fast_wide_lds_static_bytes()) > h->lds_limit) return false;
-    if (K_out) *K_out = K;
-    if (GW_out) *GW_out = GW;
+    // the pair tables hold kFastPairs pair classes, and each tube's source
+    // faces lie in one column (known once the class bytes are built)
+    if (h->fast_classes && !h->fast_wide_ok) return false;
+    const int T = threads - 64;
+    for (int i = 0; i < 3; ++i) {
+        const int K = kFastWideKs[i];
+        WideShape c;
+        if (h->max_padded_w[i] <= K * T) c = WideShape{K, h->max_wp_w[i], K, wide_source_group(h, K)};
+        else if (h->max_padded4 <= K * T) c = WideShape{K, h->max_wp4, 4, -2};
+        else continue;
+        if ((long long)(fast_wide_lds_bytes(c.K, threads, c.GW) + fast_wide_lds_static_bytes()) > h->lds_limit)
+            continue;
+        w = c;
+        return true;
+    }
+    return false;
+}
+
+// Whether the batch fits the wide-access on-chip kernel at `threads` threads.
+bool fast_wide_fits(const vs_handle* h, int threads, int* K_out, int* GW_out) {
+    WideShape w;
+    if (!wide_shape(h, threads, w)) return false;
+    if (K_out) *K_out = w.K;
+    if (GW_out) *GW_out = w.GW;
     return true;
 }
 
@@ -544,13 +601,15 @@ int prepare_fast(vs_handle* h, int threads) {
         }
         h->flux_valid = true;
     }
-    // the wide-access form also needs the batch's pair classes to fit its tables
-    int Kw = 0;
-    const bool wide = fast_wide_fits(h, threads, &Kw, nullptr);
+    // the wide-access form also needs the batch's pair classes to fit its
+    // tables and each tube's source faces in one column
+    WideShape ws;
+    const bool wide = wide_shape(h, threads, ws);
     if (!wide && !odd_fits)
         return fail(VS_E_INVALID, "batch has " + std::to_string(h->fast_pairs) + " pair classes (the wide-access form holds " +
-                    std::to_string(kFastPairs) + ") and is too large for the odd-run form at this work-group size");
-    hipError_t e = wide ? configure_fast_wide(Kw) : configure_fast(K);
+                    std::to_string(kFastPairs) + ", with each tube's source faces in one column) and is too large "
+                    "for the odd-run form at this work-group size");
+    hipError_t e = wide ? configure_fast_wide(ws.K, ws.ms) : configure_fast(K);
     if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("configure: ") + hipGetErrorString(e));
     return VS_OK;
 }
@@ -651,9 +710,10 @@ hipError_t launch_any<double>(vs_handle* h, Bufs<double>& b, int t0, int ns, con
 template <>
 hipError_t launch_any<float>(vs_handle* h, Bufs<float>& b, int t0, int ns, const int* d_t0) {
     if (h->kernel == VS_K_FAST) {
-        int Kw = 0, GW = 0;
-        if (fast_wide_fits(h, h->threads, &Kw, &GW))
-            return launch_fast_wide(Kw, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, GW, d_t0, h->stream);
+        WideShape w;
+        if (wide_shape(h, h->threads, w))
+            return launch_fast_wide(w.K, w.ms, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, w.GW, w.q,
+                                    d_t0, h->stream);
         const int K = fast_cells_per_lane(h->max_padded, h->threads);
         const int G = fast_guard_lanes(K, h->max_wp);
         return launch_fast_step(K, h->d_desc, h->d_order, h->n, b, h->fb, t0, ns, h->threads, G, d_t0, h->stream);
@@ -1125,6 +1185,7 @@ int vs_set_option(vs_handle* h, int option, int value) {
             h->barrier_spin = value; return VS_OK;
         case VS_OPT_FAST_LAYOUT_K: return fail(VS_E_INVALID, "VS_OPT_FAST_LAYOUT_K is read-only");
         case VS_OPT_FAST_PAIR_CLASSES: return fail(VS_E_INVALID, "VS_OPT_FAST_PAIR_CLASSES is read-only");
+        case VS_OPT_FAST_ROW_QUANTUM: return fail(VS_E_INVALID, "VS_OPT_FAST_ROW_QUANTUM is read-only");
         case VS_OPT_FAST_WIDE:
             if (value != 0 && value != 1) return fail(VS_E_INVALID, "fast wide must be 0 or 1");
             h->fast_wide = value; return VS_OK;
@@ -1164,6 +1225,11 @@ int vs_get_option(const vs_handle* h, int option, int* value) {
         case VS_OPT_BARRIER_SPIN: *value = h->barrier_spin; return VS_OK;
         case VS_OPT_FAST_WIDE: *value = h->fast_wide; return VS_OK;
         case VS_OPT_FAST_PAIR_CLASSES: *value = h->fast_classes ? h->fast_pairs : 0; return VS_OK;
+        case VS_OPT_FAST_ROW_QUANTUM: {
+            WideShape w;
+            *value = wide_shape(h, h->threads, w) ? w.q : 0;
+            return VS_OK;
+        }
         case VS_OPT_FAST_LAYOUT_K: {
             int Kw = 0, K = 0;
             if (fast_wide_fits(h, h->threads, &Kw, nullptr)) *value = Kw;
@@ -1654,7 +1720,7 @@ std::mutex g_adj_shape_mutex;
 
 int adj_pick_forward_threads(vs_handle* h) {
     if (h->adj_fwd_threads) return VS_OK;
-    const std::array<int, 6> key{h->n, h->max_padded, h->max_wp, h->max_padded4, h->max_wp4, h->fast_wide};
+    const std::array<int, 6> key{h->n, h->max_padded, h->max_padded_w[2], h->max_padded4, h->max_wp4, h->fast_wide};
     {
         std::lock_guard<std::mutex> lk(g_adj_shape_mutex);
         for (auto& kv : g_adj_shape_cache)
@@ -1675,13 +1741,19 @@ int adj_pick_forward_threads(vs_handle* h) {
     for (int wide = 0; wide < 2 && rc == VS_OK; ++wide)
         for (int t = kAdjThreads; t >= kAdjThreads - 4 * 64 && rc == VS_OK; t -= 64) {
             int K = 0, G = 0;
-            if (wide ? !fast_wide_fits(h, t, &K, &G) : !fast_fits(h, t, &K, &G)) continue;
-            hipError_t e = wide ? configure_fast_wide(K) : configure_fast(K);
+            WideShape w;
+            if (wide ? !wide_shape(h, t, w) : !fast_fits(h, t, &K, &G)) continue;
+            if (wide) {
+                K = w.K;
+                G = w.GW;
+            }
+            hipError_t e = wide ? configure_fast_wide(K, w.ms) : configure_fast(K);
             float ms = 1e30f;
             for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
                 e = hipEventRecord(e0, h->stream);
                 if (e == hipSuccess)
-                    e = wide ? launch_fast_wide(K, h->d_desc, h->d_order, h->n, b, f, 0, 64, t, G, nullptr, h->stream)
+                    e = wide ? launch_fast_wide(K, w.ms, h->d_desc, h->d_order, h->n, b, f, 0, 64, t, G, w.q, nullptr,
+                                                h->stream)
                              : launch_fast_step(K, h->d_desc, h->d_order, h->n, b, f, 0, 64, t, G, nullptr, h->stream);
                 if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -1723,10 +1795,15 @@ int adj_forward_fast(vs_handle* h, int S, double* probe) {
         return rc;
     int FK = 0, FG = 0;
     const bool wide = h->adj_fwd_wide != 0;
-    if (wide ? !fast_wide_fits(h, h->adj_fwd_threads, &FK, &FG) : !fast_fits(h, h->adj_fwd_threads, &FK, &FG))
+    WideShape fw;
+    if (wide ? !wide_shape(h, h->adj_fwd_threads, fw) : !fast_fits(h, h->adj_fwd_threads, &FK, &FG))
         return fail(VS_E_INVALID, "forward shape no longer fits");
+    if (wide) {
+        FK = fw.K;
+        FG = fw.GW;
+    }
     {
-        const hipError_t e = wide ? configure_fast_wide(FK) : configure_fast(FK);
+        const hipError_t e = wide ? configure_fast_wide(FK, fw.ms) : configure_fast(FK);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("configure: ") + hipGetErrorString(e));
     }
     AdjBufs<float>& a = h->af;
@@ -1739,8 +1816,8 @@ int adj_forward_fast(vs_handle* h, int S, double* probe) {
     for (int t0 = 0; t0 < h->max_steps; t0 += S) {
         if ((rc = adj_state_copy(h, h->adj_ck + (long long)(t0 / S) * 3 * h->total_cells, true, h->stream)) != VS_OK)
             return rc;
-        hipError_t e = wide ? launch_fast_wide(FK, h->d_desc, h->d_order, h->n, b, f, t0, S, h->adj_fwd_threads, FG,
-                                               nullptr, h->stream)
+        hipError_t e = wide ? launch_fast_wide(FK, fw.ms, h->d_desc, h->d_order, h->n, b, f, t0, S,
+                                               h->adj_fwd_threads, FG, fw.q, nullptr, h->stream)
                             : launch_fast_step(FK, h->d_desc, h->d_order, h->n, b, f, t0, S, h->adj_fwd_threads, FG,
                                                nullptr, h->stream);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("adjoint forward: ") + hipGetErrorString(e));

@@ -157,16 +157,18 @@ hipError_t launch_dtft(const double* d_x, long long stride, int nsig, int n, dou
 // side of the LDS regions for its widest padded row Wi + 1, and the dynamic
 // LDS bytes of a launch.
 constexpr int kFastMaxK = 23;
-// Wide-access variant (k_fast_wide): K = 4 (mod 8) up to kFastWideMaxK, rows
-// padded to a multiple of 4 cells (fast_wide_wp, vs_host.h).
+// Wide-access variant (k_fast_wide): K in kFastWideKs (4 mod 8), rows padded
+// to a multiple of row_quantum cells (K, or 4; fast_wide_wp, vs_host.h).
+// ms: the lane-run group (4 cells) holding the source column of every tube of
+// the batch when rows are padded to a multiple of K, -1 when no tube has a
+// source face, -2 when every group must handle it.
 constexpr int kFastWideMaxK = 20;
-int fast_wide_cells_per_lane(int max_padded_cells, int threads);
 size_t fast_wide_lds_bytes(int K, int threads, int guard_floats);
 size_t fast_wide_lds_static_bytes();
-hipError_t configure_fast_wide(int K);
-hipError_t launch_fast_wide(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
-                            FastBufs f, int t0, int nsteps, int threads, int guard_floats, const int* d_t0,
-                            hipStream_t st);
+hipError_t configure_fast_wide(int K, int ms);
+hipError_t launch_fast_wide(int K, int ms, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
+                            FastBufs f, int t0, int nsteps, int threads, int guard_floats, int row_quantum,
+                            const int* d_t0, hipStream_t st);
 int fast_cells_per_lane(int max_padded_cells, int threads);
 int fast_guard_lanes(int K, int max_padded_width);
 size_t fast_lds_bytes(int K, int threads, int guard);

@@ -550,52 +550,68 @@ __device__ __forceinline__ void wide_pressure(float* __restrict__ P, const float
     }
 }
 
-template <int K>
+// Face update of the run's group m (cells 4m .. 4m + 3).  SRC: the group
+// holds the tube's source column, whose cells take -ev in place of the right
+// neighbour's pressure (r = 0 in the sign bit of ay); no other cell has r = 0.
+template <int K, bool SRC>
+__device__ __forceinline__ void wide_face_group(int m, const f4v* __restrict__ P4, const f4v* __restrict__ A4,
+                                                f4v* __restrict__ U4, const float* __restrict__ P,
+                                                const f4v* __restrict__ ftab4, float (&Ux)[K],
+                                                const float (&gX)[K], const float (&gY)[K],
+                                                const uint32_t (&cp)[(K / 2 + 3) / 4], float ev, f4v& p) {
+#pragma clang fp contract(off)
+    // the group's table and state reads go out together (one wait); P of
+    // the next four cells, past the run the next lane's first cell
+    const f4v e01 = ftab4[pair_of<K>(cp, 2 * m)], e23 = ftab4[pair_of<K>(cp, 2 * m + 1)];
+    f4v pnext;
+    if (m + 1 < K / 4) pnext = P4[m + 1];
+    else pnext[0] = P[K];
+    const f4v pa = A4[m], uy = U4[m];
+    __builtin_amdgcn_sched_barrier(0);
+    const float ax[4] = {e01[0], e01[2], e23[0], e23[2]};   // s = 0 in the sign bit
+    const float ay[4] = {e01[1], e01[3], e23[1], e23[3]};   // r = 0 in the sign bit
+    f4v un;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int k = 4 * m + i;
+        const float pr = i < 3 ? p[i + 1] : pnext[0];
+        // walls with admittance (s = 0): U = -g P_neighbour; excitation (s = r = 0): Ux = g exeV
+        const float ps = flag_off(ax[i]) ? 0.f : p[i];
+        const float prx = SRC && flag_off(ay[i]) ? -ev : pr;
+        const float fx = ps - prx, fy = ps - pa[i];
+        Ux[k] = __builtin_fmaf(Ux[k], fabsf(ax[i]), gX[k] * fx);
+        un[i] = __builtin_fmaf(fabsf(ay[i]), uy[i], gY[k] * fy);
+    }
+    U4[m] = un;
+    p = pnext;
+}
+
+// MS: the group that holds the source column of every tube of the batch
+// (-1: no tube has a source face, -2: the columns differ, every group
+// handles the source).  The condition folds once the loop is unrolled.
+template <int K, int MS>
 __device__ __forceinline__ void wide_faces(const float* __restrict__ P, const float* __restrict__ PA,
                                            float* __restrict__ UY, float* __restrict__ UXE,
                                            const f4v* __restrict__ ftab4, float (&Ux)[K],
                                            const float (&gX)[K], const float (&gY)[K],
                                            const uint32_t (&cp)[(K / 2 + 3) / 4], float ev) {
-#pragma clang fp contract(off)
     const f4v* const P4 = reinterpret_cast<const f4v*>(P);
     const f4v* const A4 = reinterpret_cast<const f4v*>(PA);
     f4v* const U4 = reinterpret_cast<f4v*>(UY);
     f4v p = P4[0];
 #pragma unroll
     for (int m = 0; m < K / 4; ++m) {
-        // the group's table and state reads go out together (one wait); P of
-        // the next four cells, past the run the next lane's first cell
-        const f4v e01 = ftab4[pair_of<K>(cp, 2 * m)], e23 = ftab4[pair_of<K>(cp, 2 * m + 1)];
-        f4v pnext;
-        if (m + 1 < K / 4) pnext = P4[m + 1];
-        else pnext[0] = P[K];
-        const f4v pa = A4[m], uy = U4[m];
-        __builtin_amdgcn_sched_barrier(0);
-        const float ax[4] = {e01[0], e01[2], e23[0], e23[2]};   // s = 0 in the sign bit
-        const float ay[4] = {e01[1], e01[3], e23[1], e23[3]};   // r = 0 in the sign bit
-        f4v un;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int k = 4 * m + i;
-            const float pr = i < 3 ? p[i + 1] : pnext[0];
-            // walls with admittance (s = 0): U = -g P_neighbour; excitation (s = r = 0): Ux = g exeV
-            const float ps = flag_off(ax[i]) ? 0.f : p[i];
-            const float prx = flag_off(ay[i]) ? -ev : pr;
-            const float fx = ps - prx, fy = ps - pa[i];
-            Ux[k] = __builtin_fmaf(Ux[k], fabsf(ax[i]), gX[k] * fx);
-            un[i] = __builtin_fmaf(fabsf(ay[i]), uy[i], gY[k] * fy);
-        }
-        U4[m] = un;
-        p = pnext;
+        if (MS == -2 || m == MS) wide_face_group<K, true>(m, P4, A4, U4, P, ftab4, Ux, gX, gY, cp, ev, p);
+        else wide_face_group<K, false>(m, P4, A4, U4, P, ftab4, Ux, gX, gY, cp, ev, p);
     }
     *UXE = Ux[K - 1];
 }
 
-template <int K>
+template <int K, int MS>
 __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__ tubes,
                                                     const int* __restrict__ order, Bufs<float> b,
                                                     FastBufs f, int t0_arg, int nsteps,
-                                                    const int* __restrict__ d_t0, int GW) {
+                                                    const int* __restrict__ d_t0, int GW, int wq) {
     static_assert(K % 8 == 4, "the wide kernel needs K = 4 (mod 8)");
     extern __shared__ f4v lds4[];
     float* const lds = reinterpret_cast<float*>(lds4);
@@ -612,7 +628,10 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
     const int tube = order[blockIdx.x];
     const TubeDesc& D = tubes[tube];
     const int H = D.H, W = D.W, Hi = H - 2, Wi = W - 2;
-    const int Wp = fast_wide_wp(Wi);
+    // rows are padded to a multiple of wq: with wq = K a column sits at the
+    // same slot of its lanes' runs in every row (the source column in group
+    // MS); wq = 4 runs with MS = -2
+    const int Wp = fast_wide_wp(Wi, wq);
     const int t0 = d_t0 ? *d_t0 : t0_arg;
     const int tend = min(t0 + nsteps, D.num_steps);
     const long long co = D.cell_off;
@@ -784,7 +803,7 @@ __global__ __launch_bounds__(1024) void k_fast_wide(const TubeDesc* __restrict__
 #pragma unroll
                 for (int k = 0; k < (K / 2 + 3) / 4; ++k) asm volatile("" : "+v"(cp[k]));
                 const int j = tl * K;
-                wide_faces<K>(Pg + j, Pg + j - Wp, Ug + j, Xg + tl, ftab4, Ux, gX, gY, cp, ev);
+                wide_faces<K, MS>(Pg + j, Pg + j - Wp, Ug + j, Xg + tl, ftab4, Ux, gX, gY, cp, ev);
             }
             __syncthreads();
         }
@@ -1153,25 +1172,46 @@ hipError_t launch_rec_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs
 }
 
 #ifndef VS_FAST_SPLIT_K
-template <int K>
-hipError_t configure_wide_k() {
+template <int K, int MS>
+hipError_t configure_wide_km() {
     static std::atomic<int> done{0};
     if (done.load()) return hipSuccess;
     const int dyn = 160 * 1024 - (int)fast_wide_lds_static_bytes();
-    const hipError_t e = hipFuncSetAttribute((const void*)k_fast_wide<K>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    const hipError_t e = hipFuncSetAttribute((const void*)k_fast_wide<K, MS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             dyn);
     if (e == hipSuccess) done.store(1);
     return e;
 }
 
-template <int K>
-hipError_t launch_wide_k(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f, int t0,
-                         int nsteps, int threads, int GW, const int* d_t0, hipStream_t st) {
+template <int K, int MS>
+hipError_t launch_wide_km(const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f, int t0,
+                          int nsteps, int threads, int GW, int wq, const int* d_t0, hipStream_t st) {
+    // a source group needs rows padded to a multiple of K
+    if (MS != -2 && wq != K) return hipErrorInvalidValue;
+    if (wq != K && wq != 4) return hipErrorInvalidValue;
     const size_t lds = fast_wide_lds_bytes(K, threads, GW);
-    hipError_t e = configure_wide_k<K>();
+    hipError_t e = configure_wide_km<K, MS>();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_fast_wide<K>), dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0, nsteps, d_t0,
-                       GW);
+    hipLaunchKernelGGL((k_fast_wide<K, MS>), dim3(n), dim3(threads), lds, st, d_tubes, d_order, b, f, t0, nsteps, d_t0,
+                       GW, wq);
     return hipGetLastError();
+}
+
+// The instantiation for a source group ms in [-2, K / 4) chosen at run time.
+template <int K, int MS = K / 4 - 1>
+hipError_t configure_wide_k(int ms) {
+    if (ms == MS) return configure_wide_km<K, MS>();
+    if constexpr (MS > -2) return configure_wide_k<K, MS - 1>(ms);
+    else return hipErrorInvalidValue;
+}
+
+template <int K, int MS = K / 4 - 1>
+hipError_t launch_wide_k(int ms, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
+                         int t0, int nsteps, int threads, int GW, int wq, const int* d_t0, hipStream_t st) {
+    if (ms == MS) return launch_wide_km<K, MS>(d_tubes, d_order, n, b, f, t0, nsteps, threads, GW, wq, d_t0, st);
+    if constexpr (MS > -2)
+        return launch_wide_k<K, MS - 1>(ms, d_tubes, d_order, n, b, f, t0, nsteps, threads, GW, wq, d_t0, st);
+    else return hipErrorInvalidValue;
 }
 #endif
 
@@ -1216,14 +1256,6 @@ int fast_cells_per_lane(int max_padded_cells, int threads) {
 
 int fast_guard_lanes(int K, int max_padded_width) { return (max_padded_width + K - 1) / K + 1; }
 
-// Wide variant: K = 4 (mod 8), at most kFastWideMaxK (0: the batch needs more).
-int fast_wide_cells_per_lane(int max_padded_cells, int threads) {
-    const int T = threads - 64;
-    const int k = std::max(4, (max_padded_cells + T - 1) / T);
-    const int v = ((k - 4 + 7) / 8) * 8 + 4;
-    return v <= kFastWideMaxK ? v : 0;
-}
-
 size_t fast_wide_lds_bytes(int K, int threads, int guard_floats) {
     const int T = threads - 64;
     return (size_t)(2 * K * T + 2 * guard_floats + 8 + T) * sizeof(float);
@@ -1233,21 +1265,23 @@ size_t fast_wide_lds_static_bytes() {
     return sizeof(TwoMassState) + kFastPairs * (4 * sizeof(float) + 2 * sizeof(float)) + 2 * sizeof(double) + 64;
 }
 
-hipError_t configure_fast_wide(int K) {
+hipError_t configure_fast_wide(int K, int ms) {
     switch (K) {
-    case 4: return configure_wide_k<4>();
-    case 12: return configure_wide_k<12>();
-    case 20: return configure_wide_k<20>();
+    case 4: return configure_wide_k<4>(ms);
+    case 12: return configure_wide_k<12>(ms);
+    case 20: return configure_wide_k<20>(ms);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_fast_wide(int K, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b, FastBufs f,
-                            int t0, int nsteps, int threads, int guard_floats, const int* d_t0, hipStream_t st) {
+hipError_t launch_fast_wide(int K, int ms, const TubeDesc* d_tubes, const int* d_order, int n, Bufs<float> b,
+                            FastBufs f, int t0, int nsteps, int threads, int guard_floats, int row_quantum,
+                            const int* d_t0, hipStream_t st) {
+    const int q = row_quantum;
     switch (K) {
-    case 4: return launch_wide_k<4>(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard_floats, d_t0, st);
-    case 12: return launch_wide_k<12>(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard_floats, d_t0, st);
-    case 20: return launch_wide_k<20>(d_tubes, d_order, n, b, f, t0, nsteps, threads, guard_floats, d_t0, st);
+    case 4: return launch_wide_k<4>(ms, d_tubes, d_order, n, b, f, t0, nsteps, threads, guard_floats, q, d_t0, st);
+    case 12: return launch_wide_k<12>(ms, d_tubes, d_order, n, b, f, t0, nsteps, threads, guard_floats, q, d_t0, st);
+    case 20: return launch_wide_k<20>(ms, d_tubes, d_order, n, b, f, t0, nsteps, threads, guard_floats, q, d_t0, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -494,7 +494,7 @@ int enumerate_fast_classes(const uint16_t* code, const size_t* bounds, const flo
 }
 
 int enumerate_fast_pairs(const uint8_t* c8, const float* tab, const int* H, const int* W, const long long* off,
-                         int groups, uint8_t* p8, float* ptab, int& npairs, std::string& err) {
+                         int groups, uint8_t* p8, float* ptab, int& npairs, int* src_col, std::string& err) {
     std::vector<int> id(kFastClasses * kFastClasses, -1);
     std::vector<int> first, second;   // classes of the pair classes, in first-seen order
     auto pair_id = [&](int a, int b) {
@@ -513,6 +513,12 @@ int enumerate_fast_pairs(const uint8_t* c8, const float* tab, const int* H, cons
             return VS_E_INVALID;
         }
         const int Hi = H[g] - 2, Wi = W[g] - 2;
+        src_col[g] = -1;
+        auto note_source = [&](int cls, int col) {
+            if (tab[4 * kFastClasses + cls] != 0.0f) return;   // r = 1: no source on this face
+            if (src_col[g] == -1) src_col[g] = col;
+            else if (src_col[g] != col) src_col[g] = -2;
+        };
         for (int r = 0; r < Hi; ++r)
             for (int c = 0; c < Wi; c += 2) {
                 const long long q = off[g] + (long long)(r + 1) * W[g] + (c + 1);
@@ -523,6 +529,8 @@ int enumerate_fast_pairs(const uint8_t* c8, const float* tab, const int* H, cons
                 }
                 const int p = pair_id(a, b);
                 if (p < kFastPairs) p8[q] = (uint8_t)p;
+                note_source(a, c);
+                if (c + 1 < Wi) note_source(b, c + 1);
             }
     }
     npairs = (int)first.size();

@@ -127,14 +127,18 @@ constexpr int kFastClasses = 64;
 int enumerate_fast_classes(const uint16_t* code, const size_t* bounds, const float* luts, int groups, uint8_t* c8,
                            float* tab, int& ncls, std::string& err);
 
-// Padded row length of the wide-access on-chip step: the interior width plus
-// at least one zero pad cell, rounded up to a multiple of 4 cells.
-constexpr int fast_wide_wp(int interior_width) { return (interior_width + 1 + 3) & ~3; }
+// Padded row length of the wide-access on-chip step at K cells per lane (K a
+// multiple of 4): the interior width plus at least one zero pad cell,
+// rounded up to a multiple of K, so that every cell of a column sits at the
+// same position k of its lane's run in every row.
+constexpr int fast_wide_wp(int interior_width, int K) { return (interior_width + 1 + K - 1) / K * K; }
+// Cells per lane of the wide-access form, smallest first.
+constexpr int kFastWideKs[3] = {4, 12, 20};
 
 // Pair classes of the wide-access on-chip step.  Its lanes own runs of an
 // even number of consecutive cells of the padded interior j = r * Wp + c
-// (r < Hi, c < Wp = fast_wide_wp(Wi); columns c >= Wi are pad cells of class
-// 0), so the cells j, j + 1 with j even form pairs that never cross a row.
+// (r < Hi, c < Wp, Wp even; columns c >= Wi are pad cells of class 0), so
+// the cells j, j + 1 with j even form pairs that never cross a row.
 // A pair class is the ordered pair (class of j, class of j + 1) in
 // first-seen order over the tubes (pair class 0 is two pad cells).  For tube
 // g (frame H[g] x W[g] at cell offset off[g] of the class bytes c8, the
@@ -146,10 +150,14 @@ constexpr int fast_wide_wp(int interior_width) { return (interior_width + 1 + 3)
 // as in the class table, and the pressure entry (aP of the first cell, aP of
 // the second) at [4 kFastPairs + 2p, 4 kFastPairs + 2p + 2).  Sets npairs to
 // the number of pair classes; when it exceeds kFastPairs, returns VS_OK with
-// p8 and ptab incomplete (the wide-access form is then not used).
-// VS_E_INVALID when a tube's frame is smaller than 3 x 3.
+// p8 and ptab incomplete (the wide-access form is then not used).  For each
+// tube, src_col[g] is the interior column of its cells whose x-face carries
+// the source (r = 0), -1 when it has none and -2 when they lie in more than
+// one column (the wide-access form then is not used either: it applies the
+// source term only in the lane-run group of that column).  VS_E_INVALID when
+// a tube's frame is smaller than 3 x 3.
 constexpr int kFastPairs = 128;
 int enumerate_fast_pairs(const uint8_t* c8, const float* tab, const int* H, const int* W, const long long* off,
-                         int groups, uint8_t* p8, float* ptab, int& npairs, std::string& err);
+                         int groups, uint8_t* p8, float* ptab, int& npairs, int* src_col, std::string& err);
 
 }  // namespace vs

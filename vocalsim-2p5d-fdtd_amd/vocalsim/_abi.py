@@ -27,7 +27,7 @@ COEFFS = ["N_out", "N_in", "minVxBeta", "minVyBeta", "maxVxSigmaPrimedt",
 
 (OPT_STEPS_PER_LAUNCH, OPT_THREADS, OPT_USE_GRAPH, OPT_KERNEL, OPT_TIMING, OPT_ENGINE_VARIANT,
  OPT_ADJOINT_KERNEL, OPT_ADJOINT_BLOCK, OPT_EXACT_GROUPS, OPT_EXACT_STEPS, OPT_BARRIER_SPIN,
- OPT_FAST_WIDE, OPT_FAST_LAYOUT_K, OPT_FAST_PAIR_CLASSES) = range(14)
+ OPT_FAST_WIDE, OPT_FAST_LAYOUT_K, OPT_FAST_PAIR_CLASSES, OPT_FAST_ROW_QUANTUM) = range(15)
 F_AS_F32 = 0x1000
 K_EXACT, K_FAST = 0, 1
 
