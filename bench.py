@@ -43,14 +43,14 @@ GRAPH_BLOCKS = 4   # 256-step launches per captured step graph (the library's de
 # WRITE_SIZE passes, each scaled by the same counter over a plain copy kernel
 # of known size (FETCH x2.000, WRITE x1.000): 102.4 MB read, 62.7 MB written;
 # profiles/k_fast_wide_batch256_fp32_pmc.md.
-PMC_TRAFFIC_BYTES = 165.06e6
+PMC_TRAFFIC_BYTES = 165.06e6   # k_fast_wide<20, 1>
 PMC_THREADS = 1024
 PMC_LAYOUT_K = 20
 # Fraction of the time each pipe of k_fast_wide<20> is busy on this batch, from
 # rocprofv3 SQ counters (profiles/k_fast_wide_batch256_fp32_sq.md): VALU =
 # SQ_ACTIVE_INST_VALU x 4 waves per SIMD / SQ_WAVE_CYCLES, LDS =
-# SQ_LDS_IDX_ACTIVE / cycles per CU.  The VALU is the busiest pipe.
-PIPES_BUSY = {"valu": 0.84, "lds": 0.68}
+# SQ_LDS_IDX_ACTIVE / cycles per CU.
+PIPES_BUSY = {"valu": 0.77, "lds": 0.69}
 
 
 def log(msg):
