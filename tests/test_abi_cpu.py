@@ -39,6 +39,8 @@ def test_null_arguments_rejected_without_device():
     assert L.vs_adjoint_forward(None, 512, None, 0) == -1
     assert L.vs_adjoint_backward(None, None, 0, None, None, None, 0) == -1
     assert L.vs_destroy(None) == 0
+    # nothing cached yet: releasing makes no device call
+    assert L.vs_release_cached_memory() == 0
 
 
 def test_device_trace_buffers_are_checked():

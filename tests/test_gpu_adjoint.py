@@ -22,6 +22,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
+import vocalsim
 from vocalsim import Solver, VocalSimError, _abi, tubes
 
 pytestmark = pytest.mark.gpu
@@ -327,3 +328,29 @@ def test_onchip_reverse_blocks_spacing_and_batching(hip_device):
     for x, y in ((AB[0], A1[0]), (SAB[0], S1[0]), (MAB[0], M1[0]), (AB[1], B1[0]), (SAB[1], SB1[0]),
                  (MAB[1], MB1[0])):
         assert np.array_equal(x, y)
+
+
+def test_onchip_slabs_kept_across_handles(hip_device):
+    """After vs_destroy the on-chip pass's large slabs stay with the library
+    and the next handle takes them: its probes and gradients equal those of
+    freshly allocated slabs bit for bit, also for a smaller batch that takes
+    larger kept slabs; vs_release_cached_memory returns them to the device."""
+    import torch
+    vocalsim.release_cached_memory()
+    specs = [base_tube(seed=s, num_steps=700) for s in (3, 4, 5, 6)]
+    gs = [loss_weights(sp.num_steps, seed=i) for i, sp in enumerate(specs)]
+    fresh = adjoint_grads(specs, gs, 32, 512)              # allocates, then leaves its slabs kept
+    again = adjoint_grads(specs, gs, 32, 512)              # takes the kept slabs
+    small_kept = adjoint_grads(specs[:2], gs[:2], 32, 512)
+    torch.cuda.synchronize()
+    free_kept = torch.cuda.mem_get_info()[0]
+    vocalsim.release_cached_memory()
+    free_released = torch.cuda.mem_get_info()[0]
+    print(f"[kept slabs] released {(free_released - free_kept) / 2**20:.0f} MiB", flush=True)
+    assert free_released - free_kept >= 64 << 20
+    small_fresh = adjoint_grads(specs[:2], gs[:2], 32, 512)
+    vocalsim.release_cached_memory()
+    for got, ref, n in ((again, fresh, 4), (small_kept, small_fresh, 2)):
+        for x, y in zip(got, ref):
+            for b in range(n):
+                assert np.array_equal(x[b], y[b])

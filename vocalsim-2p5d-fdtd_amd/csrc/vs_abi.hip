@@ -96,9 +96,128 @@ void source_table(const vs_tube_spec& s, const PhysTables& ph, int num_steps, st
 
 using namespace vs;
 
+// ---------------------------------------------------------------------------
+// Cache of large device memory across handles.  A handle takes its field
+// memory in chunks, and the on-chip reverse pass of a batch its record slabs
+// (64 tubes x 512 steps: about 14 GB); mapping and unmapping them costs
+// milliseconds (about 0.1 ms per allocation), paid on every pass when the
+// caller creates one handle per forward pass, as the autograd Function does.
+// Pieces of at least kSlabCacheMin bytes that a handle releases are kept per
+// device, up to kSlabCacheCap bytes over all devices (the oldest are freed
+// beyond it), and handed to the next request of between half their size and
+// their size.  Every user of a piece zeroes or writes it before reading it,
+// as with fresh hipMalloc memory.  The cache is never destroyed, so no
+// device call runs at process exit; vs_release_cached_memory frees what it
+// holds, and a failed allocation frees the device's cached pieces before its
+// one retry.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kSlabCacheMin = size_t(64) << 20;
+constexpr size_t kSlabCacheCap = size_t(64) << 30;
+constexpr size_t kArenaChunk = kSlabCacheMin;   // smallest chunk of a handle's field memory
+struct CachedSlab {
+    int dev;
+    size_t bytes;
+    void* p;
+};
+struct SlabCache {
+    std::mutex mu;
+    std::vector<CachedSlab> slabs;   // oldest first
+    size_t held = 0;
+};
+SlabCache& slab_cache() {
+    static SlabCache* c = new SlabCache;   // intentionally never destroyed
+    return *c;
+}
+void free_on(int dev, void* p) {
+    int cur = -1;
+    const bool other = hipGetDevice(&cur) == hipSuccess && cur != dev;
+    if (other) (void)hipSetDevice(dev);
+    (void)hipFree(p);
+    if (other) (void)hipSetDevice(cur);
+}
+size_t slab_cached_bytes(int dev) {
+    SlabCache& c = slab_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    size_t b = 0;
+    for (const CachedSlab& sl : c.slabs)
+        if (sl.dev == dev) b += sl.bytes;
+    return b;
+}
+// The smallest cached slab of device dev holding bytes, at most twice them.
+void* slab_take(int dev, size_t bytes, size_t* got) {
+    if (bytes < kSlabCacheMin) return nullptr;
+    SlabCache& c = slab_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    size_t best = c.slabs.size();
+    for (size_t i = 0; i < c.slabs.size(); ++i) {
+        const CachedSlab& sl = c.slabs[i];
+        if (sl.dev == dev && sl.bytes >= bytes && sl.bytes / 2 <= bytes &&
+            (best == c.slabs.size() || sl.bytes < c.slabs[best].bytes))
+            best = i;
+    }
+    if (best == c.slabs.size()) return nullptr;
+    void* p = c.slabs[best].p;
+    *got = c.slabs[best].bytes;
+    c.held -= *got;
+    c.slabs.erase(c.slabs.begin() + (long)best);
+    return p;
+}
+void slab_give(int dev, void* p, size_t bytes) {
+    if (!p) return;
+    if (bytes < kSlabCacheMin) {
+        free_on(dev, p);
+        return;
+    }
+    SlabCache& c = slab_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    c.slabs.push_back({dev, bytes, p});
+    c.held += bytes;
+    while (c.held > kSlabCacheCap) {
+        free_on(c.slabs.front().dev, c.slabs.front().p);
+        c.held -= c.slabs.front().bytes;
+        c.slabs.erase(c.slabs.begin());
+    }
+}
+// Frees the cached slabs of device dev (of every device when dev < 0).
+void slab_release(int dev) {
+    SlabCache& c = slab_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    std::vector<CachedSlab> keep;
+    for (const CachedSlab& sl : c.slabs) {
+        if (dev < 0 || sl.dev == dev) {
+            free_on(sl.dev, sl.p);
+            c.held -= sl.bytes;
+        } else {
+            keep.push_back(sl);
+        }
+    }
+    c.slabs.swap(keep);
+}
+// Device memory for a slab: a cached one, else hipMalloc, retried once after
+// the device's cached slabs are freed.
+hipError_t slab_alloc(int dev, size_t bytes, void** p, size_t* got) {
+    if ((*p = slab_take(dev, bytes, got))) return hipSuccess;
+    *got = bytes;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && slab_cached_bytes(dev) > 0) {
+        (void)hipGetLastError();
+        slab_release(dev);
+        e = hipMalloc(p, bytes);
+    }
+    return e;
+}
+}  // namespace
+
+struct vs_handle;
+namespace {
+void release_adj_fast_slabs(vs_handle* h);
+}  // namespace
+
 struct vs_handle {
     int n = 0;
     int precision = 64;
+    int device = 0;            // the GPU current at vs_create
     std::vector<TubeDesc> desc;
     std::vector<vs_dims> dims;
     int max_steps = 0;
@@ -111,8 +230,12 @@ struct vs_handle {
     int use_graph = 0;
     int kernel = VS_K_EXACT;
     int timing = 0;
-    // device memory
-    std::vector<void*> allocs;
+    // device memory: fields are 256-byte aligned pieces of chunks of at least
+    // kArenaChunk bytes (from the slab cache, returned to it by the
+    // destructor), so a handle maps a few chunks instead of one allocation
+    // per field
+    std::vector<std::pair<void*, size_t>> chunks;   // device pointer, bytes
+    size_t chunk_used = 0;                          // bytes handed out of chunks.back()
     TubeDesc* d_desc = nullptr;
     int* d_order = nullptr;
     void* bufs = nullptr;   // Bufs<float> or Bufs<double>
@@ -183,14 +306,17 @@ struct vs_handle {
     double* adj_strace = nullptr;
     double* adj_pexc = nullptr;
     TwoMassState* adj_tm = nullptr;
-    std::vector<void*> adj_fast_slabs;
+    std::vector<std::pair<void*, size_t>> adj_fast_slabs;   // device pointer, bytes
     double adj_reverse_ms = 0, adj_chain_ms = 0;   // last vs_adjoint_backward: device part, host chain
 
     ~vs_handle() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        for (void* p : allocs) (void)hipFree(p);
+        // the chunks and slabs go back to the cache once no stream can use them
+        for (hipStream_t st : {stream, adj_rec_stream, adj_rev_stream, adj_sum_stream})
+            if (st) (void)hipStreamSynchronize(st);
+        for (const auto& c : chunks) slab_give(device, c.first, c.second);
         for (void* p : adj_slabs) (void)hipFree(p);
-        for (void* p : adj_fast_slabs) (void)hipFree(p);
+        release_adj_fast_slabs(this);
         for (hipEvent_t e : adj_ev)
             if (e) (void)hipEventDestroy(e);
         if (adj_rec_stream) (void)hipStreamDestroy(adj_rec_stream);
@@ -200,18 +326,38 @@ struct vs_handle {
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
     }
+    // count zeroed elements of T (on the handle's stream)
     template <typename T>
     int alloc(T** p, size_t count) {
-        void* q = nullptr;
-        hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
-        if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
-        allocs.push_back(q);
+        const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~size_t(255);
+        if (chunks.empty() || chunks.back().second - chunk_used < bytes) {
+            void* c = nullptr;
+            size_t got = 0;
+            const hipError_t e = slab_alloc(device, std::max(bytes, kArenaChunk), &c, &got);
+            if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+            chunks.emplace_back(c, got);
+            chunk_used = 0;
+        }
+        void* q = static_cast<char*>(chunks.back().first) + chunk_used;
+        chunk_used += bytes;
         *p = static_cast<T*>(q);
-        e = hipMemsetAsync(q, 0, std::max<size_t>(count, 1) * sizeof(T), stream);
+        const hipError_t e = hipMemsetAsync(q, 0, bytes, stream);
         if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
         return VS_OK;
     }
 };
+
+namespace {
+// Hands the on-chip pass's slabs of h to the cache once the work of every
+// stream that may still use them has finished.
+void release_adj_fast_slabs(vs_handle* h) {
+    if (h->adj_fast_slabs.empty()) return;
+    for (hipStream_t st : {h->stream, h->adj_rec_stream, h->adj_rev_stream, h->adj_sum_stream})
+        if (st) (void)hipStreamSynchronize(st);
+    for (const auto& sl : h->adj_fast_slabs) slab_give(h->device, sl.first, sl.second);
+    h->adj_fast_slabs.clear();
+}
+}  // namespace
 
 namespace {
 
@@ -884,6 +1030,7 @@ int vs_create(const vs_tube_spec* specs, int n_tubes, int precision, vs_handle**
         {
             int dev = 0;
             VS_HIP(hipGetDevice(&dev));
+            h->device = dev;
             hipDeviceProp_t prop;
             VS_HIP(hipGetDeviceProperties(&prop, dev));
             h->lds_limit = (int)std::max(prop.sharedMemPerBlockOptin, prop.sharedMemPerBlock);
@@ -900,6 +1047,13 @@ int vs_destroy(vs_handle* h) {
         if (!h) return VS_OK;
         (void)hipStreamSynchronize(h->stream);
         delete h;
+        return VS_OK;
+    });
+}
+
+int vs_release_cached_memory(void) {
+    return guarded("vs_release_cached_memory", [&]() -> int {
+        slab_release(-1);
         return VS_OK;
     });
 }
@@ -1587,26 +1741,27 @@ int adj_fast_prepare(vs_handle* h, int S) {
     int rc = prepare_fast(h, kAdjThreads);
     if (rc != VS_OK) return rc;
     if (h->adj_fast_S == S && h->adj_K == K) return VS_OK;
-    VS_HIP(hipStreamSynchronize(h->stream));
-    for (void* p : h->adj_fast_slabs) (void)hipFree(p);
-    h->adj_fast_slabs.clear();
+    release_adj_fast_slabs(h);
     h->adj_fast_S = 0;
     h->adj_forward = false;
     const long long KT = (long long)K * kAdjLanes;
     const long long per_step = 2LL * h->n * 3 * KT * (long long)sizeof(float);   // two buffers of rec (2) + adj (1)
     size_t mem_free = 0, mem_total = 0;
     VS_HIP(hipMemGetInfo(&mem_free, &mem_total));
-    const long long budget = std::min<long long>(kAdjSlabCap, (long long)(mem_free / 2));
+    // slabs in the cache count as free: the requests below take them first
+    const long long budget =
+        std::min<long long>(kAdjSlabCap, (long long)((mem_free + slab_cached_bytes(h->device)) / 2));
     int B = (int)std::max(1LL, std::min<long long>(S, budget / per_step));
     if (h->adj_block > 0) B = std::min(B, h->adj_block);
     const long long C = h->total_cells;
     const long long nck = (h->max_steps + S - 1) / S;
     const long long nsub = B < S ? (S + B - 1) / B : 0;
     auto get = [&](void** p, long long bytes, const char* what) -> int {
-        hipError_t e = hipMalloc(p, (size_t)std::max(bytes, 1LL));
+        size_t got = 0;
+        hipError_t e = slab_alloc(h->device, (size_t)std::max(bytes, 1LL), p, &got);
         if (e != hipSuccess)
             return fail(VS_E_DEVICE, std::string(what) + " of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
-        h->adj_fast_slabs.push_back(*p);
+        h->adj_fast_slabs.emplace_back(*p, got);
         return VS_OK;
     };
     void *ck, *sub, *rec0, *rec1, *adj0, *adj1, *sa0, *sa1, *st, *sk, *tr, *pe, *tm;

@@ -37,7 +37,7 @@ EXPORTS = [
     "vs_get_source_trace", "vs_set_option", "vs_get_option", "vs_field_ptr", "vs_stream",
     "vs_last_run_timing", "vs_cells_per_step", "vs_dtft", "vs_probe_dtft", "vs_autotune",
     "vs_adjoint_forward", "vs_adjoint_backward", "vs_probe_audio", "vs_butter2_lowpass", "vs_write_wav",
-    "vs_copy_to_host", "vs_measure_lds_bandwidth", "vs_dtft_noalloc",
+    "vs_copy_to_host", "vs_measure_lds_bandwidth", "vs_dtft_noalloc", "vs_release_cached_memory",
     "vs_adjoint_timing",
 ]
 
@@ -120,6 +120,7 @@ def load():
     L.vs_measure_lds_bandwidth.argtypes = [c_int, _D]
     L.vs_dtft_noalloc.argtypes = [_D, c_int, ctypes.c_double, c_int, _D, c_int, _D, _D]
     L.vs_adjoint_timing.argtypes = [c_void, _D, _D]
+    L.vs_release_cached_memory.argtypes = []
     for name in EXPORTS:
         if name not in ("vs_last_error", "vs_abi_version"):
             getattr(L, name).restype = c_int
