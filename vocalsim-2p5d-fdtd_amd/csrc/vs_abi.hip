@@ -401,15 +401,40 @@ template <typename Real>
 int build(vs_handle* h, const vs_tube_spec* specs) {
     const bool fp64 = sizeof(Real) == 8;
     std::vector<Prelude> pre(h->n);
-    for (int k = 0; k < h->n; ++k) {
-        std::string err;
+    {
+        // the host preludes are independent per tube and run on up to 16
+        // host threads; single-plane tubes search their chords on the
+        // device, so a batch with one runs them in tube order on this thread
+        std::vector<int> rcs(h->n, VS_OK);
+        std::vector<std::string> errs(h->n);
         const hipStream_t st = h->stream;
         const MaxChordFn chord = [st](const std::vector<float>& y, const std::vector<float>& z, int sections, int npt,
                                       std::vector<float>& dist, std::vector<int>& far_idx, std::string& e) {
             return max_chord_on_device(st, y, z, sections, npt, dist, far_idx, e);
         };
-        int rc = prelude(specs[k], fp64, pre[k], chord, err);
-        if (rc != VS_OK) return fail(rc, "tube " + std::to_string(k) + ": " + err);
+        bool any_plane = false;
+        for (int k = 0; k < h->n; ++k) any_plane = any_plane || specs[k].cross_sectional_shape == 3;
+        std::atomic<int> next{0};
+        auto worker = [&] {
+            for (int k; (k = next.fetch_add(1)) < h->n;) {
+                try {
+                    rcs[k] = prelude(specs[k], fp64, pre[k], chord, errs[k]);
+                } catch (const std::exception& ex) {
+                    rcs[k] = VS_E_STATE;
+                    errs[k] = ex.what();
+                }
+            }
+        };
+        const int nthr = any_plane ? 1 : std::max(1, std::min({h->n, 16, (int)std::thread::hardware_concurrency()}));
+        if (nthr == 1) {
+            worker();
+        } else {
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nthr; ++t) pool.emplace_back(worker);
+            for (std::thread& t : pool) t.join();
+        }
+        for (int k = 0; k < h->n; ++k)
+            if (rcs[k] != VS_OK) return fail(rcs[k], "tube " + std::to_string(k) + ": " + errs[k]);
     }
     h->desc.resize(h->n);
     h->dims.resize(h->n);
@@ -564,12 +589,15 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
     long long maxHW = 0;
     for (auto& d : h->desc) maxHW = std::max(maxHW, (long long)d.H * d.W);
     Real* tmp = nullptr;
-    VS_HIP(hipMalloc(&tmp, sizeof(Real) * (size_t)maxHW * h->n * 4));
+    void* tmp_raw = nullptr;
+    size_t tmp_got = 0;
+    VS_HIP(slab_alloc(h->device, sizeof(Real) * (size_t)maxHW * h->n * 4, &tmp_raw, &tmp_got));
+    tmp = static_cast<Real*>(tmp_raw);
     const size_t slab = (size_t)maxHW * h->n;
     hipError_t e = launch_setup<Real>(h->d_desc, h->desc.data(), h->n, b, tmp, tmp + slab, tmp + 2 * slab,
                                       tmp + 3 * slab, st);
     hipError_t e2 = hipStreamSynchronize(st);
-    (void)hipFree(tmp);
+    slab_give(h->device, tmp_raw, tmp_got);
     if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("setup launch: ") + hipGetErrorString(e));
     if (e2 != hipSuccess) return fail(VS_E_DEVICE, std::string("setup: ") + hipGetErrorString(e2));
     unsigned status = 0;
