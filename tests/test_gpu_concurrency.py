@@ -2,12 +2,15 @@
 
 Each handle owns its stream, buffers and captured graphs, and the library's
 shared state (the autotuner's and the adjoint forward's shape caches, the
-per-kernel LDS limits, the error string) is locked, atomic or thread-local.
-So batches created and stepped concurrently must give the probes they give
-one after another, bit for bit.  The set covers graph replays of the on-chip
-FP32 kernel, the cooperative FP64 kernel (whose grid barrier has a bounded
-wait), a single-plane tube (its max-chord search runs on the handle's stream
-during creation) and a small FP64 batch on the per-step exact kernel.
+per-kernel LDS limits, the device memory kept for reuse after a handle is
+destroyed, the error string) is locked, atomic or thread-local.  So batches
+created and stepped concurrently must give the probes they give one after
+another, bit for bit.  The set covers graph replays of the on-chip FP32
+kernel, the cooperative FP64 kernel (whose grid barrier has a bounded wait),
+a single-plane tube (its max-chord search runs on the handle's stream during
+creation), a small FP64 batch on the per-step exact kernel and an FP32
+gradient pass on the on-chip reverse kernels (record slabs from the kept
+memory, taken and returned by every thread).
 """
 import threading
 
@@ -27,11 +30,21 @@ def jobs():
         "fp64 two-mass, cooperative": ([tubes.vowel_tube(0, num_steps=1200)], 64, _abi.K_EXACT, 1024, False),
         "fp32 single-plane": ([plane], 32, _abi.K_FAST, 1024, False),
         "fp64 batch": (tubes.batch_tubes(3, seed=4, num_steps=700), 64, _abi.K_EXACT, 256, False),
+        "fp32 gradient, on chip": ([tubes.vowel_tube(k, num_steps=700, source=tubes.SOURCE_GAUSSIAN)
+                                    for k in (2, 3)], 32, None, 1024, False),
     }
 
 
 def run(job):
     specs, precision, kernel, threads, graph = job
+    if kernel is None:
+        # probes and the three gradients of J = sum_n w[n] probe[n]
+        rng = np.random.default_rng(11)
+        w = [rng.standard_normal(sp.num_steps) for sp in specs]
+        with Solver(specs, precision=precision) as s:
+            probes = s.adjoint_forward(256)
+            grads = s.adjoint_backward(w)
+        return list(probes) + [g for gs in grads for g in gs]
     with Solver(specs, precision=precision, threads=threads, kernel=kernel) as s:
         if graph:
             s.set_option(_abi.OPT_USE_GRAPH, 1)
@@ -52,7 +65,7 @@ def test_handles_on_threads_match_sequential_runs(hip_device):
             errors.append((key, repr(e)))
 
     for round_ in range(3):
-        print(f"[concurrency] round {round_}: 8 threads, every job twice", flush=True)
+        print(f"[concurrency] round {round_}: {2 * len(todo)} threads, every job twice", flush=True)
         threads = [threading.Thread(target=worker, args=(name, job, (name, 2 * round_ + r)))
                    for r in range(2) for name, job in todo.items()]
         for t in threads:
