@@ -330,11 +330,12 @@ int vs_adjoint_timing(const vs_handle* h, double* reverse_ms, double* chain_ms);
  * counterpart in the reference: for callers that create and destroy handles
  * in a loop, such as an optimisation loop that builds a handle per forward
  * pass of the gradient).  A handle takes its field memory in chunks of
- * 64 MiB or more, and the on-chip FP32 reverse pass its record slabs
- * (several GB for a large batch).  When the handle is destroyed, every such
- * piece of 64 MiB or more is kept, per device and up to 64 GiB in all (the
- * oldest are freed beyond that), and a later request of between half a kept
- * piece's size and its size takes it instead of allocating.  Kept memory
+ * 4 to 64 MiB (sized from its batch), and the on-chip FP32 reverse pass its
+ * record slabs (several GB for a large batch).  When the handle is
+ * destroyed, every such piece of 4 MiB or more is kept, per device and up to
+ * 64 GiB in all (the oldest are freed beyond that), and a later request of
+ * between half a kept piece's size and its size takes it instead of
+ * allocating.  Kept memory
  * counts as free when a handle sizes its record slabs, and a failed
  * allocation frees the device's kept memory before it is retried once.
  * Call this when the work is done and the memory is wanted elsewhere. */

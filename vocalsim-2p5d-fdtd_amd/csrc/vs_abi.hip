@@ -112,9 +112,10 @@ using namespace vs;
 // one retry.
 // ---------------------------------------------------------------------------
 namespace {
-constexpr size_t kSlabCacheMin = size_t(64) << 20;
+constexpr size_t kSlabCacheMin = size_t(4) << 20;
 constexpr size_t kSlabCacheCap = size_t(64) << 30;
-constexpr size_t kArenaChunk = kSlabCacheMin;   // smallest chunk of a handle's field memory
+// a handle's field chunks: its estimated set-up field memory, within these bounds
+constexpr size_t kArenaChunkMin = kSlabCacheMin, kArenaChunkMax = size_t(64) << 20;
 struct CachedSlab {
     int dev;
     size_t bytes;
@@ -230,12 +231,13 @@ struct vs_handle {
     int use_graph = 0;
     int kernel = VS_K_EXACT;
     int timing = 0;
-    // device memory: fields are 256-byte aligned pieces of chunks of at least
-    // kArenaChunk bytes (from the slab cache, returned to it by the
+    // device memory: fields are 256-byte aligned pieces of chunks of
+    // chunk_bytes or more (from the slab cache, returned to it by the
     // destructor), so a handle maps a few chunks instead of one allocation
     // per field
     std::vector<std::pair<void*, size_t>> chunks;   // device pointer, bytes
     size_t chunk_used = 0;                          // bytes handed out of chunks.back()
+    size_t chunk_bytes = kArenaChunkMax;            // size of a new chunk (set by the build from the batch)
     TubeDesc* d_desc = nullptr;
     int* d_order = nullptr;
     void* bufs = nullptr;   // Bufs<float> or Bufs<double>
@@ -333,7 +335,7 @@ struct vs_handle {
         if (chunks.empty() || chunks.back().second - chunk_used < bytes) {
             void* c = nullptr;
             size_t got = 0;
-            const hipError_t e = slab_alloc(device, std::max(bytes, kArenaChunk), &c, &got);
+            const hipError_t e = slab_alloc(device, std::max(bytes, chunk_bytes), &c, &got);
             if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
             chunks.emplace_back(c, got);
             chunk_used = 0;
@@ -479,6 +481,15 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
         dm.midY = p.midY; dm.num_sections = nsec; dm.mic_cells = p.mic;
     }
     h->total_cells = cell; h->total_cols = col; h->total_walls = wall; h->total_secs = sec; h->total_src = src;
+    {
+        // chunk size: about the set-up fields below (P, V, depths, classes,
+        // columns, walls, sections, per-step source, probe and trace), so a
+        // one-tube handle takes a few MB and a large batch 64 MiB chunks
+        const size_t R = sizeof(Real);
+        const size_t est = (size_t)cell * (6 * R + 10) + (size_t)col * R + (size_t)wall * 12 +
+                           (size_t)sec * (8 + 3 * R) + (size_t)src * (R + 16);
+        h->chunk_bytes = std::min(kArenaChunkMax, std::max(kArenaChunkMin, est));
+    }
     Bufs<Real>& b = bufs_of<Real>(h);
     int rc;
 #define ALLOC(ptr, n) if ((rc = h->alloc(&(ptr), (n))) != VS_OK) return rc
