@@ -338,7 +338,9 @@ int vs_adjoint_timing(const vs_handle* h, double* reverse_ms, double* chain_ms);
  * allocating.  Kept memory
  * counts as free when a handle sizes its record slabs, and a failed
  * allocation frees the device's kept memory before it is retried once.
- * Call this when the work is done and the memory is wanted elsewhere. */
+ * The pinned host blocks vs_adjoint_backward copies its per-cell results
+ * into (tens of MB, up to 4 GiB kept) are freed here too.  Call this when
+ * the work is done and the memory is wanted elsewhere. */
 int vs_release_cached_memory(void);
 
 #ifdef __cplusplus

@@ -195,6 +195,67 @@ void slab_release(int dev) {
     }
     c.slabs.swap(keep);
 }
+// Pinned host blocks for the gradient chain's copies back, kept across
+// calls (pinning tens of MB costs milliseconds per call): up to
+// kPinnedCacheCap bytes, the oldest freed beyond it.  Like the device cache,
+// never destroyed; vs_release_cached_memory frees them.
+constexpr size_t kPinnedCacheCap = size_t(4) << 30;
+struct PinnedCache {
+    std::mutex mu;
+    std::vector<std::pair<void*, size_t>> blocks;   // oldest first
+    size_t held = 0;
+};
+PinnedCache& pinned_cache() {
+    static PinnedCache* c = new PinnedCache;   // intentionally never destroyed
+    return *c;
+}
+// A kept block of bytes to four times bytes (the smallest), else a new one.
+hipError_t pinned_take(size_t bytes, void** p, size_t* got) {
+    {
+        PinnedCache& c = pinned_cache();
+        std::lock_guard<std::mutex> g(c.mu);
+        size_t best = c.blocks.size();
+        for (size_t i = 0; i < c.blocks.size(); ++i)
+            if (c.blocks[i].second >= bytes && c.blocks[i].second / 4 <= bytes &&
+                (best == c.blocks.size() || c.blocks[i].second < c.blocks[best].second))
+                best = i;
+        if (best < c.blocks.size()) {
+            *p = c.blocks[best].first;
+            *got = c.blocks[best].second;
+            c.held -= *got;
+            c.blocks.erase(c.blocks.begin() + (long)best);
+            return hipSuccess;
+        }
+    }
+    *got = bytes;
+    return hipHostMalloc(p, bytes, hipHostMallocDefault);
+}
+void pinned_give(void* p, size_t bytes) {
+    if (!p) return;
+    PinnedCache& c = pinned_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    c.blocks.emplace_back(p, bytes);
+    c.held += bytes;
+    while (c.held > kPinnedCacheCap) {
+        (void)hipHostFree(c.blocks.front().first);
+        c.held -= c.blocks.front().second;
+        c.blocks.erase(c.blocks.begin());
+    }
+}
+void pinned_release() {
+    PinnedCache& c = pinned_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    for (const auto& b : c.blocks) (void)hipHostFree(b.first);
+    c.blocks.clear();
+    c.held = 0;
+}
+// Returns a pinned block to the cache when the scope ends.
+struct PinnedBlock {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~PinnedBlock() { pinned_give(p, bytes); }
+};
+
 // Device memory for a slab: a cached one, else hipMalloc, retried once after
 // the device's cached slabs are freed.
 hipError_t slab_alloc(int dev, size_t bytes, void** p, size_t* got) {
@@ -1093,6 +1154,7 @@ int vs_destroy(vs_handle* h) {
 int vs_release_cached_memory(void) {
     return guarded("vs_release_cached_memory", [&]() -> int {
         slab_release(-1);
+        pinned_release();
         return VS_OK;
     });
 }
@@ -2110,13 +2172,6 @@ int adj_forward(vs_handle* h, int S, double* probe, size_t len) {
     return adj_probe_out(h, a.probe, probe);
 }
 
-template <typename T>
-int fetch(hipStream_t st, std::vector<T>& dst, const T* src, long long n) {
-    dst.resize((size_t)n);
-    VS_HIP(copy_on(st, dst.data(), src, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost));
-    return VS_OK;
-}
-
 template <typename Real>
 int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area, double* g_scale, double* g_mu,
                  size_t nsec) {
@@ -2153,48 +2208,97 @@ int adj_backward(vs_handle* h, const double* gprobe, size_t glen, double* g_area
     // raw depth maps for the chain through smoothing and clamping
     long long maxHW = 0;
     for (auto& d : h->desc) maxHW = std::max(maxHW, (long long)d.H * d.W);
-    Real* raw = nullptr;
-    VS_HIP(hipMalloc(&raw, sizeof(Real) * (size_t)maxHW * h->n * 2));
+    void* raw_v = nullptr;
+    size_t raw_got = 0;
+    VS_HIP(slab_alloc(h->device, sizeof(Real) * (size_t)maxHW * h->n * 2, &raw_v, &raw_got));
+    Real* raw = static_cast<Real*>(raw_v);
     hipError_t e = launch_depth_raw<Real>(h->d_desc, h->desc.data(), h->n, b, raw, raw + maxHW * h->n, maxHW, h->stream);
     hipError_t e2 = hipStreamSynchronize(h->stream);
-    std::vector<Real> rawh;
-    if (e == hipSuccess && e2 == hipSuccess) rc = fetch(h->stream, rawh, raw, maxHW * h->n * 2);
-    (void)hipFree(raw);
+    if (e != hipSuccess || e2 != hipSuccess) slab_give(h->device, raw_v, raw_got);
     if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("raw depth: ") + hipGetErrorString(e));
     if (e2 != hipSuccess) return fail(VS_E_DEVICE, std::string("reverse pass: ") + hipGetErrorString(e2));
-    if (rc != VS_OK) return rc;
     const auto t_device = std::chrono::steady_clock::now();
-    // Lp, Lx, Ly: d J / d log of each cell's kP, x-face and y-face coefficient
-    std::vector<double> Lp, Lx, Ly;
-    std::vector<Real> dP, dX, dY, zcol, minor, dscale;
-    std::vector<uint8_t> cls;
-    std::vector<int> seg, upper, lower;
+    // everything the host chain reads, copied back into one pinned block:
+    // Lp, Lx, Ly (d J / d log of each cell's kP, x-face and y-face
+    // coefficient), the raw depth maps and the set-up fields it chains through
     const long long C = h->total_cells;
-    if ((rc = fetch(h->stream, Lp, a.GkP, C)) || (rc = fetch(h->stream, Lx, a.Gx, C)) || (rc = fetch(h->stream, Ly, a.Gy, C))) return rc;
-    if (!h->adj_onchip) {
-        // the per-step kernels accumulate d J / d (kP, cx, cy)
-        std::vector<Real> kP, cx, cy;
-        if ((rc = fetch(h->stream, kP, a.kP, C)) || (rc = fetch(h->stream, cx, a.cx, C)) || (rc = fetch(h->stream, cy, a.cy, C))) return rc;
+    size_t total = 0;
+    auto place = [&](size_t bytes) {
+        const size_t o = total;
+        total += (bytes + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t R = sizeof(Real);
+    const size_t oLp = place(8 * C), oLx = place(8 * C), oLy = place(8 * C);
+    const size_t oRaw = place(R * (size_t)maxHW * h->n * 2);
+    const size_t odP = place(R * C), odX = place(R * C), odY = place(R * C), ocls = place((size_t)C);
+    const size_t ozc = place(R * h->total_cols), oseg = place(4 * (size_t)h->total_walls),
+                 oup = place(4 * (size_t)h->total_walls), olo = place(4 * (size_t)h->total_walls),
+                 omin = place(R * h->total_secs), odsc = place(R * h->total_secs);
+    const bool coef = !h->adj_onchip;   // the per-step kernels accumulate d J / d (kP, cx, cy)
+    const size_t okP = coef ? place(R * C) : 0, ocx = coef ? place(R * C) : 0, ocy = coef ? place(R * C) : 0;
+    PinnedBlock pin;
+    e = pinned_take(total, &pin.p, &pin.bytes);
+    if (e != hipSuccess) {
+        pin.p = nullptr;
+        slab_give(h->device, raw_v, raw_got);
+        return fail(VS_E_DEVICE, std::string("pinned staging: ") + hipGetErrorString(e));
+    }
+    char* const P = static_cast<char*>(pin.p);
+    struct Copy {
+        size_t off;
+        const void* src;
+        size_t bytes;
+    };
+    std::vector<Copy> copies = {{oLp, a.GkP, 8 * (size_t)C}, {oLx, a.Gx, 8 * (size_t)C}, {oLy, a.Gy, 8 * (size_t)C},
+                                {oRaw, raw, R * (size_t)maxHW * h->n * 2},
+                                {odP, b.dP, R * C}, {odX, b.dX, R * C}, {odY, b.dY, R * C}, {ocls, b.cls, (size_t)C},
+                                {ozc, b.zcol, R * h->total_cols}, {oseg, b.seg, 4 * (size_t)h->total_walls},
+                                {oup, b.upper, 4 * (size_t)h->total_walls}, {olo, b.lower, 4 * (size_t)h->total_walls},
+                                {omin, b.minor, R * h->total_secs}, {odsc, b.dscale, R * h->total_secs}};
+    if (coef) {
+        copies.push_back({okP, a.kP, R * C});
+        copies.push_back({ocx, a.cx, R * C});
+        copies.push_back({ocy, a.cy, R * C});
+    }
+    for (const Copy& c : copies)
+        if (e == hipSuccess && c.bytes) e = hipMemcpyAsync(P + c.off, c.src, c.bytes, hipMemcpyDeviceToHost, h->stream);
+    e2 = hipStreamSynchronize(h->stream);
+    slab_give(h->device, raw_v, raw_got);
+    if (e != hipSuccess) return fail(VS_E_DEVICE, std::string("copy back: ") + hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(VS_E_DEVICE, std::string("copy back: ") + hipGetErrorString(e2));
+    double* const Lp = reinterpret_cast<double*>(P + oLp);
+    double* const Lx = reinterpret_cast<double*>(P + oLx);
+    double* const Ly = reinterpret_cast<double*>(P + oLy);
+    const Real* const rawh = reinterpret_cast<const Real*>(P + oRaw);
+    const Real* const dP = reinterpret_cast<const Real*>(P + odP);
+    const Real* const dX = reinterpret_cast<const Real*>(P + odX);
+    const Real* const dY = reinterpret_cast<const Real*>(P + odY);
+    const uint8_t* const cls = reinterpret_cast<const uint8_t*>(P + ocls);
+    const Real* const zcol = reinterpret_cast<const Real*>(P + ozc);
+    const int* const seg = reinterpret_cast<const int*>(P + oseg);
+    const int* const upper = reinterpret_cast<const int*>(P + oup);
+    const int* const lower = reinterpret_cast<const int*>(P + olo);
+    const Real* const minor = reinterpret_cast<const Real*>(P + omin);
+    const Real* const dscale = reinterpret_cast<const Real*>(P + odsc);
+    if (coef) {
+        const Real* const kP = reinterpret_cast<const Real*>(P + okP);
+        const Real* const cx = reinterpret_cast<const Real*>(P + ocx);
+        const Real* const cy = reinterpret_cast<const Real*>(P + ocy);
         for (long long q = 0; q < C; ++q) {
             Lp[q] *= (double)kP[q];
             Lx[q] *= (double)cx[q];
             Ly[q] *= (double)cy[q];
         }
     }
-    if ((rc = fetch(h->stream, dP, b.dP, C)) || (rc = fetch(h->stream, dX, b.dX, C)) || (rc = fetch(h->stream, dY, b.dY, C)) ||
-        (rc = fetch(h->stream, cls, b.cls, C)) || (rc = fetch(h->stream, zcol, b.zcol, h->total_cols)) ||
-        (rc = fetch(h->stream, seg, b.seg, h->total_walls)) || (rc = fetch(h->stream, upper, b.upper, h->total_walls)) ||
-        (rc = fetch(h->stream, lower, b.lower, h->total_walls)) || (rc = fetch(h->stream, minor, b.minor, h->total_secs)) ||
-        (rc = fetch(h->stream, dscale, b.dscale, h->total_secs)))
-        return rc;
     // tubes are independent: each reads the shared host copies and writes
     // only its own sections, so they are chained on up to 16 host threads
     auto chain_tube = [&](int k) {
         const TubeDesc& d = h->desc[k];
         const int H = d.H, W = d.W, HW = H * W;
         const long long base = d.cell_off;
-        const Real* X = rawh.data() + (size_t)k * maxHW;
-        const Real* Y = rawh.data() + (size_t)(maxHW * h->n) + (size_t)k * maxHW;
+        const Real* X = rawh + (size_t)k * maxHW;
+        const Real* Y = rawh + (size_t)(maxHW * h->n) + (size_t)k * maxHW;
         const int mid = d.midY * W + d.te;
         const Real open = (Real)0.50f, mnR = (Real)0.001f;
         const bool depth3d = d.solver_type == 1;
