@@ -485,6 +485,9 @@ int build(vs_handle* h, const vs_tube_spec* specs) {
                 } catch (const std::exception& ex) {
                     rcs[k] = VS_E_STATE;
                     errs[k] = ex.what();
+                } catch (...) {
+                    rcs[k] = VS_E_STATE;
+                    errs[k] = "unexpected exception";
                 }
             }
         };
