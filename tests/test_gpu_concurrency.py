@@ -78,3 +78,23 @@ def test_handles_on_threads_match_sequential_runs(hip_device):
         assert len(probes) == len(ref[name])
         for i, (a, b) in enumerate(zip(probes, ref[name])):
             assert np.array_equal(a, b), f"{name} (run {r}), tube {i}: probe differs from the sequential run"
+
+
+def test_small_handles_take_small_chunks(hip_device):
+    """A handle takes its field memory in chunks sized from its batch (4 MiB
+    for one tube), so many live one-tube handles hold a few MB each; the
+    memory stays with the library after they close until it is released."""
+    import torch
+    import vocalsim
+    vocalsim.release_cached_memory()
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    spec = tubes.vowel_tube(1, num_steps=2000)
+    handles = [Solver([spec], precision=64) for _ in range(16)]
+    torch.cuda.synchronize()
+    held = free0 - torch.cuda.mem_get_info()[0]
+    print(f"[small handles] 16 one-tube FP64 handles hold {held / 2**20:.0f} MiB", flush=True)
+    for h in handles:
+        h.close()
+    vocalsim.release_cached_memory()
+    assert held <= 16 * (32 << 20)
